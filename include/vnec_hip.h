@@ -1,0 +1,102 @@
+/*
+ * vnec_hip.h — the thin C-ABI between the C host code of libvnec_mi355x.so
+ * and its hand-written HIP kernels for gfx950.  Plain pointers (device
+ * addresses unless named *_host) and sizes only; every call enqueues on the
+ * given stream and returns 0 on success or a hipError_t code.
+ *
+ * Internal to the library (no reference counterpart); declared here so the
+ * boundary between host logic and kernels is explicit and testable.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Merge-path coordinates of diagonals min(t * ceil(M / P), M), t = 0..P,
+ * written as (x, y) int pairs to coords (2 (P + 1) ints).  snap != 0
+ * applies the S-variant row snap.  ptr: nrows + 1 int32 row offsets. */
+int vnec_hip_partition(const int32_t *ptr, int64_t nrows, int64_t nnz, int64_t P, int snap,
+                       int32_t *coords, void *stream);
+
+/* Exclusive sum scan of n int32 values (out may alias in); *total_host
+ * (if not NULL) receives the sum after a stream synchronisation.  Allocates
+ * its own scratch (builder use only). */
+int vnec_hip_scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, int64_t *total_host,
+                           void *stream);
+
+/* ECR by per-tile LDS sort (tiles of at most VNEC_ECR_TILE_MAX nonzeros).
+ * Pass 1 writes the tile-local rank of each nonzero (lcol16 and/or ecr32,
+ * either may be NULL), the sorted distinct columns of tile t at
+ * uniq[y_t .. y_t + nec[t]), nec[t] and the number of runs of consecutive
+ * columns nruns[t].  Pass 2 writes each tile's runs (first column, window
+ * position) at run_off[t]... */
+#define VNEC_ECR_TILE_MAX 4096
+int vnec_hip_ecr_tiles_pass1(const int32_t *col, const int32_t *coords, int64_t ntiles,
+                             uint16_t *lcol16, int32_t *ecr32, int32_t *uniq, int32_t *nec,
+                             int32_t *nruns, void *stream);
+int vnec_hip_ecr_tiles_pass2(const int32_t *coords, int64_t ntiles, const int32_t *uniq,
+                             const int32_t *nec, const int32_t *run_off, int32_t *run_start,
+                             int32_t *run_wpos, void *stream);
+/* Largest nonzero count of any tile (device reduction, result to host). */
+int vnec_hip_max_tile_nnz(const int32_t *coords, int64_t ntiles, int32_t *max_host, void *stream);
+
+/* ECR by dense flags over P x ncols (few large partitions).  flags and rank
+ * are caller scratch of P * ncols int32 each.  Writes ecr (nnz), win
+ * (global ranks: win_off[t] = rank[t * ncols]) and win_off (P + 1). */
+int vnec_hip_ecr_dense(const int32_t *col, int64_t nnz, int64_t ncols, const int32_t *coords,
+                       int64_t P, int32_t *flags, int32_t *rank, int32_t *ecr, int32_t *win,
+                       int64_t *win_off, void *stream);
+
+/* VNEC-D block layout at (P, L) from ECR ranks ecr (nnz): pass 1 writes
+ * each row's group count to cnt (nrows + 1 entries, cnt[nrows] = 0; its
+ * exclusive scan is v_row_ptr); pass 2 writes col_start (G) and, if
+ * val_align is not NULL, val_align (G * L, caller-zeroed). */
+int vnec_hip_d_count(const int32_t *ptr, const int32_t *ecr, int64_t nrows, const int32_t *coords,
+                     int64_t P, int L, int32_t *cnt, void *stream);
+int vnec_hip_d_fill(const int32_t *ptr, const int32_t *ecr, const double *val, int64_t nrows,
+                    const int32_t *coords, int64_t P, int L, const int32_t *v_row_ptr,
+                    int32_t *col_start, double *val_align, void *stream);
+
+/* FP64 -> FP32 hi + FP32 lo split: hi = fl32(v), lo = fl32(v - hi). */
+int vnec_hip_split_hilo(const double *val, int64_t n, float *hi, float *lo, void *stream);
+
+/* Merge-path SpMV arguments (variant L or E; see vnec_ext.h). */
+typedef struct {
+    const int32_t *ptr;       /* nrows + 1                         */
+    const int32_t *col;       /* nnz, global column (variant L)    */
+    const uint16_t *lcol;     /* nnz, tile-local rank (variant E)  */
+    const double *val;        /* nnz, encoding FP64                */
+    const float *hi, *lo;     /* nnz, encodings HILO / HI          */
+    const int32_t *coords;    /* 2 (ntiles + 1)                    */
+    int64_t ntiles;
+    const int32_t *run_off;   /* ntiles + 1 (variant E)            */
+    const int32_t *run_start; /* runs (variant E)                  */
+    const int32_t *run_wpos;  /* runs (variant E)                  */
+    const int32_t *nec;       /* ntiles (variant E)                */
+    const double *x;          /* ncols * nrhs                      */
+    double *y;                /* nrows * nrhs                      */
+    double *carry;            /* ntiles * nrhs scratch             */
+    int64_t nrows;
+    int ipt;
+    int variant;
+    int encoding;
+    int nrhs;
+} vnec_merge_args;
+
+int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
+/* Adds each tile's carry to y[row] in tile order (one add per row; rows
+ * >= nrows, the carries of empty trailing tiles, are skipped). */
+int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, int64_t ntiles, double *y,
+                         int64_t nrows, int nrhs, void *stream);
+/* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
+int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
+
+/* Overwrites n bytes (cache flush helper for timing). */
+int vnec_hip_scribble(void *p, size_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,113 @@
+"""CPU-side checks of the C-ABI library (no GPU needed): it loads, exports
+every function include/*.h declares, and its host object logic matches the
+reference's (struct layout prefix, CSR build order, error returns)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from vnec_amd import _lib as L
+from vnec_amd import spb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("OpenSparseBLAS.h", "vnec_ext.h", "vnec_hip.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        txt = re.sub(r"typedef[^;]*;", "", txt, flags=re.S)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\([^;{]*\)\s*;", txt):
+            names.add(m.group(1))
+    names.discard("if")
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = L.load()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers them all
+    assert not [n for n in names if n not in L.SIGNATURES]
+
+
+class RefMatrixLayout(C.Structure):
+    """src/core/include/SpB_Matrix.h:10-17 + SpB_Matrix_CSC_or_CSR.h:12-21."""
+    _fields_ = [("element_type", C.c_int), ("format_type", C.c_int), ("row", C.c_uint64),
+                ("col", C.c_uint64), ("element_size", C.c_size_t), ("val", C.c_void_p),
+                ("indices", C.POINTER(C.c_uint64)), ("ptr", C.POINTER(C.c_uint64)),
+                ("ptr_len", C.c_uint64), ("nnz", C.c_uint64), ("capacity", C.c_uint64)]
+
+
+class RefVectorLayout(C.Structure):
+    """src/core/include/SpB_Vector.h:10-15 + SpB_Vector_Dense.h:15-19."""
+    _fields_ = [("element_type", C.c_int), ("format_type", C.c_int), ("n", C.c_uint64),
+                ("element_size", C.c_size_t), ("values", C.POINTER(C.c_double))]
+
+
+def _view(h, cls):
+    return C.cast(h, C.POINTER(cls)).contents
+
+
+def test_matrix_build_matches_reference_layout_and_order():
+    rng = np.random.default_rng(0)
+    n, m, z = 50, 40, 300
+    r = rng.integers(0, n, z)
+    c = rng.integers(0, m, z)
+    v = rng.uniform(-1, 1, z)
+    A = spb.Matrix.from_coo(n, m, r, c, v)
+    s = _view(A.h, RefMatrixLayout)
+    assert (s.element_type, s.format_type, s.row, s.col, s.element_size) == (11, 1, n, m, 8)
+    assert s.nnz == z and s.ptr_len == n + 1
+    ptr = np.ctypeslib.as_array(s.ptr, shape=(n + 1,))
+    ind = np.ctypeslib.as_array(s.indices, shape=(z,))
+    val = np.ctypeslib.as_array(C.cast(s.val, C.POINTER(C.c_double)), shape=(z,))
+    # sorted by (row, col), duplicates kept in input order (a stable sort)
+    order = np.lexsort((np.arange(z), c, r))
+    assert np.array_equal(ind, c[order]) and np.array_equal(val, v[order])
+    expect_ptr = np.concatenate([[0], np.cumsum(np.bincount(r, minlength=n))])
+    assert np.array_equal(ptr, expect_ptr)  # trailing empty rows filled too
+
+
+def test_vector_layout_and_elements():
+    x = spb.Vector(10)
+    s = _view(x.h, RefVectorLayout)
+    assert (s.element_type, s.format_type, s.n, s.element_size) == (11, 0, 10, 8)
+    vals = np.ctypeslib.as_array(s.values, shape=(10,))
+    assert np.all(vals == 0)
+    x.set(3, 2.5)
+    assert x.get(3) == 2.5 and vals[3] == 2.5
+    with pytest.raises(RuntimeError):
+        x.set(10, 1.0)
+
+
+def test_error_paths_return_failure_not_exit():
+    lib = spb.init()
+    h = C.c_void_p()
+    assert lib.SpB_Matrix_new(C.byref(h), 10, 3, 3) == 1  # FP32 is not on the path
+    assert "SpB_FP64" in L.last_error()
+    A = spb.Matrix(4, 4)
+    bad_r = np.array([0, 9], np.uint64)
+    cc = np.array([0, 1], np.uint64)
+    vv = np.array([1.0, 2.0])
+    assert lib.SpB_Matrix_build_FP64(A.h, bad_r.ctypes.data, cc.ctypes.data, vv.ctypes.data, 2,
+                                     0) == 1
+    assert "out of range" in L.last_error()
+    assert lib.SpB_Matrix_build_FP64(A.h, cc.ctypes.data, cc.ctypes.data, vv.ctypes.data, 2,
+                                     5) == 1
+    x, y = spb.Vector(5), spb.Vector(4)
+    assert lib.SpB_mxv(y.h, A.h, x.h, None) == 1  # dimension mismatch
+    assert "dimension" in L.last_error()
+
+
+def test_plan_without_device_fails_loudly():
+    if spb.device_count() > 0:
+        pytest.skip("a GPU is present")
+    A = spb.Matrix.from_csr(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        spb.Plan(A)

@@ -1,0 +1,271 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle and
+the reference's golden fixtures.
+
+Integer structures (merge-path coordinates, ECR maps, D layouts, IRD
+decision) must be bit-exact; FP64 results must satisfy, per row,
+|y - y_ref| <= 1e-12 * (|A||x|)_i (FP32 hi+lo too); FP32 hi-only is
+bounded by 2^-23 * (|A||x|)_i.  Integer-valued known answers must be exact.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden_names, load_golden
+from vnec_amd import matgen as g
+from vnec_amd import spb
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+HI_TOL = 2.0 ** -23
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if spb.device_count() < 1:
+        pytest.fail("no GPU visible: the gpu tests need an MI355X")
+    spb.init()
+
+
+def _mat(m):
+    return spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+
+
+def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1):
+    p = spb.Plan(A, variant, ipt, encoding=enc, nrhs=nrhs)
+    xv = spb.Vector.from_array(x)
+    yv = spb.Vector(A.nrows * nrhs)
+    p.spmv(yv, xv)
+    y = yv.to_array()
+    p.free()
+    return y
+
+
+def _cases():
+    out = [("rmat12", g.rmat(12, 16, seed=11)),
+           ("stencil16", g.stencil27(16, seed=12)),
+           ("powerlaw", g.powerlaw(6000, 1.7, 2, 5000, seed=13)),
+           ("banded", g.banded(5000, 40, seed=14)),
+           ("blockdiag", g.blockdiag(4000, 64, seed=15))]
+    # one dense row spanning many tiles, empty rows around it
+    n = 3000
+    lens = np.zeros(n, np.int64)
+    lens[::3] = 5
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(np.random.default_rng(i).choice(n, 5, replace=False))
+                          for i in range(n // 3)]).astype(np.int32)
+    dense_row = 1500
+    ptr2 = ptr.copy()
+    ptr2[dense_row + 1:] += n
+    col2 = np.concatenate([col[:ptr[dense_row]], np.arange(n, dtype=np.int32),
+                           col[ptr[dense_row]:]])
+    out.append(("denserow", g.from_pattern(n, n, ptr2, col2, seed=16)))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 15), ("E", 5), ("E", 11)])
+def test_spmv_fp64_matches_oracle(name, m, variant, ipt):
+    A = _mat(m)
+    print(f"case {name} variant {variant} ipt {ipt}", flush=True)
+    y = _run_plan(A, m.x, variant, ipt)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y0, ax, TOL)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("variant", ["L", "E"])
+@pytest.mark.parametrize("enc", ["hilo", "hi"])
+def test_encodings_error_bounds(variant, enc):
+    m = g.stencil27(20, seed=21)
+    A = _mat(m)
+    y = _run_plan(A, m.x, variant, 7, enc)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y0, ax, TOL if enc == "hilo" else HI_TOL)
+    assert ok, worst
+    if enc == "hi":
+        assert worst > 1e-12  # hi-only really drops the low bits
+
+
+@pytest.mark.parametrize("k", [2, 4, 8, 16])
+def test_multi_rhs(k):
+    m = g.rmat(11, 16, seed=31)
+    rng = np.random.default_rng(k)
+    X = rng.uniform(-1, 1, m.ncols * k)
+    A = _mat(m)
+    Y = _run_plan(A, X, "L", 5, nrhs=k)
+    Y0 = O.spmm_serial(m.ptr, m.col, m.val, X, k)
+    ax = np.empty(m.nrows * k)
+    for c in range(k):
+        ax[c::k] = O.spmv_abs(m.ptr, m.col, m.val, X[c::k])[0]
+    assert O.check_rows(Y, Y0, ax, TOL)[0]
+
+
+def test_known_answer_is_exact():
+    m = g.rmat(13, 8, seed=41)
+    val = np.repeat(np.arange(m.nrows, dtype=np.float64), np.diff(m.ptr))
+    x = np.arange(m.ncols, dtype=np.float64)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
+    rows = np.repeat(np.arange(m.nrows), np.diff(m.ptr))
+    expect = np.bincount(rows, weights=rows * m.col.astype(np.float64), minlength=m.nrows)
+    for variant in ("L", "E"):
+        assert np.array_equal(_run_plan(A, x, variant, 7), expect)
+
+
+def test_empty_rows_and_empty_matrix():
+    A = spb.Matrix.from_csr(5, 4, [0, 0, 0, 0, 0, 0], [], [])
+    assert np.array_equal(_run_plan(A, np.ones(4), "L", 7), np.zeros(5))
+    m = g.from_pattern(6, 6, np.array([0, 0, 2, 2, 2, 3, 3]), np.array([1, 4, 0], np.int32), 3)
+    A = _mat(m)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    for variant in ("L", "E"):
+        assert np.allclose(_run_plan(A, m.x, variant, 3), y0, rtol=0, atol=1e-15)
+
+
+# -------------------------------------------------- builder structures
+@pytest.mark.parametrize("name", golden_names())
+def test_partition_bit_exact_with_reference(name):
+    d = load_golden(name)
+    A = spb.Matrix.from_csr(int(d["nrows"]), int(d["ncols"]), d["ptr"], d["col"], d["val"])
+    co = spb.partition(A, int(d["P"]))
+    assert np.array_equal(np.concatenate([co[:-1], co[1:]], 1), d["coords"])
+
+
+@pytest.mark.parametrize("P", [1, 7, 36, 500, 4097])
+@pytest.mark.parametrize("snap", [False, True])
+def test_partition_bit_exact_with_oracle(P, snap):
+    m = g.powerlaw(20000, 2.2, 2, 3000, seed=P)
+    A = _mat(m)
+    co = spb.partition(A, P, snap)
+    s, e = O.partition(m.ptr, P, snap)
+    assert np.array_equal(co[:-1], s) and np.array_equal(co[1:], e)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_ecr_bit_exact_with_reference(name):
+    d = load_golden(name)
+    A = spb.Matrix.from_csr(int(d["nrows"]), int(d["ncols"]), d["ptr"], d["col"], d["val"])
+    co = spb.partition(A, int(d["P"]))
+    for method in (1, 2):
+        if method == 2 and np.diff(co[:, 1]).max() > 4096:
+            continue
+        e, nec, off, win = spb.ecr(A, co, method)
+        assert np.array_equal(e, d["ecr"]) and np.array_equal(nec, d["nec"])
+        assert np.array_equal(win, d["win"])
+
+
+@pytest.mark.parametrize("ipt", [3, 7, 15])
+def test_ecr_tiles_bit_exact_with_oracle(ipt):
+    m = g.rmat(14, 16, seed=ipt)
+    A = _mat(m)
+    P = -(-(m.nrows + m.nnz) // (256 * ipt))
+    co = spb.partition(A, P)
+    e, nec, off, win = spb.ecr(A, co, 2)
+    ref = O.ecr(m.col, co[:-1], co[1:])
+    assert np.array_equal(e, ref.ecr) and np.array_equal(nec, ref.nec)
+    assert np.array_equal(off, ref.win_off) and np.array_equal(win, ref.win)
+
+
+@pytest.mark.parametrize("name", golden_names())
+def test_d_layout_and_ird_bit_exact_with_reference(name):
+    d = load_golden(name)
+    A = spb.Matrix.from_csr(int(d["nrows"]), int(d["ncols"]), d["ptr"], d["col"], d["val"])
+    vrp, cs, va = spb.d_layout(A, int(d["P"]), int(d["L"]))
+    assert np.array_equal(vrp, d["v_row_ptr"]) and np.array_equal(cs, d["col_start"])
+    assert np.array_equal(va, d["val_align"])
+    assert spb.ird_vnec(A) == int(d["ird"])
+
+
+@pytest.mark.parametrize("P,L_", [(1, 1), (7, 4), (64, 64), (300, 8)])
+def test_d_layout_bit_exact_with_oracle(P, L_):
+    m = g.banded(4000, 9, seed=P + L_)
+    A = _mat(m)
+    vrp, cs, va = spb.d_layout(A, P, L_)
+    ref = O.d_build(m.ptr, m.col, m.val, P, L_)
+    assert np.array_equal(vrp, ref.v_row_ptr) and np.array_equal(cs, ref.col_start)
+    assert np.array_equal(va, ref.val_align)
+
+
+# ------------------------------------------------- reference entry points
+@pytest.mark.parametrize("name", ["rmat10", "stencil10", "banded3000_hb5", "rmat11_kat"])
+def test_reference_api_build_and_spmv(name):
+    d = load_golden(name)
+    n = int(d["nrows"])
+    A = spb.Matrix.from_csr(n, int(d["ncols"]), d["ptr"], d["col"], d["val"])
+    x = spb.Vector.from_array(d["x"])
+    y0 = O.spmv_serial(d["ptr"], d["col"], d["val"], d["x"])
+    ax, _ = O.spmv_abs(d["ptr"], d["col"], d["val"], d["x"])
+    for kind in ("L", "D", "S"):
+        plan = spb.build_vnec(kind, A)
+        y = spb.Vector(n)
+        assert spb.spmv_vnec(kind, y, A, x, plan) == ""
+        yy = y.to_array()
+        assert O.check_rows(yy, y0, ax, TOL)[0], kind
+        if "y_kat" in d:
+            assert np.array_equal(yy, d["y_kat"])
+        p = plan.contents
+        if kind == "L":
+            co = np.array([[p.thread_coord_start[t].x, p.thread_coord_start[t].y,
+                            p.thread_coord_end[t].x, p.thread_coord_end[t].y] for t in range(36)])
+            assert np.array_equal(co, d["l_coords"])
+        if kind == "D":
+            nz = int(d["ptr"][-1])
+            assert np.array_equal(np.ctypeslib.as_array(p.ecr_indices, (nz,)), d["ecr"])
+            assert np.array_equal(np.ctypeslib.as_array(p.v_row_ptr, (n + 1,)), d["v_row_ptr"])
+            G = len(d["col_start"])
+            if G:
+                assert np.array_equal(np.ctypeslib.as_array(p.col_start, (G,)), d["col_start"])
+            nec = np.ctypeslib.as_array(p.NEC_NUM, (36,))
+            assert np.array_equal(nec, d["nec"])
+        spb.free_vnec(kind, plan)
+    # SpB_mxv with no plan builds and caches one
+    y = spb.Vector(n)
+    spb.mxv(y, A, x)
+    assert O.check_rows(y.to_array(), y0, ax, TOL)[0]
+
+
+def test_resident_vectors_and_repeated_calls():
+    m = g.stencil27(12, seed=51)
+    A = _mat(m)
+    p = spb.Plan(A, "E", 7)
+    x = spb.Vector.from_array(m.x)
+    y = spb.Vector(m.nrows)
+    x.resident(True)
+    y.resident(True)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    for _ in range(3):
+        p.spmv(y, x)
+    assert O.check_rows(y.to_array(), y0, ax, TOL)[0]
+    x.set(0, 0.0)  # host write marks the device copy stale
+    xm = m.x.copy()
+    xm[0] = 0.0
+    p.spmv(y, x)
+    assert O.check_rows(y.to_array(), O.spmv_serial(m.ptr, m.col, m.val, xm), ax, TOL)[0]
+
+
+def test_graph_timing_path_gives_same_result():
+    m = g.rmat(12, 16, seed=61)
+    A = _mat(m)
+    p = spb.Plan(A, "L", 7)
+    dx = spb.DeviceArray.from_array(m.x)
+    dy = spb.DeviceArray(8 * m.nrows)
+    ms = p.time(dx.ptr, dy.ptr, 30, graph=True)
+    assert ms > 0
+    y = dy.to_array(np.float64, m.nrows)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    assert O.check_rows(y, O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
+
+
+@pytest.mark.parametrize("variant", ["L", "E"])
+def test_full_size_stencil_per_row_gate(variant):
+    m = g.stencil27(128, seed=1)
+    A = _mat(m)
+    y = _run_plan(A, m.x, variant, 7)
+    ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    assert O.check_rows(y, yl, ax, TOL)[0]

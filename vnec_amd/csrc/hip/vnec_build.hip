@@ -1,0 +1,416 @@
+// vnec_build.hip — GPU builders: merge-path partitioner, exclusive scan,
+// ECR column compression (per-tile LDS sort and dense-flag forms) and the
+// FP32 hi/lo value split.
+#include <hip/hip_runtime.h>
+
+#include "vnec_common.h"
+#include "vnec_hip.h"
+
+#define HIPCHK(x)                               \
+    do {                                        \
+        hipError_t e_ = (x);                    \
+        if (e_ != hipSuccess) return (int)e_;   \
+    } while (0)
+
+// ------------------------------------------------------------ partitioner
+// One lane per diagonal d_t = min(t * ipt, M), ipt = ceil(M / P): the
+// binary search of MergePathDivide (spmv_vnec_common.c:3-33) on the row
+// ends ptr[1..nrows] against 0..nnz-1; snap applies the S row alignment
+// of spmv_vnec_S_for_mxv.c:363-370 (x + 1 clamped to nrows, y = ptr[x]).
+__global__ void k_partition(const int32_t *__restrict__ ptr, int64_t nrows, int64_t nnz, int64_t P,
+                            int64_t ipt, int snap, int2 *__restrict__ coords)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > P) return;
+    int64_t M = nrows + nnz;
+    int64_t d = t * ipt < M ? t * ipt : M;
+    int64_t x = merge_search(ptr + 1, nrows, 0, nnz, d);
+    int64_t y = d - x;
+    if (snap) {
+        if (t == 0)
+            x = 0;
+        else if (t == P)
+            x = nrows;
+        else
+            x = x + 1 < nrows ? x + 1 : nrows;
+        y = ptr[x];
+    }
+    coords[t] = make_int2((int)x, (int)y);
+}
+
+extern "C" int vnec_hip_partition(const int32_t *ptr, int64_t nrows, int64_t nnz, int64_t P,
+                                  int snap, int32_t *coords, void *stream)
+{
+    if (P < 1) return (int)hipErrorInvalidValue;
+    int64_t M = nrows + nnz;
+    int64_t ipt = (M + P - 1) / P;
+    int64_t n = P + 1;
+    int64_t grid = (n + 255) / 256;
+    hipLaunchKernelGGL(k_partition, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, ptr,
+                       nrows, nnz, P, ipt, snap, (int2 *)coords);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------ scan
+#define SCAN_ITEMS 8
+#define SCAN_TILE (VNEC_BLOCK * SCAN_ITEMS)
+
+__global__ __launch_bounds__(VNEC_BLOCK) void k_scan_reduce(const int32_t *__restrict__ in, int64_t n,
+                                                            int32_t *__restrict__ sums)
+{
+    __shared__ int s_tmp[VNEC_NWAVES + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int v = 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_ITEMS; u++) {
+        int64_t i = base + (int64_t)u * VNEC_BLOCK + threadIdx.x;
+        if (i < n) v += in[i];
+    }
+    int tot;
+    block_excl_scan_i32(v, s_tmp, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan of one tile (thread-contiguous items), plus offset.
+__global__ __launch_bounds__(VNEC_BLOCK) void k_scan_apply(const int32_t *__restrict__ in, int32_t *out,
+                                                           int64_t n, const int32_t *__restrict__ offs)
+{
+    __shared__ int s_tmp[VNEC_NWAVES + 1];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int run = 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_ITEMS; u++) {
+        int64_t i = base + u;
+        v[u] = i < n ? in[i] : 0;
+        run += v[u];
+    }
+    int tot;
+    int ex = block_excl_scan_i32(run, s_tmp, &tot) + (offs ? offs[blockIdx.x] : 0);
+#pragma unroll
+    for (int u = 0; u < SCAN_ITEMS; u++) {
+        int64_t i = base + u;
+        if (i < n) out[i] = ex;
+        ex += v[u];
+    }
+}
+
+static int scan_rec(const int32_t *in, int32_t *out, int64_t n, hipStream_t s)
+{
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_apply, dim3(1), dim3(VNEC_BLOCK), 0, s, in, out, n,
+                           (const int32_t *)nullptr);
+        return (int)hipGetLastError();
+    }
+    int32_t *sums = nullptr;
+    HIPCHK(hipMalloc(&sums, sizeof(int32_t) * (size_t)nb));
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(VNEC_BLOCK), 0, s, in, n, sums);
+    int rc = scan_rec(sums, sums, nb, s);
+    if (rc == 0) {
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(VNEC_BLOCK), 0, s, in, out, n,
+                           (const int32_t *)sums);
+        rc = (int)hipGetLastError();
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(sums);
+    return rc ? rc : (int)e;
+}
+
+extern "C" int vnec_hip_scan_excl_i32(const int32_t *in, int32_t *out, int64_t n,
+                                      int64_t *total_host, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (n <= 0) {
+        if (total_host) *total_host = 0;
+        return 0;
+    }
+    int32_t last_in = 0;
+    if (total_host) HIPCHK(hipMemcpyAsync(&last_in, in + n - 1, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int rc = scan_rec(in, out, n, s);
+    if (rc) return rc;
+    if (total_host) {
+        int32_t last_out = 0;
+        HIPCHK(hipMemcpyAsync(&last_out, out + n - 1, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        *total_host = (int64_t)last_out + last_in;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------- ECR, tile form
+// One workgroup per tile: sort the tile's columns (with their positions)
+// in LDS, mark first occurrences, scan -> rank of every nonzero among the
+// tile's distinct columns (spmv_vnec_D_for_mxv.c:292-319 computes the same
+// ranks with dense flags).  Runs of consecutive distinct columns are
+// counted for the compact window description.
+__global__ __launch_bounds__(VNEC_BLOCK) void k_ecr_tile_pass1(
+    const int32_t *__restrict__ col, const int2 *__restrict__ coords, int64_t ntiles,
+    uint16_t *__restrict__ lcol16, int32_t *__restrict__ ecr32, int32_t *__restrict__ uniq,
+    int32_t *__restrict__ nec, int32_t *__restrict__ nruns)
+{
+    __shared__ uint32_t s_key[VNEC_ECR_TILE_MAX];
+    __shared__ uint16_t s_idx[VNEC_ECR_TILE_MAX];
+    __shared__ int s_tmp[VNEC_NWAVES + 1];
+    int64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    int y0 = coords[t].y, y1 = coords[t + 1].y;
+    int n = y1 - y0;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int k = threadIdx.x; k < n2; k += VNEC_BLOCK) {
+        s_key[k] = k < n ? (uint32_t)col[y0 + k] : 0xFFFFFFFFu;
+        s_idx[k] = (uint16_t)k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += VNEC_BLOCK) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    bool up = (i & k) == 0;
+                    uint32_t a = s_key[i], b = s_key[ixj];
+                    if ((a > b) == up) {
+                        s_key[i] = b;
+                        s_key[ixj] = a;
+                        uint16_t ti = s_idx[i];
+                        s_idx[i] = s_idx[ixj];
+                        s_idx[ixj] = ti;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int uniq_base = 0, run_base = 0;
+    for (int base = 0; base < n; base += VNEC_BLOCK) {
+        int k = base + threadIdx.x;
+        int f = 0, g = 0;
+        uint32_t key = 0;
+        if (k < n) {
+            key = s_key[k];
+            f = (k == 0) || (key != s_key[k - 1]);
+            g = f && ((k == 0) || (key != s_key[k - 1] + 1u));
+        }
+        int totf, totg;
+        int exf = block_excl_scan_i32(f, s_tmp, &totf);
+        block_excl_scan_i32(g, s_tmp, &totg);
+        if (k < n) {
+            int rank = uniq_base + exf + f - 1;
+            int pos = y0 + s_idx[k];
+            if (lcol16) lcol16[pos] = (uint16_t)rank;
+            if (ecr32) ecr32[pos] = rank;
+            if (f) uniq[y0 + rank] = (int32_t)key;
+        }
+        uniq_base += totf;
+        run_base += totg;
+    }
+    if (threadIdx.x == 0) {
+        nec[t] = uniq_base;
+        nruns[t] = run_base;
+    }
+}
+
+__global__ __launch_bounds__(VNEC_BLOCK) void k_ecr_tile_pass2(
+    const int2 *__restrict__ coords, int64_t ntiles, const int32_t *__restrict__ uniq,
+    const int32_t *__restrict__ nec, const int32_t *__restrict__ run_off,
+    int32_t *__restrict__ run_start, int32_t *__restrict__ run_wpos)
+{
+    __shared__ int s_tmp[VNEC_NWAVES + 1];
+    int64_t t = blockIdx.x;
+    if (t >= ntiles) return;
+    int y0 = coords[t].y;
+    int n = nec[t];
+    int out = run_off[t];
+    for (int base = 0; base < n; base += VNEC_BLOCK) {
+        int r = base + threadIdx.x;
+        int g = 0, u = 0;
+        if (r < n) {
+            u = uniq[y0 + r];
+            g = (r == 0) || (u != uniq[y0 + r - 1] + 1);
+        }
+        int tot;
+        int ex = block_excl_scan_i32(g, s_tmp, &tot);
+        if (g) {
+            run_start[out + ex] = u;
+            run_wpos[out + ex] = r;
+        }
+        out += tot;
+    }
+}
+
+extern "C" int vnec_hip_ecr_tiles_pass1(const int32_t *col, const int32_t *coords, int64_t ntiles,
+                                        uint16_t *lcol16, int32_t *ecr32, int32_t *uniq,
+                                        int32_t *nec, int32_t *nruns, void *stream)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_ecr_tile_pass1, dim3((unsigned)ntiles), dim3(VNEC_BLOCK), 0,
+                       (hipStream_t)stream, col, (const int2 *)coords, ntiles, lcol16, ecr32, uniq,
+                       nec, nruns);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_ecr_tiles_pass2(const int32_t *coords, int64_t ntiles, const int32_t *uniq,
+                                        const int32_t *nec, const int32_t *run_off,
+                                        int32_t *run_start, int32_t *run_wpos, void *stream)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_ecr_tile_pass2, dim3((unsigned)ntiles), dim3(VNEC_BLOCK), 0,
+                       (hipStream_t)stream, (const int2 *)coords, ntiles, uniq, nec, run_off,
+                       run_start, run_wpos);
+    return (int)hipGetLastError();
+}
+
+__global__ __launch_bounds__(VNEC_BLOCK) void k_max_tile_nnz(const int2 *__restrict__ coords,
+                                                             int64_t ntiles, int32_t *out)
+{
+    int m = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ntiles;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        int n = coords[t + 1].y - coords[t].y;
+        m = n > m ? n : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        int v = __shfl_down(m, o, VNEC_WAVE);
+        m = v > m ? v : m;
+    }
+    if (lane_id() == 0) atomicMax(out, m);
+}
+
+extern "C" int vnec_hip_max_tile_nnz(const int32_t *coords, int64_t ntiles, int32_t *max_host,
+                                     void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int32_t *d = nullptr;
+    HIPCHK(hipMalloc(&d, 4));
+    HIPCHK(hipMemsetAsync(d, 0, 4, s));
+    int64_t grid = (ntiles + 255) / 256;
+    if (grid > 1024) grid = 1024;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_max_tile_nnz, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, s,
+                       (const int2 *)coords, ntiles, d);
+    int rc = (int)hipGetLastError();
+    if (!rc) rc = (int)hipMemcpyAsync(max_host, d, 4, hipMemcpyDeviceToHost, s);
+    if (!rc) rc = (int)hipStreamSynchronize(s);
+    hipFree(d);
+    return rc;
+}
+
+// ------------------------------------------------------ ECR, dense form
+// The reference's own formulation (spmv_vnec_D_for_mxv.c:283-329): a flag
+// per (partition, column), a prefix count, and the rank read back per
+// nonzero.  One global exclusive scan over the P x ncols flags gives every
+// partition's ranks at once (offset by the partition's first entry).
+__device__ __forceinline__ int64_t part_of(const int2 *coords, int64_t P, int64_t j)
+{
+    int64_t lo = 0, hi = P;  // largest t in [0, P) with coords[t].y <= j
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (coords[mid].y <= j) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_ecr_mark(const int32_t *__restrict__ col, int64_t nnz, int64_t ncols,
+                           const int2 *__restrict__ coords, int64_t P, int32_t *flags)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = part_of(coords, P, j);
+        flags[t * ncols + col[j]] = 1;
+    }
+}
+
+__global__ void k_ecr_assign(const int32_t *__restrict__ col, int64_t nnz, int64_t ncols,
+                             const int2 *__restrict__ coords, int64_t P,
+                             const int32_t *__restrict__ rank, int32_t *__restrict__ ecr)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t = part_of(coords, P, j);
+        ecr[j] = rank[t * ncols + col[j]] - rank[t * ncols];
+    }
+}
+
+__global__ void k_ecr_win(int64_t total, int64_t ncols, const int32_t *__restrict__ flags,
+                          const int32_t *__restrict__ rank, int32_t *__restrict__ win)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) win[rank[i]] = (int32_t)(i % ncols);
+}
+
+__global__ void k_ecr_off(int64_t P, int64_t ncols, const int32_t *__restrict__ rank,
+                          int64_t grand_total, int64_t *__restrict__ win_off)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < P) win_off[t] = rank[t * ncols];
+    if (t == P) win_off[P] = grand_total;
+}
+
+extern "C" int vnec_hip_ecr_dense(const int32_t *col, int64_t nnz, int64_t ncols,
+                                  const int32_t *coords, int64_t P, int32_t *flags, int32_t *rank,
+                                  int32_t *ecr, int32_t *win, int64_t *win_off, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    int64_t total = P * ncols;
+    HIPCHK(hipMemsetAsync(flags, 0, sizeof(int32_t) * (size_t)total, s));
+    const int2 *c2 = (const int2 *)coords;
+    unsigned g1 = (unsigned)((nnz + 255) / 256 < 65536 ? (nnz + 255) / 256 : 65536);
+    if (g1 == 0) g1 = 1;
+    hipLaunchKernelGGL(k_ecr_mark, dim3(g1), dim3(256), 0, s, col, nnz, ncols, c2, P, flags);
+    HIPCHK(hipGetLastError());
+    int64_t grand = 0;
+    int rc = vnec_hip_scan_excl_i32(flags, rank, total, &grand, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_ecr_assign, dim3(g1), dim3(256), 0, s, col, nnz, ncols, c2, P, rank, ecr);
+    HIPCHK(hipGetLastError());
+    unsigned g2 = (unsigned)((total + 255) / 256 < 65536 ? (total + 255) / 256 : 65536);
+    hipLaunchKernelGGL(k_ecr_win, dim3(g2), dim3(256), 0, s, total, ncols, flags, rank, win);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_ecr_off, dim3((unsigned)((P + 1 + 255) / 256)), dim3(256), 0, s, P, ncols,
+                       rank, grand, win_off);
+    return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------- value split
+__global__ void k_split_hilo(const double *__restrict__ v, int64_t n, float *__restrict__ hi,
+                             float *__restrict__ lo)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double a = v[i];
+        float h = (float)a;
+        hi[i] = h;
+        if (lo) lo[i] = (float)(a - (double)h);
+    }
+}
+
+extern "C" int vnec_hip_split_hilo(const double *val, int64_t n, float *hi, float *lo,
+                                   void *stream)
+{
+    if (n <= 0) return 0;
+    unsigned g = (unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536);
+    hipLaunchKernelGGL(k_split_hilo, dim3(g), dim3(256), 0, (hipStream_t)stream, val, n, hi, lo);
+    return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------- scribble
+__global__ void k_scribble(uint4 *p, int64_t n16, uint32_t salt)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4((uint32_t)i ^ salt, salt, (uint32_t)(i >> 32), ~salt);
+}
+
+extern "C" int vnec_hip_scribble(void *p, size_t n, void *stream)
+{
+    static uint32_t salt = 0x9E3779B9u;
+    salt = salt * 1664525u + 1013904223u;
+    int64_t n16 = (int64_t)(n / 16);
+    if (n16 <= 0) return 0;
+    hipLaunchKernelGGL(k_scribble, dim3(4096), dim3(256), 0, (hipStream_t)stream, (uint4 *)p, n16,
+                       salt);
+    return (int)hipGetLastError();
+}

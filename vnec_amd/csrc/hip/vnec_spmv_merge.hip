@@ -1,0 +1,325 @@
+// vnec_spmv_merge.hip — merge-path FP64 SpMV for gfx950 (variants L and E).
+//
+// Restates the merge-based CSR of spmv_vnec_L_for_mxv.c:124-165 for wave64
+// workgroups: a tile is a fixed share of the merge list (row ends +
+// nonzeros) found by the GPU partitioner, so every workgroup does the same
+// amount of work whatever the row lengths.  Per tile:
+//   1. (E only) the tile's compressed x window — its distinct columns,
+//      described as runs of consecutive columns — is gathered into LDS
+//      (the GPU form of the compact-x gather, spmv_vnec_D_for_mxv.c:416-436);
+//   2. row ends and the products a_j * x(col_j) are staged in LDS with
+//      coalesced loads (values FP64, or FP32 hi + lo / hi only);
+//   3. each thread walks IPT merge items sequentially, writing rows it
+//      completes and keeping the partial sums at its two ends;
+//   4. a segmented scan over the threads (DPP within the wavefront, LDS
+//      across the four wavefronts) joins rows that span threads;
+//   5. the tile's partial last row is left in carry[t]; a second kernel
+//      adds the carries to y in tile order (deterministic fix-up).
+#include <hip/hip_runtime.h>
+
+#include "vnec_common.h"
+#include "vnec_hip.h"
+
+template <int ENC>
+__device__ __forceinline__ double load_value(const vnec_merge_args &a, int64_t j)
+{
+    if (ENC == 0) return __builtin_nontemporal_load(a.val + j);
+    if (ENC == 1)
+        return (double)__builtin_nontemporal_load(a.hi + j) +
+               (double)__builtin_nontemporal_load(a.lo + j);
+    return (double)__builtin_nontemporal_load(a.hi + j);
+}
+
+// Joins per-thread (flag, tail) pairs across the workgroup.  Returns the
+// exclusive carry into this thread; *incl gets the inclusive value (the
+// last thread's is the tile carry).  s_f / s_v: VNEC_NWAVES + 1 entries.
+__device__ __forceinline__ double block_seg_carry(int flag, double tail, int *s_f, double *s_v,
+                                                  double *incl)
+{
+    int f = flag;
+    double v = tail;
+    wave_seg_scan_f64(f, v);
+    int w = wave_id();
+    if (lane_id() == VNEC_WAVE - 1) {
+        s_f[w] = f;
+        s_v[w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c = 0.0;
+        for (int k = 0; k < VNEC_NWAVES; k++) {
+            double tw = s_v[k];
+            int fw = s_f[k];
+            s_v[k] = c;  // carry into wave k
+            c = fw ? tw : c + tw;
+        }
+    }
+    __syncthreads();
+    double cw = s_v[w];
+    double blk = f ? v : v + cw;
+    *incl = blk;
+    return wave_shift_up1_f64(blk, cw);
+}
+
+template <int IPT, bool WIN, int ENC>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, int64_t grid)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_prod[T];
+    __shared__ int s_rowend[T];
+    __shared__ double s_win[WIN ? T : 1];
+    __shared__ int s_f[VNEC_NWAVES + 1];
+    __shared__ double s_v[VNEC_NWAVES + 1];
+
+    const int64_t t = xcd_tile(blockIdx.x, grid);
+    if (t >= a.ntiles) return;
+    const int2 *co = (const int2 *)a.coords;
+    const int2 c0 = co[t], c1 = co[t + 1];
+    const int x0 = c0.x, y0 = c0.y;
+    const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+    const int tid = threadIdx.x;
+
+    if (WIN) {
+        // 1. window fill: runs to LDS (aliasing s_prod), then x gathered
+        int *s_rs = (int *)s_prod;
+        int *s_rw = s_rs + T;
+        const int r0 = a.run_off[t], nr = a.run_off[t + 1] - r0;
+        const int nec = a.nec[t];
+        for (int i = tid; i < nr; i += VNEC_BLOCK) {
+            s_rs[i] = a.run_start[r0 + i];
+            s_rw[i] = a.run_wpos[r0 + i];
+        }
+        __syncthreads();
+        for (int p = tid; p < nec; p += VNEC_BLOCK) {
+            int lo = 0, hi = nr - 1;  // last run with wpos <= p
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_rw[mid] <= p) lo = mid; else hi = mid - 1;
+            }
+            s_win[p] = a.x[s_rs[lo] + (p - s_rw[lo])];
+        }
+        __syncthreads();
+    }
+
+    // 2. row ends and products
+    for (int i = tid; i < nrow; i += VNEC_BLOCK) s_rowend[i] = a.ptr[x0 + 1 + i];
+    {
+        double av[IPT];
+        int cv[IPT];
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            int k = u * VNEC_BLOCK + tid;
+            if (k < nz) {
+                int64_t j = (int64_t)y0 + k;
+                av[u] = load_value<ENC>(a, j);
+                cv[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
+                            : __builtin_nontemporal_load(a.col + j);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            int k = u * VNEC_BLOCK + tid;
+            if (k < nz) s_prod[k] = av[u] * (WIN ? s_win[cv[u]] : a.x[cv[u]]);
+        }
+    }
+    __syncthreads();
+
+    // 3. thread-level merge walk over IPT items
+    const int items = nrow + nz;
+    const int d0 = tid * IPT < items ? tid * IPT : items;
+    const int d1 = d0 + IPT < items ? d0 + IPT : items;
+    int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+    int ny = d0 - rx;
+    double acc = 0.0, head_val = 0.0;
+    int flag = 0, head_row = 0;
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        if (d0 + u < d1) {
+            if (rx < nrow && s_rowend[rx] <= y0 + ny) {
+                if (!flag) {
+                    head_row = x0 + rx;
+                    head_val = acc;
+                    flag = 1;
+                } else {
+                    a.y[x0 + rx] = acc;
+                }
+                acc = 0.0;
+                rx++;
+            } else {
+                acc += s_prod[ny];
+                ny++;
+            }
+        }
+    }
+
+    // 4. join rows that span threads
+    double incl;
+    double cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
+    if (flag) a.y[head_row] = head_val + cin;
+    // 5. tile carry (partial row c1.x)
+    if (tid == VNEC_BLOCK - 1) a.carry[t] = incl;
+}
+
+// Multi-RHS (variant L): x is [ncols][K], y is [nrows][K].  Values and
+// columns are staged in LDS; the walk gathers K contiguous x entries.
+template <int IPT, int K, int ENC>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeK(vnec_merge_args a, int64_t grid)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_val[T];
+    __shared__ int s_col[T];
+    __shared__ int s_rowend[T];
+    __shared__ int s_f[VNEC_NWAVES + 1];
+    __shared__ double s_v[VNEC_NWAVES + 1];
+
+    const int64_t t = xcd_tile(blockIdx.x, grid);
+    if (t >= a.ntiles) return;
+    const int2 *co = (const int2 *)a.coords;
+    const int2 c0 = co[t], c1 = co[t + 1];
+    const int x0 = c0.x, y0 = c0.y;
+    const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < nrow; i += VNEC_BLOCK) s_rowend[i] = a.ptr[x0 + 1 + i];
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        int k = u * VNEC_BLOCK + tid;
+        if (k < nz) {
+            int64_t j = (int64_t)y0 + k;
+            s_val[k] = load_value<ENC>(a, j);
+            s_col[k] = __builtin_nontemporal_load(a.col + j);
+        }
+    }
+    __syncthreads();
+
+    const int items = nrow + nz;
+    const int d0 = tid * IPT < items ? tid * IPT : items;
+    const int d1 = d0 + IPT < items ? d0 + IPT : items;
+    int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+    int ny = d0 - rx;
+    double acc[K], head_val[K];
+#pragma unroll
+    for (int c = 0; c < K; c++) acc[c] = head_val[c] = 0.0;
+    int flag = 0, head_row = 0;
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        if (d0 + u < d1) {
+            if (rx < nrow && s_rowend[rx] <= y0 + ny) {
+                if (!flag) {
+                    head_row = x0 + rx;
+#pragma unroll
+                    for (int c = 0; c < K; c++) head_val[c] = acc[c];
+                    flag = 1;
+                } else {
+                    double *yr = a.y + (int64_t)(x0 + rx) * K;
+#pragma unroll
+                    for (int c = 0; c < K; c++) yr[c] = acc[c];
+                }
+#pragma unroll
+                for (int c = 0; c < K; c++) acc[c] = 0.0;
+                rx++;
+            } else {
+                const double v = s_val[ny];
+                const double *xr = a.x + (int64_t)s_col[ny] * K;
+#pragma unroll
+                for (int c = 0; c < K; c++) acc[c] += v * xr[c];
+                ny++;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+        double incl;
+        double cin = block_seg_carry(flag, acc[c], s_f, s_v, &incl);
+        if (flag) a.y[(int64_t)head_row * K + c] = head_val[c] + cin;
+        if (tid == VNEC_BLOCK - 1) a.carry[t * K + c] = incl;
+    }
+}
+
+// Deterministic fix-up: the carry of tile t belongs to row coords[t+1].x.
+// The first tile of each run of tiles carrying into one row sums the run
+// in tile order and adds it once (the reference's parallel `+=` races when
+// two partitions carry into one row, spmv_vnec_L_for_mxv.c:160-164).
+// Trailing tiles can be empty (ipt = ceil(M / P) may cover M early); their
+// carries belong to row nrows, which does not exist, and are zero.
+__global__ void k_carry_fixup(const int2 *__restrict__ co, const double *__restrict__ carry,
+                              int64_t ntiles, double *__restrict__ y, int64_t nrows, int K)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles - 1) return;
+    int row = co[t + 1].x;
+    if (row >= nrows) return;
+    if (t > 0 && co[t].x == row) return;  // not the first of its run
+    for (int c = 0; c < K; c++) {
+        double s = 0.0;
+        for (int64_t u = t; u < ntiles - 1 && co[u + 1].x == row; u++) s += carry[u * K + c];
+        y[(int64_t)row * K + c] += s;
+    }
+}
+
+// ----------------------------------------------------------- dispatch
+typedef void (*merge_kernel_t)(vnec_merge_args, int64_t);
+
+#define IPT_LIST(X, ...) X(3, __VA_ARGS__) X(5, __VA_ARGS__) X(7, __VA_ARGS__) X(9, __VA_ARGS__) \
+    X(11, __VA_ARGS__) X(13, __VA_ARGS__) X(15, __VA_ARGS__)
+
+static merge_kernel_t pick1(int ipt, bool win, int enc)
+{
+#define CASE1(I, W, E) \
+    if (ipt == I && win == W && enc == E) return k_spmv_merge1<I, W, E>;
+#define ENC3(I, W) CASE1(I, W, 0) CASE1(I, W, 1) CASE1(I, W, 2)
+#define WIN2(I, _) ENC3(I, false) ENC3(I, true)
+    IPT_LIST(WIN2, 0)
+    return nullptr;
+#undef WIN2
+#undef ENC3
+#undef CASE1
+}
+
+static merge_kernel_t pickK(int ipt, int k, int enc)
+{
+#define CASEK(I, K_, E) \
+    if (ipt == I && k == K_ && enc == E) return k_spmv_mergeK<I, K_, E>;
+#define ENCK(I, K_) CASEK(I, K_, 0) CASEK(I, K_, 1) CASEK(I, K_, 2)
+#define KS(I, _) ENCK(I, 2) ENCK(I, 4) ENCK(I, 8) ENCK(I, 16)
+    if (ipt == 3) { KS(3, 0) }
+    if (ipt == 5) { KS(5, 0) }
+    if (ipt == 7) { KS(7, 0) }
+    return nullptr;
+#undef KS
+#undef ENCK
+#undef CASEK
+}
+
+static merge_kernel_t pick(int variant, int ipt, int enc, int nrhs)
+{
+    if (variant != 0 && variant != 1) return nullptr;
+    if (nrhs == 1) return pick1(ipt, variant == 1, enc);
+    if (variant != 0) return nullptr;
+    return pickK(ipt, nrhs, enc);
+}
+
+extern "C" int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs)
+{
+    return pick(variant, ipt, encoding, nrhs) != nullptr;
+}
+
+extern "C" int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream)
+{
+    merge_kernel_t k = pick(a->variant, a->ipt, a->encoding, a->nrhs);
+    if (!k) return (int)hipErrorInvalidValue;
+    if (a->ntiles <= 0) return 0;
+    int64_t grid = xcd_grid(a->ntiles);
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, grid);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, int64_t ntiles,
+                                    double *y, int64_t nrows, int nrhs, void *stream)
+{
+    if (ntiles <= 1) return 0;
+    int64_t grid = (ntiles - 1 + 255) / 256;
+    hipLaunchKernelGGL(k_carry_fixup, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+                       (const int2 *)coords, carry, ntiles, y, nrows, nrhs);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,108 @@
+/*
+ * spb_internal.h — private object layouts of libvnec_mi355x.so.
+ *
+ * The matrix and vector structs start with the reference's own layouts
+ * (src/core/include/SpB_Matrix.h:10-17 + SpB_Matrix_CSC_or_CSR.h:12-21,
+ * SpB_Vector.h:10-15 + SpB_Vector_Dense.h:15-19), so code that downcasts a
+ * handle the way the reference's operation layer and test harness do keeps
+ * reading the same fields.  Private fields (device mirrors, residency
+ * flags, cached plans) follow the reference prefix.
+ */
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdarg.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "OpenSparseBLAS.h"
+#include "vnec_ext.h"
+#include "vnec_hip.h"
+
+#define SPB_MAGIC_MATRIX 0x4D58564Eu /* "NVXM" */
+#define SPB_MAGIC_VECTOR 0x5658564Eu
+#define SPB_MAGIC_PLAN 0x5050564Eu
+
+struct SpB_Matrix_opaque {
+    SpB_Type element_type;
+    SpB_Matrix_Format_Value_ext format_type;
+    SpB_Index row;
+    SpB_Index col;
+    size_t element_size;
+};
+
+struct SpB_Vector_opaque {
+    SpB_Type element_type;
+    SpB_Vector_Format_Value_ext format_type;
+    SpB_Index n;
+    size_t element_size;
+};
+
+typedef struct spb_csr {
+    struct SpB_Matrix_opaque base;
+    void *val;          /* nnz FP64 values (host)            */
+    SpB_Index *indices; /* nnz column indices (host, 64-bit) */
+    SpB_Index *ptr;     /* ptr_len row offsets (host)        */
+    SpB_Index ptr_len;
+    SpB_Index nnz;
+    SpB_Index capacity;
+    /* ---- private ---- */
+    uint32_t magic;
+    int dev_ready;      /* device mirror matches the host arrays */
+    int32_t *d_ptr;     /* nrows + 1 */
+    int32_t *d_col;     /* nnz       */
+    double *d_val;      /* nnz       */
+    struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
+} spb_csr;
+
+typedef struct spb_dense {
+    struct SpB_Vector_opaque base;
+    void *values; /* n FP64 (host) */
+    /* ---- private ---- */
+    uint32_t magic;
+    double *d_values;
+    int resident;   /* device copy is authoritative between calls      */
+    int host_stale; /* device holds newer values than host             */
+    int dev_stale;  /* host holds newer values than device             */
+} spb_dense;
+
+struct vnec_plan_s {
+    uint32_t magic;
+    vnec_tuning tune;
+    int64_t nrows, ncols, nnz;
+    int64_t ntiles;
+    int32_t *coords;   /* 2 (ntiles + 1) */
+    double *carry;     /* ntiles * nrhs  */
+    /* variant E */
+    uint16_t *lcol;
+    int32_t *run_off, *run_start, *run_wpos, *nec;
+    /* encodings */
+    float *hi, *lo;
+    /* borrowed from the matrix mirror */
+    const int32_t *ptr, *col;
+    const double *val;
+    const spb_csr *A;
+    int64_t dev_bytes;
+    /* cached graph for timing */
+    hipGraphExec_t gexec;
+    const double *g_x;
+    double *g_y;
+    int g_reps;
+};
+
+/* ---- library runtime (spb_core.c) ---- */
+hipStream_t vnec_stream(void);
+SpB_Info vnec_fail(const char *fmt, ...);
+SpB_Info vnec_hip_ok(int rc, const char *what);
+int vnec_runtime_ready(void);
+void *vnec_malloc_dev(size_t bytes, int64_t *acct);
+void vnec_free_dev(void *p);
+spb_csr *spb_as_csr(const SpB_Matrix A);
+spb_dense *spb_as_dense(const SpB_Vector v);
+SpB_Info spb_matrix_upload(spb_csr *a);
+SpB_Info spb_vector_to_device(spb_dense *v);
+SpB_Info spb_vector_from_device(spb_dense *v);
+SpB_Info spb_vector_device_alloc(spb_dense *v);
+
+/* ---- plans (spb_plan.c) ---- */
+SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev);
+int vnec_select_tuning(const spb_csr *a, vnec_tuning *t);

@@ -1,0 +1,389 @@
+/*
+ * spb_plan.c — GPU execution plans: tiles from the GPU merge-path
+ * partitioner, the per-tile ECR windows (variant E), value encodings, the
+ * launch sequence (SpMV kernel + carry fix-up) and event timing.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "spb_internal.h"
+
+#define TILE_THREADS 256
+
+static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
+
+static void plan_release(struct vnec_plan_s *p)
+{
+    if (p->gexec) hipGraphExecDestroy(p->gexec);
+    vnec_free_dev(p->coords);
+    vnec_free_dev(p->carry);
+    vnec_free_dev(p->lcol);
+    vnec_free_dev(p->run_off);
+    vnec_free_dev(p->run_start);
+    vnec_free_dev(p->run_wpos);
+    vnec_free_dev(p->nec);
+    vnec_free_dev(p->hi);
+    vnec_free_dev(p->lo);
+    memset(p, 0, sizeof *p);
+}
+
+SpB_Info vnec_plan_free_ext(vnec_plan *plan)
+{
+    if (!plan || !*plan) return vnec_fail("vnec_plan_free_ext: NULL plan");
+    struct vnec_plan_s *p = *plan;
+    if (p->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_plan_free_ext: not a plan");
+    hipStreamSynchronize(vnec_stream());
+    plan_release(p);
+    free(p);
+    *plan = NULL;
+    return SpB_SUCCESS;
+}
+
+/* Per-tile ECR windows for variant E. */
+static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a)
+{
+    hipStream_t s = vnec_stream();
+    int64_t P = p->ntiles, nnz = p->nnz;
+    int32_t mx = 0;
+    if (vnec_hip_ok(vnec_hip_max_tile_nnz(p->coords, P, &mx, s), "max tile nnz")) return SpB_FAILURE;
+    if (mx > VNEC_ECR_TILE_MAX) return vnec_fail("tile holds %d nonzeros (> %d)", mx, VNEC_ECR_TILE_MAX);
+    int32_t *uniq = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    int32_t *nruns = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), NULL);
+    p->lcol = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * (size_t)(nnz ? nnz : 1), &p->dev_bytes);
+    p->nec = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)P, &p->dev_bytes);
+    p->run_off = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), &p->dev_bytes);
+    SpB_Info rc = (uniq && nruns && p->lcol && p->nec && p->run_off) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass1(a->d_col, p->coords, P, p->lcol, NULL, uniq,
+                                                  p->nec, nruns, s), "ecr pass 1");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemsetAsync(nruns + P, 0, sizeof(int32_t), s), "memset");
+    int64_t total = 0;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_scan_excl_i32(nruns, p->run_off, P + 1, &total, s), "run scan");
+    if (rc == SpB_SUCCESS) {
+        p->run_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(total ? total : 1), &p->dev_bytes);
+        p->run_wpos = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(total ? total : 1), &p->dev_bytes);
+        if (!p->run_start || !p->run_wpos) rc = SpB_FAILURE;
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass2(p->coords, P, uniq, p->nec, p->run_off,
+                                                  p->run_start, p->run_wpos, s), "ecr pass 2");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "windows sync");
+    vnec_free_dev(uniq);
+    vnec_free_dev(nruns);
+    return rc;
+}
+
+SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tuning *t)
+{
+    if (!plan) return vnec_fail("vnec_plan_build_ext: NULL plan handle");
+    *plan = NULL;
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return SpB_FAILURE;
+    if (!vnec_runtime_ready()) return SpB_FAILURE;
+    vnec_tuning tune;
+    if (t && t->variant >= 0) {
+        tune = *t;
+    } else {
+        if (vnec_select_tuning(a, &tune)) return SpB_FAILURE;
+        if (t) {
+            tune.encoding = t->encoding;
+            if (t->nrhs > 0) tune.nrhs = t->nrhs;
+        }
+    }
+    if (tune.nrhs <= 0) tune.nrhs = 1;
+    if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_E)
+        return vnec_fail("variant %d is not available", tune.variant);
+    if (!valid_ipt(tune.ipt)) return vnec_fail("ipt %d: use an odd value in [3, 15]", tune.ipt);
+    if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
+    if (!vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs))
+        return vnec_fail("no kernel for variant %d, ipt %d, encoding %d, nrhs %d", tune.variant,
+                         tune.ipt, tune.encoding, tune.nrhs);
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+
+    struct vnec_plan_s *p = (struct vnec_plan_s *)calloc(1, sizeof *p);
+    if (!p) return vnec_fail("out of memory");
+    p->magic = SPB_MAGIC_PLAN;
+    p->tune = tune;
+    p->nrows = (int64_t)a->base.row;
+    p->ncols = (int64_t)a->base.col;
+    p->nnz = (int64_t)a->nnz;
+    p->ptr = a->d_ptr;
+    p->col = a->d_col;
+    p->val = a->d_val;
+    p->A = a;
+    int64_t M = p->nrows + p->nnz;
+    int64_t T = (int64_t)TILE_THREADS * tune.ipt;
+    p->ntiles = M > 0 ? (M + T - 1) / T : 0;
+    hipStream_t s = vnec_stream();
+    SpB_Info rc = SpB_SUCCESS;
+    if (p->ntiles > 0) {
+        p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
+        p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(p->ntiles * tune.nrhs), &p->dev_bytes);
+        if (!p->coords || !p->carry) rc = SpB_FAILURE;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
+                             "partition");
+        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) rc = build_windows(p, a);
+        if (rc == SpB_SUCCESS && tune.encoding != VNEC_ENC_FP64 && p->nnz > 0) {
+            p->hi = (float *)vnec_malloc_dev(sizeof(float) * (size_t)p->nnz, &p->dev_bytes);
+            if (tune.encoding == VNEC_ENC_HILO)
+                p->lo = (float *)vnec_malloc_dev(sizeof(float) * (size_t)p->nnz, &p->dev_bytes);
+            if (!p->hi || (tune.encoding == VNEC_ENC_HILO && !p->lo)) rc = SpB_FAILURE;
+            if (rc == SpB_SUCCESS)
+                rc = vnec_hip_ok(vnec_hip_split_hilo(a->d_val, p->nnz, p->hi, p->lo, s), "split");
+        }
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "plan sync");
+    }
+    if (rc != SpB_SUCCESS) {
+        plan_release(p);
+        free(p);
+        return SpB_FAILURE;
+    }
+    *plan = p;
+    return SpB_SUCCESS;
+}
+
+SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int64_t *dev_bytes)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_plan_info_ext: not a plan");
+    if (t) *t = plan->tune;
+    if (ntiles) *ntiles = plan->ntiles;
+    if (dev_bytes) *dev_bytes = plan->dev_bytes;
+    return SpB_SUCCESS;
+}
+
+/* Enqueue one SpMV (kernel + fix-up) on the library stream. */
+SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev)
+{
+    hipStream_t s = vnec_stream();
+    if (p->ntiles == 0) {
+        if (p->nrows > 0)
+            return vnec_hip_ok(hipMemsetAsync(y_dev, 0, sizeof(double) * (size_t)(p->nrows * p->tune.nrhs), s),
+                               "zero y");
+        return SpB_SUCCESS;
+    }
+    vnec_merge_args m;
+    memset(&m, 0, sizeof m);
+    m.ptr = p->ptr;
+    m.col = p->col;
+    m.lcol = p->lcol;
+    m.val = p->val;
+    m.hi = p->hi;
+    m.lo = p->lo;
+    m.coords = p->coords;
+    m.ntiles = p->ntiles;
+    m.run_off = p->run_off;
+    m.run_start = p->run_start;
+    m.run_wpos = p->run_wpos;
+    m.nec = p->nec;
+    m.x = x_dev;
+    m.y = y_dev;
+    m.carry = p->carry;
+    m.nrows = p->nrows;
+    m.ipt = p->tune.ipt;
+    m.variant = p->tune.variant;
+    m.encoding = p->tune.encoding;
+    m.nrhs = p->tune.nrhs;
+    if (vnec_hip_ok(vnec_hip_spmv_merge(&m, s), "spmv kernel")) return SpB_FAILURE;
+    return vnec_hip_ok(vnec_hip_carry_fixup(p->coords, p->carry, p->ntiles, y_dev, p->nrows, p->tune.nrhs, s),
+                       "carry fix-up");
+}
+
+SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmv_dev_ext: not a plan");
+    return vnec_plan_run(plan, x_dev, y_dev);
+}
+
+SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmv_ext: not a plan");
+    spb_dense *xv = spb_as_dense(x), *yv = spb_as_dense(y);
+    if (!xv || !yv) return SpB_FAILURE;
+    int k = plan->tune.nrhs;
+    if ((int64_t)xv->base.n != plan->ncols * k || (int64_t)yv->base.n != plan->nrows * k)
+        return vnec_fail("vnec_spmv_ext: x has %lu entries (need %ld), y %lu (need %ld)",
+                         xv->base.n, (long)(plan->ncols * k), yv->base.n, (long)(plan->nrows * k));
+    if (spb_vector_to_device(xv) != SpB_SUCCESS) return SpB_FAILURE;
+    if (spb_vector_device_alloc(yv) != SpB_SUCCESS) return SpB_FAILURE;
+    if (vnec_plan_run(plan, xv->d_values, yv->d_values) != SpB_SUCCESS) return SpB_FAILURE;
+    yv->dev_stale = 0;
+    yv->host_stale = 1;
+    if (!yv->resident) return spb_vector_from_device(yv);
+    return vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "spmv sync");
+}
+
+/* ---------------------------------------------------------------- timing */
+static void *g_flush = NULL;
+static size_t g_flush_bytes = 0;
+
+SpB_Info vnec_time_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+                           int use_graph, size_t flush_bytes, double *ms)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN || iters < 1 || !ms)
+        return vnec_fail("vnec_time_dev_ext: bad arguments");
+    hipStream_t s = vnec_stream();
+    hipEvent_t e0, e1;
+    if (vnec_hip_ok(hipEventCreate(&e0), "event") || vnec_hip_ok(hipEventCreate(&e1), "event"))
+        return SpB_FAILURE;
+    SpB_Info rc = SpB_SUCCESS;
+    if (flush_bytes > 0 && flush_bytes != g_flush_bytes) {
+        vnec_free_dev(g_flush);
+        g_flush = vnec_malloc_dev(flush_bytes, NULL);
+        g_flush_bytes = g_flush ? flush_bytes : 0;
+        if (!g_flush) rc = SpB_FAILURE;
+    }
+    /* warm-up */
+    if (rc == SpB_SUCCESS) rc = vnec_plan_run(plan, x_dev, y_dev);
+    if (rc == SpB_SUCCESS && use_graph && !flush_bytes &&
+        (!plan->gexec || plan->g_x != x_dev || plan->g_y != y_dev)) {
+        if (plan->gexec) {
+            hipGraphExecDestroy(plan->gexec);
+            plan->gexec = NULL;
+        }
+        int reps = iters < 20 ? iters : 20;
+        hipGraph_t g;
+        rc = vnec_hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+        for (int r = 0; rc == SpB_SUCCESS && r < reps; r++) rc = vnec_plan_run(plan, x_dev, y_dev);
+        SpB_Info rc2 = vnec_hip_ok(hipStreamEndCapture(s, &g), "end capture");
+        if (rc == SpB_SUCCESS) rc = rc2;
+        if (rc == SpB_SUCCESS) {
+            rc = vnec_hip_ok(hipGraphInstantiate(&plan->gexec, g, NULL, NULL, 0), "instantiate");
+            hipGraphDestroy(g);
+        }
+        if (rc == SpB_SUCCESS) {
+            plan->g_x = x_dev;
+            plan->g_y = y_dev;
+            plan->g_reps = reps;
+            rc = vnec_hip_ok(hipGraphLaunch(plan->gexec, s), "graph warm-up");
+        }
+    }
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "warm-up sync");
+    double total_ms = 0.0;
+    int done = 0;
+    if (rc == SpB_SUCCESS && flush_bytes) {
+        for (int i = 0; rc == SpB_SUCCESS && i < iters; i++) {
+            rc = vnec_hip_ok(vnec_hip_scribble(g_flush, flush_bytes, s), "flush");
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
+            if (rc == SpB_SUCCESS) rc = vnec_plan_run(plan, x_dev, y_dev);
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e1, s), "record");
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(e1), "sync");
+            float f = 0.f;
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&f, e0, e1), "elapsed");
+            total_ms += f;
+            done++;
+        }
+    } else if (rc == SpB_SUCCESS) {
+        rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
+        if (use_graph && plan->gexec) {
+            for (done = 0; rc == SpB_SUCCESS && done < iters; done += plan->g_reps)
+                rc = vnec_hip_ok(hipGraphLaunch(plan->gexec, s), "graph launch");
+        } else {
+            for (done = 0; rc == SpB_SUCCESS && done < iters; done++)
+                rc = vnec_plan_run(plan, x_dev, y_dev);
+        }
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e1, s), "record");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(e1), "sync");
+        float f = 0.f;
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&f, e0, e1), "elapsed");
+        total_ms = f;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc != SpB_SUCCESS) return rc;
+    *ms = total_ms / (done > 0 ? done : 1);
+    return SpB_SUCCESS;
+}
+
+/* Mean duration of the SpMV kernel and of the fix-up kernel, each between
+ * its own pair of events on the library stream (iters SpMVs). */
+SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+                               double *ms_main, double *ms_fixup)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN || iters < 1)
+        return vnec_fail("vnec_time_kernels_ext: bad arguments");
+    if (plan->ntiles == 0) {
+        if (ms_main) *ms_main = 0.0;
+        if (ms_fixup) *ms_fixup = 0.0;
+        return SpB_SUCCESS;
+    }
+    hipStream_t s = vnec_stream();
+    hipEvent_t ev[3];
+    for (int i = 0; i < 3; i++)
+        if (vnec_hip_ok(hipEventCreate(&ev[i]), "event")) return SpB_FAILURE;
+    vnec_merge_args m;
+    memset(&m, 0, sizeof m);
+    m.ptr = plan->ptr;
+    m.col = plan->col;
+    m.lcol = plan->lcol;
+    m.val = plan->val;
+    m.hi = plan->hi;
+    m.lo = plan->lo;
+    m.coords = plan->coords;
+    m.ntiles = plan->ntiles;
+    m.run_off = plan->run_off;
+    m.run_start = plan->run_start;
+    m.run_wpos = plan->run_wpos;
+    m.nec = plan->nec;
+    m.x = x_dev;
+    m.y = y_dev;
+    m.carry = plan->carry;
+    m.nrows = plan->nrows;
+    m.ipt = plan->tune.ipt;
+    m.variant = plan->tune.variant;
+    m.encoding = plan->tune.encoding;
+    m.nrhs = plan->tune.nrhs;
+    SpB_Info rc = vnec_plan_run(plan, x_dev, y_dev);
+    double t_main = 0.0, t_fix = 0.0;
+    for (int i = 0; rc == SpB_SUCCESS && i < iters; i++) {
+        float a = 0.f, b = 0.f;
+        rc = vnec_hip_ok(hipEventRecord(ev[0], s), "record");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_spmv_merge(&m, s), "spmv");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[1], s), "record");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(vnec_hip_carry_fixup(plan->coords, plan->carry, plan->ntiles, y_dev,
+                                                  plan->nrows, plan->tune.nrhs, s), "fixup");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[2], s), "record");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(ev[2]), "sync");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&b, ev[1], ev[2]), "elapsed");
+        t_main += a;
+        t_fix += b;
+    }
+    for (int i = 0; i < 3; i++) hipEventDestroy(ev[i]);
+    if (rc != SpB_SUCCESS) return rc;
+    if (ms_main) *ms_main = t_main / iters;
+    if (ms_fixup) *ms_fixup = t_fix / iters;
+    return SpB_SUCCESS;
+}
+
+/* -------------------------------------------------------------- selector
+ * Placeholder rule until the learned selector is wired in: locality-rich
+ * matrices (small mean column spread per row) use the LDS window. */
+int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
+{
+    memset(t, 0, sizeof *t);
+    t->variant = VNEC_VAR_L;
+    t->ipt = 7;
+    t->lanes = 64;
+    t->encoding = VNEC_ENC_FP64;
+    t->nrhs = 1;
+    int64_t nr = (int64_t)a->base.row;
+    if (nr > 0 && a->nnz > 0) {
+        /* sampled bandwidth: mean |col - row| over up to 4096 rows */
+        int64_t step = nr / 4096 > 0 ? nr / 4096 : 1, rows = 0;
+        double spread = 0.0;
+        for (int64_t r = 0; r < nr; r += step) {
+            SpB_Index b = a->ptr[r], e = a->ptr[r + 1];
+            if (e > b) {
+                spread += (double)(a->indices[e - 1] - a->indices[b]);
+                rows++;
+            }
+        }
+        if (rows && spread / rows < 65536.0) t->variant = VNEC_VAR_E;
+    }
+    return 0;
+}

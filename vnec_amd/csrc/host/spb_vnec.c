@@ -1,0 +1,592 @@
+/*
+ * spb_vnec.c — the reference's VNEC entry points (IRD_VNEC, Build_VNEC_*,
+ * SpMV_VNEC_*_FP64) and SpB_mxv, on the GPU.
+ *
+ * Each Build_VNEC_* call returns the reference's public plan struct with
+ * its host fields filled at the reference's compiled parameters (P = 36
+ * partitions, L = 8 lanes, src/operation/include/SpB_mxv.h:26-33), computed
+ * by the GPU builders, and keeps behind it a GPU execution plan laid out
+ * for wave64 tiles.  SpMV_VNEC_* runs that GPU plan; y is valid on return.
+ * The heavy FP64 payload fields (val_align, reorder_mat.nnz) stay NULL:
+ * the device plan holds the values, and the reference's val_align would
+ * take 64 bytes per nonzero of host memory.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "spb_internal.h"
+
+#define REF_P 36     /* NUM_THREADS, SpB_mxv.h:28 */
+#define REF_L 8      /* nLanes_f64,  SpB_mxv.h:30 */
+#define REF_IRD_THR 0.477 /* IRD_thr_fp64, SpB_mxv.h:32 */
+
+typedef enum { KIND_L = 1, KIND_D = 2, KIND_S = 3 } plan_kind;
+
+/* Registry of public plans handed out, so SpB_mxv can tell them apart. */
+typedef struct wrap {
+    void *pub;
+    plan_kind kind;
+    struct vnec_plan_s *gpu;
+    const spb_csr *A;
+    struct wrap *next;
+} wrap;
+static wrap *g_wraps = NULL;
+
+static wrap *find_wrap(const void *pub)
+{
+    for (wrap *w = g_wraps; w; w = w->next)
+        if (w->pub == pub) return w;
+    return NULL;
+}
+
+static void add_wrap(void *pub, plan_kind kind, struct vnec_plan_s *gpu, const spb_csr *A)
+{
+    wrap *w = (wrap *)calloc(1, sizeof *w);
+    w->pub = pub;
+    w->kind = kind;
+    w->gpu = gpu;
+    w->A = A;
+    w->next = g_wraps;
+    g_wraps = w;
+}
+
+static struct vnec_plan_s *drop_wrap(const void *pub)
+{
+    wrap **pp = &g_wraps;
+    while (*pp) {
+        if ((*pp)->pub == pub) {
+            wrap *w = *pp;
+            struct vnec_plan_s *g = w->gpu;
+            *pp = w->next;
+            free(w);
+            return g;
+        }
+        pp = &(*pp)->next;
+    }
+    return NULL;
+}
+
+vnec_plan VNEC_plan_of_ext(const void *reference_plan)
+{
+    wrap *w = find_wrap(reference_plan);
+    return w ? w->gpu : NULL;
+}
+
+/* ------------------------------------------------ structure exports */
+static int64_t ipt_of(int64_t nrows, int64_t nnz, int64_t P)
+{
+    int64_t M = nrows + nnz;
+    return (M + P - 1) / P;
+}
+
+SpB_Info vnec_partition_ext(const SpB_Matrix A, int64_t P, int snap, int32_t *coords)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a || !coords || P < 1) return vnec_fail("vnec_partition_ext: bad arguments");
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    int32_t *d = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(P + 1), NULL);
+    if (!d) return SpB_FAILURE;
+    SpB_Info rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, (int64_t)a->base.row, (int64_t)a->nnz, P,
+                                                 snap, d, s), "partition");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(coords, d, sizeof(int32_t) * 2 * (size_t)(P + 1),
+                                        hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    vnec_free_dev(d);
+    return rc;
+}
+
+/* Device ECR at the partitions in d_coords (device).  Writes device ecr
+ * (nnz), win (<= nnz), win_off (P + 1) and host nec sums. */
+static SpB_Info ecr_device(spb_csr *a, int64_t P, const int32_t *d_coords, int method,
+                           int32_t *d_ecr, int32_t *d_win, int64_t *d_win_off)
+{
+    hipStream_t s = vnec_stream();
+    int64_t nnz = (int64_t)a->nnz, ncols = (int64_t)a->base.col;
+    int32_t mx = 0;
+    if (vnec_hip_ok(vnec_hip_max_tile_nnz(d_coords, P, &mx, s), "max part nnz")) return SpB_FAILURE;
+    int dense_ok = P * (ncols > 0 ? ncols : 1) < ((int64_t)1 << 31) - 1;
+    int tile_ok = mx <= VNEC_ECR_TILE_MAX;
+    if (method == 0) method = tile_ok ? 2 : 1;
+    if (method == 1 && !dense_ok) return vnec_fail("dense ECR needs P * ncols < 2^31");
+    if (method == 2 && !tile_ok) return vnec_fail("tile ECR needs partitions of <= %d nonzeros", VNEC_ECR_TILE_MAX);
+    if (method == 1) {
+        int64_t tot = P * ncols;
+        int32_t *flags = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(tot ? tot : 1), NULL);
+        int32_t *rank = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(tot ? tot : 1), NULL);
+        SpB_Info rc = (flags && rank) ? SpB_SUCCESS : SpB_FAILURE;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(vnec_hip_ecr_dense(a->d_col, nnz, ncols, d_coords, P, flags, rank, d_ecr,
+                                                d_win, d_win_off, s), "dense ecr");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+        vnec_free_dev(flags);
+        vnec_free_dev(rank);
+        return rc;
+    }
+    /* tile form: uniq lists sit at each partition's first nonzero; compact */
+    int32_t *uniq = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    int32_t *nec = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), NULL);
+    int32_t *nruns = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), NULL);
+    SpB_Info rc = (uniq && nec && nruns) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass1(a->d_col, d_coords, P, NULL, d_ecr, uniq, nec, nruns, s),
+                         "tile ecr");
+    int32_t *hnec = NULL, *hco = NULL;
+    int64_t *hoff = NULL;
+    if (rc == SpB_SUCCESS) {
+        hnec = (int32_t *)malloc(sizeof(int32_t) * (size_t)P);
+        hco = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(P + 1));
+        hoff = (int64_t *)malloc(sizeof(int64_t) * (size_t)(P + 1));
+        rc = (hnec && hco && hoff) ? SpB_SUCCESS : vnec_fail("out of memory");
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(hnec, nec, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(hco, d_coords, sizeof(int32_t) * 2 * (size_t)(P + 1),
+                                        hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc == SpB_SUCCESS) {
+        int64_t off = 0;
+        for (int64_t t = 0; t < P && rc == SpB_SUCCESS; t++) {
+            hoff[t] = off;
+            if (hnec[t] > 0)
+                rc = vnec_hip_ok(hipMemcpyAsync(d_win + off, uniq + hco[2 * t + 1],
+                                                sizeof(int32_t) * (size_t)hnec[t],
+                                                hipMemcpyDeviceToDevice, s), "d2d");
+            off += hnec[t];
+        }
+        hoff[P] = off;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipMemcpyAsync(d_win_off, hoff, sizeof(int64_t) * (size_t)(P + 1),
+                                            hipMemcpyHostToDevice, s), "h2d");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    }
+    free(hnec);
+    free(hco);
+    free(hoff);
+    vnec_free_dev(uniq);
+    vnec_free_dev(nec);
+    vnec_free_dev(nruns);
+    return rc;
+}
+
+SpB_Info vnec_ecr_ext(const SpB_Matrix A, int64_t P, const int32_t *coords, int method, int32_t *ecr,
+                      int32_t *nec, int64_t *win_off, int32_t *win)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a || !coords || P < 1 || !ecr || !nec || !win_off || !win)
+        return vnec_fail("vnec_ecr_ext: bad arguments");
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    int64_t nnz = (int64_t)a->nnz;
+    int32_t *dco = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(P + 1), NULL);
+    int32_t *decr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    int32_t *dwin = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    int64_t *doff = (int64_t *)vnec_malloc_dev(sizeof(int64_t) * (size_t)(P + 1), NULL);
+    SpB_Info rc = (dco && decr && dwin && doff) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(dco, coords, sizeof(int32_t) * 2 * (size_t)(P + 1),
+                                        hipMemcpyHostToDevice, s), "h2d");
+    if (rc == SpB_SUCCESS) rc = ecr_device(a, P, dco, method, decr, dwin, doff);
+    if (rc == SpB_SUCCESS && nnz)
+        rc = vnec_hip_ok(hipMemcpyAsync(ecr, decr, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(win_off, doff, sizeof(int64_t) * (size_t)(P + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc == SpB_SUCCESS && win_off[P] > 0)
+        rc = vnec_hip_ok(hipMemcpyAsync(win, dwin, sizeof(int32_t) * (size_t)win_off[P], hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc == SpB_SUCCESS)
+        for (int64_t t = 0; t < P; t++) nec[t] = (int32_t)(win_off[t + 1] - win_off[t]);
+    vnec_free_dev(dco);
+    vnec_free_dev(decr);
+    vnec_free_dev(dwin);
+    vnec_free_dev(doff);
+    return rc;
+}
+
+/* D layout on the device at (P, L).  Device outputs: coords, ecr, win,
+ * win_off, v_row_ptr (nrows + 1), col_start (G), val_align (G * L, only
+ * when want_vals).  *G receives the group count. */
+typedef struct {
+    int32_t *coords, *ecr, *win, *vrp, *col_start;
+    int64_t *win_off;
+    double *val_align;
+    int64_t G;
+} dlay;
+
+static void dlay_free(dlay *d)
+{
+    vnec_free_dev(d->coords);
+    vnec_free_dev(d->ecr);
+    vnec_free_dev(d->win);
+    vnec_free_dev(d->vrp);
+    vnec_free_dev(d->col_start);
+    vnec_free_dev(d->win_off);
+    vnec_free_dev(d->val_align);
+    memset(d, 0, sizeof *d);
+}
+
+static SpB_Info d_layout_device(spb_csr *a, int64_t P, int L, int want_vals, dlay *d)
+{
+    memset(d, 0, sizeof *d);
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
+    d->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(P + 1), NULL);
+    d->ecr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    d->win = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    d->win_off = (int64_t *)vnec_malloc_dev(sizeof(int64_t) * (size_t)(P + 1), NULL);
+    d->vrp = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nr + 1), NULL);
+    SpB_Info rc = (d->coords && d->ecr && d->win && d->win_off && d->vrp) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, nr, nnz, P, 0, d->coords, s), "partition");
+    if (rc == SpB_SUCCESS) rc = ecr_device(a, P, d->coords, 0, d->ecr, d->win, d->win_off);
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_d_count(a->d_ptr, d->ecr, nr, d->coords, P, L, d->vrp, s), "d count");
+    int64_t G = 0;
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_scan_excl_i32(d->vrp, d->vrp, nr + 1, &G, s), "d scan");
+    if (rc == SpB_SUCCESS) {
+        d->G = G;
+        d->col_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(G ? G : 1), NULL);
+        if (!d->col_start) rc = SpB_FAILURE;
+        if (want_vals && rc == SpB_SUCCESS) {
+            d->val_align = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(G * L ? G * L : 1), NULL);
+            if (!d->val_align) rc = SpB_FAILURE;
+            else rc = vnec_hip_ok(hipMemsetAsync(d->val_align, 0, sizeof(double) * (size_t)(G * L), s), "zero");
+        }
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_d_fill(a->d_ptr, d->ecr, a->d_val, nr, d->coords, P, L, d->vrp,
+                                         d->col_start, d->val_align, s), "d fill");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc != SpB_SUCCESS) dlay_free(d);
+    return rc;
+}
+
+/* Group count only (what IRD needs). */
+SpB_Info vnec_d_layout_ext(const SpB_Matrix A, int64_t P, int L, int32_t *v_row_ptr,
+                           int32_t *col_start, double *val_align, int64_t *groups)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a || P < 1 || L < 1) return vnec_fail("vnec_d_layout_ext: bad arguments");
+    dlay d;
+    if (d_layout_device(a, P, L, val_align != NULL, &d) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    int64_t nr = (int64_t)a->base.row;
+    SpB_Info rc = SpB_SUCCESS;
+    if (groups) *groups = d.G;
+    if (v_row_ptr)
+        rc = vnec_hip_ok(hipMemcpyAsync(v_row_ptr, d.vrp, sizeof(int32_t) * (size_t)(nr + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && col_start && d.G)
+        rc = vnec_hip_ok(hipMemcpyAsync(col_start, d.col_start, sizeof(int32_t) * (size_t)d.G, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && val_align && d.G)
+        rc = vnec_hip_ok(hipMemcpyAsync(val_align, d.val_align, sizeof(double) * (size_t)(d.G * L), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    dlay_free(&d);
+    return rc;
+}
+
+/* --------------------------------------------------------------- IRD */
+/* ird_calc.c:2-123 — IRD = (float)nnz / (float)(G * L) at P = 36, L = 8,
+ * D when IRD >= (float)0.477. */
+SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *ird, int *decision)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return SpB_FAILURE;
+    int64_t G = 0;
+    if (vnec_d_layout_ext(A, P, L, NULL, NULL, NULL, &G) != SpB_SUCCESS) return SpB_FAILURE;
+    int32_t slots = (int32_t)(G * L);
+    float r = (float)a->nnz / (float)slots;
+    if (ird) *ird = r;
+    if (decision) *decision = r >= (float)thr ? SpB_VNEC_D : SpB_VNEC_S;
+    return SpB_SUCCESS;
+}
+
+SpB_VNEC_type IRD_VNEC(const SpB_Matrix A, SpB_Type type)
+{
+    int dec = SpB_VNEC_S;
+    if (type != SpB_FP64) {
+        vnec_fail("IRD_VNEC: only SpB_FP64 is on this path");
+        return SpB_VNEC_S;
+    }
+    if (vnec_ird_ext(A, REF_P, REF_L, REF_IRD_THR, NULL, &dec) != SpB_SUCCESS) return SpB_VNEC_S;
+    return (SpB_VNEC_type)dec;
+}
+
+/* ------------------------------------------------------ public plans */
+static struct vnec_plan_s *gpu_plan_for(spb_csr *a)
+{
+    vnec_plan p = NULL;
+    if (vnec_plan_build_ext(&p, (SpB_Matrix)a, NULL) != SpB_SUCCESS) return NULL;
+    return p;
+}
+
+/* spmv_vnec_L_for_mxv.c:2-35 */
+struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
+    VNEC_L *l = (VNEC_L *)calloc(1, sizeof(VNEC_L));
+    int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
+    if (!l || !co) {
+        free(l);
+        free(co);
+        vnec_fail("Build_VNEC_L: out of memory");
+        return NULL;
+    }
+    if (vnec_partition_ext(A, REF_P, 0, co) != SpB_SUCCESS) {
+        free(l);
+        free(co);
+        return NULL;
+    }
+    int64_t ipt = ipt_of(nr, nnz, REF_P), M = nr + nnz;
+    l->num_merge_items = (int)M;
+    l->items_per_thread = (int)ipt;
+    l->diagonal_start = (int *)malloc(sizeof(int) * REF_P);
+    l->diagonal_end = (int *)malloc(sizeof(int) * REF_P);
+    l->thread_coord_start = (coord *)malloc(sizeof(coord) * REF_P);
+    l->thread_coord_end = (coord *)malloc(sizeof(coord) * REF_P);
+    l->nz_indices = (int *)malloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+    for (int t = 0; t < REF_P; t++) {
+        int64_t ds = ipt * t < M ? ipt * t : M;
+        l->diagonal_start[t] = (int)ds;
+        l->diagonal_end[t] = (int)(ds + ipt < M ? ds + ipt : M);
+        l->thread_coord_start[t].x = co[2 * t];
+        l->thread_coord_start[t].y = co[2 * t + 1];
+        l->thread_coord_end[t].x = co[2 * t + 2];
+        l->thread_coord_end[t].y = co[2 * t + 3];
+    }
+    for (int64_t j = 0; j < nnz; j++) l->nz_indices[j] = (int)j;
+    free(co);
+    struct vnec_plan_s *g = gpu_plan_for(a);
+    if (!g) {
+        VNEC_L_free_ext(l);
+        return NULL;
+    }
+    add_wrap(l, KIND_L, g, a);
+    return l;
+}
+
+/* spmv_vnec_D_for_mxv.c:242-398, host fields at P = 36, L = 8 */
+struct VNEC_D_FP64 *Build_VNEC_D_FP64(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
+    dlay d;
+    if (d_layout_device(a, REF_P, REF_L, 0, &d) != SpB_SUCCESS) return NULL;
+    VNEC_D_FP64 *p = (VNEC_D_FP64 *)calloc(1, sizeof(VNEC_D_FP64));
+    int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (REF_P + 1));
+    int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    hipStream_t s = vnec_stream();
+    SpB_Info rc = (p && co && off && win) ? SpB_SUCCESS : vnec_fail("Build_VNEC_D_FP64: out of memory");
+    if (rc == SpB_SUCCESS) {
+        p->thread_coord_start = (coord *)malloc(sizeof(coord) * REF_P);
+        p->thread_coord_end = (coord *)malloc(sizeof(coord) * REF_P);
+        p->ecr_indices = (int *)calloc((size_t)nnz + 10, sizeof(int));
+        p->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
+        p->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
+        p->ecr_xx_val = (double **)calloc(REF_P + 1, sizeof(double *));
+        p->v_row_ptr = (int *)malloc(sizeof(int) * (size_t)(nr + 1));
+        p->col_start = (int *)malloc(sizeof(int) * (size_t)(d.G ? d.G : 1));
+        if (!p->thread_coord_start || !p->thread_coord_end || !p->ecr_indices || !p->NEC_NUM ||
+            !p->use_x_indices || !p->ecr_xx_val || !p->v_row_ptr || !p->col_start)
+            rc = vnec_fail("Build_VNEC_D_FP64: out of memory");
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(co, d.coords, sizeof(int32_t) * 2 * (REF_P + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(off, d.win_off, sizeof(int64_t) * (REF_P + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && nnz)
+        rc = vnec_hip_ok(hipMemcpyAsync(p->ecr_indices, d.ecr, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(p->v_row_ptr, d.vrp, sizeof(int32_t) * (size_t)(nr + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && d.G)
+        rc = vnec_hip_ok(hipMemcpyAsync(p->col_start, d.col_start, sizeof(int32_t) * (size_t)d.G, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && nnz)
+        rc = vnec_hip_ok(hipMemcpyAsync(win, d.win, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc == SpB_SUCCESS) {
+        for (int t = 0; t < REF_P; t++) {
+            p->thread_coord_start[t].x = co[2 * t];
+            p->thread_coord_start[t].y = co[2 * t + 1];
+            p->thread_coord_end[t].x = co[2 * t + 2];
+            p->thread_coord_end[t].y = co[2 * t + 3];
+            int n = (int)(off[t + 1] - off[t]);
+            p->NEC_NUM[t] = n;
+            p->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
+            p->ecr_xx_val[t] = (double *)malloc(sizeof(double) * (size_t)(n + REF_L));
+            if (p->use_x_indices[t]) memcpy(p->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
+        }
+    }
+    free(co);
+    free(off);
+    free(win);
+    dlay_free(&d);
+    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a) : NULL;
+    if (!g) {
+        if (p) VNEC_D_free_fp64_ext(p);
+        return NULL;
+    }
+    add_wrap(p, KIND_D, g, a);
+    return p;
+}
+
+/* spmv_vnec_S_for_mxv.c:336-588; ECR fields at the snapped partitions */
+struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    int64_t nnz = (int64_t)a->nnz;
+    VNEC_S_FP64 *p = (VNEC_S_FP64 *)calloc(1, sizeof(VNEC_S_FP64));
+    int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
+    int32_t *ecr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    int32_t *nec = (int32_t *)malloc(sizeof(int32_t) * REF_P);
+    int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (REF_P + 1));
+    int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
+    SpB_Info rc = (p && co && ecr && nec && off && win) ? SpB_SUCCESS : vnec_fail("Build_VNEC_S_FP64: out of memory");
+    if (rc == SpB_SUCCESS) rc = vnec_partition_ext(A, REF_P, 1, co);
+    if (rc == SpB_SUCCESS) rc = vnec_ecr_ext(A, REF_P, co, 0, ecr, nec, off, win);
+    if (rc == SpB_SUCCESS) {
+        p->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
+        p->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
+        p->ecr_xx_val = (double **)calloc(REF_P + 1, sizeof(double *));
+        if (!p->NEC_NUM || !p->use_x_indices || !p->ecr_xx_val) rc = vnec_fail("out of memory");
+    }
+    if (rc == SpB_SUCCESS) {
+        for (int t = 0; t < REF_P; t++) {
+            int n = nec[t];
+            p->NEC_NUM[t] = n;
+            p->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
+            p->ecr_xx_val[t] = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
+            if (p->use_x_indices[t]) memcpy(p->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
+        }
+    }
+    free(co);
+    free(ecr);
+    free(nec);
+    free(off);
+    free(win);
+    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a) : NULL;
+    if (!g) {
+        if (p) VNEC_S_free_fp64_ext(p);
+        return NULL;
+    }
+    add_wrap(p, KIND_S, g, a);
+    return p;
+}
+
+/* ------------------------------------------------------------- SpMV */
+static SpB_Info run_wrapped(const void *pub, plan_kind kind, SpB_Vector y, const SpB_Matrix A,
+                            const SpB_Vector x)
+{
+    wrap *w = find_wrap(pub);
+    if (!w || w->kind != kind) return vnec_fail("plan was not made by this library's Build_VNEC_*");
+    if ((const spb_csr *)A != w->A) return vnec_fail("plan was built for another matrix");
+    return vnec_spmv_ext(w->gpu, y, x);
+}
+
+void SpMV_VNEC_L_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_L *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_L, y, A, x);
+}
+
+void SpMV_VNEC_D_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_D_FP64 *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_D, y, A, x);
+}
+
+void SpMV_VNEC_S_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_S_FP64 *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_S, y, A, x);
+}
+
+SpB_Info SpB_mxv(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, void *VNEC)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return SpB_FAILURE;
+    spb_dense *xv = spb_as_dense(x), *yv = spb_as_dense(y);
+    if (!xv || !yv) return SpB_FAILURE;
+    if (xv->base.n != a->base.col || yv->base.n != a->base.row)
+        return vnec_fail("SpB_mxv: dimension mismatch (A %lux%lu, x %lu, y %lu)", a->base.row,
+                         a->base.col, xv->base.n, yv->base.n);
+    if (VNEC) {
+        wrap *w = find_wrap(VNEC);
+        if (w) {
+            if (w->A != a) return vnec_fail("SpB_mxv: plan was built for another matrix");
+            return vnec_spmv_ext(w->gpu, y, x);
+        }
+        struct vnec_plan_s *p = (struct vnec_plan_s *)VNEC;
+        if (p->magic != SPB_MAGIC_PLAN) return vnec_fail("SpB_mxv: unknown plan handle");
+        if (p->A != a) return vnec_fail("SpB_mxv: plan was built for another matrix");
+        return vnec_spmv_ext(p, y, x);
+    }
+    if (!a->auto_plan && vnec_plan_build_ext(&a->auto_plan, A, NULL) != SpB_SUCCESS) return SpB_FAILURE;
+    return vnec_spmv_ext(a->auto_plan, y, x);
+}
+
+/* ---------------------------------------------------------- freeing */
+SpB_Info VNEC_L_free_ext(VNEC_L *l)
+{
+    if (!l) return vnec_fail("VNEC_L_free_ext: NULL plan");
+    struct vnec_plan_s *g = drop_wrap(l);
+    if (g) vnec_plan_free_ext(&g);
+    free(l->diagonal_start);
+    free(l->diagonal_end);
+    free(l->thread_coord_start);
+    free(l->thread_coord_end);
+    free(l->nz_indices);
+    free(l);
+    return SpB_SUCCESS;
+}
+
+SpB_Info VNEC_D_free_fp64_ext(VNEC_D_FP64 *p)
+{
+    if (!p) return vnec_fail("VNEC_D_free_fp64_ext: NULL plan");
+    struct vnec_plan_s *g = drop_wrap(p);
+    if (g) vnec_plan_free_ext(&g);
+    for (int t = 0; t < REF_P; t++) {
+        if (p->use_x_indices) free(p->use_x_indices[t]);
+        if (p->ecr_xx_val) free(p->ecr_xx_val[t]);
+    }
+    free(p->thread_coord_start);
+    free(p->thread_coord_end);
+    free(p->ecr_indices);
+    free(p->NEC_NUM);
+    free(p->use_x_indices);
+    free(p->ecr_xx_val);
+    free(p->v_row_ptr);
+    free(p->col_start);
+    free(p->val_align);
+    free(p);
+    return SpB_SUCCESS;
+}
+
+SpB_Info VNEC_S_free_fp64_ext(VNEC_S_FP64 *p)
+{
+    if (!p) return vnec_fail("VNEC_S_free_fp64_ext: NULL plan");
+    struct vnec_plan_s *g = drop_wrap(p);
+    if (g) vnec_plan_free_ext(&g);
+    for (int t = 0; t < REF_P; t++) {
+        if (p->use_x_indices) free(p->use_x_indices[t]);
+        if (p->ecr_xx_val) free(p->ecr_xx_val[t]);
+        if (p->tasks) free(p->tasks[t]);
+    }
+    free(p->NEC_NUM);
+    free(p->use_x_indices);
+    free(p->ecr_xx_val);
+    free(p->spvv8_len);
+    free(p->tasks);
+    free(p->reorder_mat.nnz);
+    free(p->reorder_mat.col);
+    free(p->reorder_mat.row_begin);
+    free(p->reorder_mat.row_end);
+    free(p->reorder_mat.task_start);
+    free(p->reorder_mat.task_end);
+    free(p);
+    return SpB_SUCCESS;
+}

@@ -1,0 +1,311 @@
+"""Python mirror of the VNEC path of the OpenSparseBLAS API, over the HIP
+C-ABI of libvnec_mi355x.so.
+
+Names follow the reference (SpB_Matrix / SpB_Vector objects, IRD_VNEC,
+Build_VNEC_* / SpMV_VNEC_*_FP64, SpB_mxv); errors raise RuntimeError with
+the library's reason.  Everything runs on the GPU through the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+SpB_FP64 = 11
+SpB_MATRIX_FORMAT = 1
+SpB_CSR = 1
+VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3}
+VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2}
+
+_init = False
+
+
+def init():
+    global _init
+    lib = L.load()
+    if not _init:
+        lib.SpB_init()
+        f = L.SpB_Object_Format_ext()
+        f.matrix_format = SpB_CSR
+        L.check(lib.SpB_Global_Option_set_ext(SpB_MATRIX_FORMAT, f), "SpB_Global_Option_set_ext")
+        _init = True
+    return lib
+
+
+def device_count() -> int:
+    return int(L.load().vnec_device_count_ext())
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data
+
+
+class Matrix:
+    """SpB_Matrix (CSR, FP64)."""
+
+    def __init__(self, nrows: int, ncols: int):
+        lib = init()
+        self.h = C.c_void_p()
+        L.check(lib.SpB_Matrix_new(C.byref(self.h), SpB_FP64, nrows, ncols), "SpB_Matrix_new")
+        self.nrows, self.ncols = nrows, ncols
+        self.nnz = 0
+
+    @classmethod
+    def from_csr(cls, nrows, ncols, ptr, col, val) -> "Matrix":
+        m = cls(nrows, ncols)
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        col = np.ascontiguousarray(col, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        L.check(L.load().SpB_Matrix_import_csr_ext(m.h, _p(ptr), _p(col), _p(val)),
+                "SpB_Matrix_import_csr_ext")
+        m.nnz = int(ptr[-1])
+        return m
+
+    @classmethod
+    def from_coo(cls, nrows, ncols, rows, cols, vals) -> "Matrix":
+        m = cls(nrows, ncols)
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        c = np.ascontiguousarray(cols, dtype=np.uint64)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        L.check(L.load().SpB_Matrix_build_FP64(m.h, _p(r), _p(c), _p(v), len(v), 0),
+                "SpB_Matrix_build_FP64")
+        m.nnz = len(v)
+        return m
+
+    def free(self):
+        if self.h:
+            L.check(L.load().SpB_Matrix_free(C.byref(self.h)), "SpB_Matrix_free")
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.load().SpB_Matrix_free(C.byref(self.h))
+        except Exception:
+            pass
+
+
+class Vector:
+    """SpB_Vector (dense, FP64)."""
+
+    def __init__(self, n: int):
+        lib = init()
+        self.h = C.c_void_p()
+        L.check(lib.SpB_Vector_new(C.byref(self.h), SpB_FP64, n), "SpB_Vector_new")
+        self.n = n
+
+    @classmethod
+    def from_array(cls, a) -> "Vector":
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        v = cls(len(a))
+        L.check(L.load().SpB_Vector_import_ext(v.h, _p(a)), "SpB_Vector_import_ext")
+        return v
+
+    def to_array(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float64)
+        L.check(L.load().SpB_Vector_export_ext(self.h, _p(out)), "SpB_Vector_export_ext")
+        return out
+
+    def set(self, i: int, x: float):
+        L.check(L.load().SpB_Vector_setElement_FP64(self.h, float(x), i),
+                "SpB_Vector_setElement_FP64")
+
+    def get(self, i: int) -> float:
+        out = C.c_double()
+        L.check(L.load().SpB_Vector_extractElement_FP64(C.byref(out), self.h, i),
+                "SpB_Vector_extractElement_FP64")
+        return out.value
+
+    def resident(self, on: bool = True):
+        L.check(L.load().SpB_Vector_device_resident_ext(self.h, on), "device_resident")
+
+    def device_ptr(self) -> int:
+        p = L.load().SpB_Vector_device_ptr_ext(self.h)
+        if not p:
+            raise RuntimeError(L.last_error())
+        return p
+
+    def free(self):
+        if self.h:
+            L.check(L.load().SpB_Vector_free(C.byref(self.h)), "SpB_Vector_free")
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.load().SpB_Vector_free(C.byref(self.h))
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A raw device buffer owned by the library's allocator."""
+
+    def __init__(self, nbytes: int):
+        init()
+        self.nbytes = nbytes
+        self.ptr = L.load().vnec_dev_alloc_ext(nbytes)
+        if not self.ptr:
+            raise RuntimeError(L.last_error())
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        L.check(L.load().vnec_memcpy_h2d_ext(d.ptr, _p(a), a.nbytes), "h2d")
+        return d
+
+    def to_array(self, dtype, n) -> np.ndarray:
+        out = np.empty(n, dtype=dtype)
+        L.check(L.load().vnec_memcpy_d2h_ext(_p(out), self.ptr, out.nbytes), "d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            L.load().vnec_dev_free_ext(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class TuningInfo:
+    variant: str
+    ipt: int
+    lanes: int
+    encoding: str
+    nrhs: int
+    ntiles: int
+    dev_bytes: int
+
+
+class Plan:
+    """GPU execution plan (vnec_plan_build_ext)."""
+
+    def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
+                 encoding: str = "fp64", nrhs: int = 1):
+        init()
+        t = L.Tuning(VNEC_VAR[variant] if variant else -1, ipt, lanes, VNEC_ENC[encoding], nrhs)
+        self.h = C.c_void_p()
+        self.A = A
+        L.check(L.load().vnec_plan_build_ext(C.byref(self.h), A.h, C.byref(t)),
+                "vnec_plan_build_ext")
+
+    def info(self) -> TuningInfo:
+        t = L.Tuning()
+        nt, db = C.c_int64(), C.c_int64()
+        L.check(L.load().vnec_plan_info_ext(self.h, C.byref(t), C.byref(nt), C.byref(db)),
+                "vnec_plan_info_ext")
+        inv = {v: k for k, v in VNEC_VAR.items()}
+        inve = {v: k for k, v in VNEC_ENC.items()}
+        return TuningInfo(inv[t.variant], t.ipt, t.lanes, inve[t.encoding], t.nrhs, nt.value,
+                          db.value)
+
+    def spmv(self, y: Vector, x: Vector):
+        L.check(L.load().vnec_spmv_ext(self.h, y.h, x.h), "vnec_spmv_ext")
+
+    def spmv_dev(self, x_ptr: int, y_ptr: int):
+        L.check(L.load().vnec_spmv_dev_ext(self.h, x_ptr, y_ptr), "vnec_spmv_dev_ext")
+
+    def time(self, x_ptr: int, y_ptr: int, iters: int, graph: bool = True,
+             flush_bytes: int = 0) -> float:
+        ms = C.c_double()
+        L.check(L.load().vnec_time_dev_ext(self.h, x_ptr, y_ptr, iters, int(graph), flush_bytes,
+                                           C.byref(ms)), "vnec_time_dev_ext")
+        return ms.value
+
+    def time_kernels(self, x_ptr: int, y_ptr: int, iters: int) -> tuple[float, float]:
+        """(SpMV kernel ms, carry fix-up ms), each between its own events."""
+        a, b = C.c_double(), C.c_double()
+        L.check(L.load().vnec_time_kernels_ext(self.h, x_ptr, y_ptr, iters, C.byref(a),
+                                               C.byref(b)), "vnec_time_kernels_ext")
+        return a.value, b.value
+
+    def free(self):
+        if self.h:
+            L.check(L.load().vnec_plan_free_ext(C.byref(self.h)), "vnec_plan_free_ext")
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            if self.h:
+                L.load().vnec_plan_free_ext(C.byref(self.h))
+        except Exception:
+            pass
+
+
+# ------------------------------------------------ reference entry points
+def ird_vnec(A: Matrix) -> int:
+    return int(L.load().IRD_VNEC(A.h, SpB_FP64))
+
+
+def ird_ext(A: Matrix, P=36, L_=8, thr=0.477) -> tuple[int, float]:
+    r, d = C.c_float(), C.c_int()
+    L.check(L.load().vnec_ird_ext(A.h, P, L_, thr, C.byref(r), C.byref(d)), "vnec_ird_ext")
+    return d.value, r.value
+
+
+def build_vnec(kind: str, A: Matrix):
+    f = {"L": "Build_VNEC_L", "D": "Build_VNEC_D_FP64", "S": "Build_VNEC_S_FP64"}[kind]
+    p = getattr(L.load(), f)(A.h)
+    if not p:
+        raise RuntimeError(f"{f}: {L.last_error()}")
+    return p
+
+
+def spmv_vnec(kind: str, y: Vector, A: Matrix, x: Vector, plan):
+    f = {"L": "SpMV_VNEC_L_FP64", "D": "SpMV_VNEC_D_FP64", "S": "SpMV_VNEC_S_FP64"}[kind]
+    getattr(L.load(), f)(y.h, A.h, x.h, C.cast(plan, C.c_void_p))
+    err = L.last_error()
+    return err
+
+
+def free_vnec(kind: str, plan):
+    f = {"L": "VNEC_L_free_ext", "D": "VNEC_D_free_fp64_ext", "S": "VNEC_S_free_fp64_ext"}[kind]
+    L.check(getattr(L.load(), f)(C.cast(plan, C.c_void_p)), f)
+
+
+def mxv(y: Vector, A: Matrix, x: Vector, plan=None):
+    p = None if plan is None else (plan.h if isinstance(plan, Plan) else C.cast(plan, C.c_void_p))
+    L.check(L.load().SpB_mxv(y.h, A.h, x.h, p), "SpB_mxv")
+
+
+# ------------------------------------------------ builder structures
+def partition(A: Matrix, P: int, snap: bool = False) -> np.ndarray:
+    out = np.zeros((P + 1, 2), dtype=np.int32)
+    L.check(L.load().vnec_partition_ext(A.h, P, int(snap), _p(out)), "vnec_partition_ext")
+    return out
+
+
+def ecr(A: Matrix, coords: np.ndarray, method: int = 0):
+    coords = np.ascontiguousarray(coords, dtype=np.int32)
+    P = coords.shape[0] - 1
+    nnz = A.nnz
+    e = np.zeros(max(nnz, 1), np.int32)
+    nec = np.zeros(P, np.int32)
+    off = np.zeros(P + 1, np.int64)
+    win = np.zeros(max(nnz, 1), np.int32)
+    L.check(L.load().vnec_ecr_ext(A.h, P, _p(coords), method, _p(e), _p(nec), _p(off), _p(win)),
+            "vnec_ecr_ext")
+    return e[:nnz], nec, off, win[: int(off[-1])]
+
+
+def d_layout(A: Matrix, P: int, L_: int, with_values: bool = True):
+    G = C.c_int64()
+    L.check(L.load().vnec_d_layout_ext(A.h, P, L_, None, None, None, C.byref(G)),
+            "vnec_d_layout_ext")
+    g = G.value
+    vrp = np.zeros(A.nrows + 1, np.int32)
+    cs = np.zeros(max(g, 1), np.int32)
+    va = np.zeros(max(g * L_, 1), np.float64) if with_values else None
+    L.check(L.load().vnec_d_layout_ext(A.h, P, L_, _p(vrp), _p(cs),
+                                       _p(va) if with_values else None, C.byref(G)),
+            "vnec_d_layout_ext")
+    return vrp, cs[:g], (va[: g * L_] if with_values else None)
