@@ -64,6 +64,9 @@ typedef struct {
     int lanes;    /* D block width / S slice height                        */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV (1, or 2..16)               */
+    int grid;     /* 0: persistent, one workgroup per resident slot;
+                     > 0: persistent with this many workgroups (multiple
+                     of 8); -1: one workgroup per tile                     */
 } vnec_tuning;
 
 typedef struct vnec_plan_s *vnec_plan;
@@ -91,6 +94,10 @@ SpB_Info vnec_time_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev, i
  * each timed between its own HIP events on the library stream. */
 SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
                                double *ms_main, double *ms_fixup);
+
+/* Measured streaming-copy bandwidth: bytes read + bytes written per
+ * second over iters copies of a `bytes`-sized buffer (GB/s). */
+SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps);
 
 /* ------------------------------------------------- device memory helpers */
 void *vnec_dev_alloc_ext(size_t bytes);

@@ -91,8 +91,25 @@ int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
  * >= nrows, the carries of empty trailing tiles, are skipped). */
 int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, int64_t ntiles, double *y,
                          int64_t nrows, int nrhs, void *stream);
+/* Persistent pipelined form (one right-hand side): nwg workgroups (a
+ * multiple of 8) each own a contiguous run of tiles; each leaves the
+ * partial row at its end in wg_row / wg_carry (nwg entries) for
+ * vnec_hip_wg_fixup.  vnec_hip_stream_grid gives the resident workgroup
+ * count (occupancy x CUs, rounded down to a multiple of 8). */
+int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg);
+/* The pipelined variant-E kernel holds at most this many window runs per
+ * tile (one per thread); plans with a larger tile fall back to one
+ * workgroup per tile. */
+#define VNEC_E_STREAM_MAX_RUNS 256
+int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *wg_row, double *wg_carry,
+                         void *stream);
+int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, double *y,
+                      int64_t nrows, void *stream);
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
+
+/* Streaming copy of bytes (multiple of 16) from src to dst. */
+int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
 
 /* Overwrites n bytes (cache flush helper for timing). */
 int vnec_hip_scribble(void *p, size_t n, void *stream);

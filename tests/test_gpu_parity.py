@@ -31,8 +31,8 @@ def _mat(m):
     return spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
 
 
-def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1):
-    p = spb.Plan(A, variant, ipt, encoding=enc, nrhs=nrhs)
+def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1, grid=0):
+    p = spb.Plan(A, variant, ipt, encoding=enc, nrhs=nrhs, grid=grid)
     xv = spb.Vector.from_array(x)
     yv = spb.Vector(A.nrows * nrhs)
     p.spmv(yv, xv)
@@ -68,10 +68,13 @@ CASES = _cases()
 
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 15), ("E", 5), ("E", 11)])
-def test_spmv_fp64_matches_oracle(name, m, variant, ipt):
+@pytest.mark.parametrize("grid", [0, 8, -1])
+def test_spmv_fp64_matches_oracle(name, m, variant, ipt, grid):
+    """grid 0: persistent pipelined kernel, 8: persistent with few workgroups
+    (long tile runs and carries across many tiles), -1: one workgroup per tile."""
     A = _mat(m)
-    print(f"case {name} variant {variant} ipt {ipt}", flush=True)
-    y = _run_plan(A, m.x, variant, ipt)
+    print(f"case {name} variant {variant} ipt {ipt} grid {grid}", flush=True)
+    y = _run_plan(A, m.x, variant, ipt, grid=grid)
     y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
     ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     ok, worst = O.check_rows(y, y0, ax, TOL)

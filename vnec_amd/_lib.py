@@ -21,7 +21,7 @@ Info = C.c_int
 
 class Tuning(C.Structure):
     _fields_ = [("variant", C.c_int), ("ipt", C.c_int), ("lanes", C.c_int),
-                ("encoding", C.c_int), ("nrhs", C.c_int)]
+                ("encoding", C.c_int), ("nrhs", C.c_int), ("grid", C.c_int)]
 
 
 class Coord(C.Structure):
@@ -104,6 +104,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_double)]),
     "vnec_time_kernels_ext": (Info, [vp, vp, vp, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    "vnec_copy_bandwidth_ext": (Info, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "vnec_dev_alloc_ext": (vp, [C.c_size_t]),
     "vnec_dev_free_ext": (Info, [vp]),
     "vnec_memcpy_h2d_ext": (Info, [vp, vp, C.c_size_t]),
@@ -131,6 +132,10 @@ SIGNATURES = {
     "vnec_hip_spmv_merge": (C.c_int, [vp, vp]),
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "vnec_hip_merge_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
+    "vnec_hip_stream_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "vnec_hip_spmv_stream": (C.c_int, [vp, C.c_int, vp, vp, vp]),
+    "vnec_hip_wg_fixup": (C.c_int, [vp, vp, C.c_int, vp, i64, vp]),
+    "vnec_hip_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
 }
 

@@ -414,3 +414,26 @@ extern "C" int vnec_hip_scribble(void *p, size_t n, void *stream)
                        salt);
     return (int)hipGetLastError();
 }
+
+// ------------------------------------------------------------ copy probe
+// Streaming copy of n16 16-byte words: the measured HBM ceiling that the
+// SpMV kernels are compared with, and the calibration kernel for the
+// hardware byte counters (read n bytes, write n bytes per launch).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4 *__restrict__ src,
+                                                     u32x4 *__restrict__ dst, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (int64_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+extern "C" int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream)
+{
+    int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 <= 0) return 0;
+    hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, (hipStream_t)stream,
+                       (const u32x4 *)src, (u32x4 *)dst, n16);
+    return (int)hipGetLastError();
+}

@@ -125,6 +125,47 @@ static inline int64_t xcd_grid(int64_t ntiles)
     return (ntiles + VNEC_XCDS - 1) / VNEC_XCDS * VNEC_XCDS;
 }
 
+// Gathers a tile's compressed x window (its distinct columns, stored as
+// runs of consecutive columns: run r starts at column run_start[r] and
+// window position run_wpos[r]) into s_win[0 .. nec).
+// Long runs: wavefront w copies runs w, w + 4, ..., lanes reading
+// consecutive x entries (coalesced; run descriptors are wave-uniform, read
+// with scalar loads).  Short runs (the window of a scattered tile): one
+// lane per window position, finding its run by binary search over the run
+// descriptors staged in s_runs (2 * nr ints of LDS).
+__device__ __forceinline__ void fill_window(const double *__restrict__ x,
+                                            const int32_t *__restrict__ run_start,
+                                            const int32_t *__restrict__ run_wpos, int r0, int r1,
+                                            int nec, double *s_win, int *s_runs)
+{
+    const int nr = r1 - r0;
+    if (nr <= 0) return;
+    if (nec >= 16 * nr) {
+        const int w = __builtin_amdgcn_readfirstlane(wave_id());
+        for (int r = r0 + w; r < r1; r += VNEC_NWAVES) {
+            const int start = run_start[r], wpos = run_wpos[r];
+            int end = nec;
+            if (r + 1 < r1) end = run_wpos[r + 1];
+            for (int k = lane_id(); k < end - wpos; k += VNEC_WAVE) s_win[wpos + k] = x[start + k];
+        }
+    } else {
+        int *s_rs = s_runs, *s_rw = s_runs + nr;
+        for (int i = threadIdx.x; i < nr; i += VNEC_BLOCK) {
+            s_rs[i] = run_start[r0 + i];
+            s_rw[i] = run_wpos[r0 + i];
+        }
+        __syncthreads();
+        for (int p = threadIdx.x; p < nec; p += VNEC_BLOCK) {
+            int lo = 0, hi = nr - 1;
+            while (lo < hi) {
+                int mid = (lo + hi + 1) >> 1;
+                if (s_rw[mid] <= p) lo = mid; else hi = mid - 1;
+            }
+            s_win[p] = x[s_rs[lo] + (p - s_rw[lo])];
+        }
+    }
+}
+
 // Merge-path search (spmv_vnec_common.c:3-33 semantics): merge list A =
 // row ends re[0..na), list B = b0 + 0..nb; returns the number of A items
 // consumed on diagonal d, consuming A first on ties (A[i] <= B[j]).

@@ -80,24 +80,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     const int tid = threadIdx.x;
 
     if (WIN) {
-        // 1. window fill: runs to LDS (aliasing s_prod), then x gathered
-        int *s_rs = (int *)s_prod;
-        int *s_rw = s_rs + T;
-        const int r0 = a.run_off[t], nr = a.run_off[t + 1] - r0;
-        const int nec = a.nec[t];
-        for (int i = tid; i < nr; i += VNEC_BLOCK) {
-            s_rs[i] = a.run_start[r0 + i];
-            s_rw[i] = a.run_wpos[r0 + i];
-        }
-        __syncthreads();
-        for (int p = tid; p < nec; p += VNEC_BLOCK) {
-            int lo = 0, hi = nr - 1;  // last run with wpos <= p
-            while (lo < hi) {
-                int mid = (lo + hi + 1) >> 1;
-                if (s_rw[mid] <= p) lo = mid; else hi = mid - 1;
-            }
-            s_win[p] = a.x[s_rs[lo] + (p - s_rw[lo])];
-        }
+        // 1. window fill (run descriptors staged in s_prod when needed)
+        fill_window(a.x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], a.nec[t], s_win,
+                    (int *)s_prod);
         __syncthreads();
     }
 

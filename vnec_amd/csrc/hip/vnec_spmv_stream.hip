@@ -1,0 +1,485 @@
+// vnec_spmv_stream.hip — persistent, software-pipelined merge-path FP64 SpMV
+// (variants L and E, one right-hand side).
+//
+// Each workgroup owns one contiguous run of merge-path tiles (workgroups on
+// one XCD own neighbouring runs, so the x entries they share stay in that
+// XCD's L2).  While a tile is reduced out of LDS, the loads of the next
+// tile (values, columns, row ends, window runs) are already in flight in
+// registers, so HBM never waits for the LDS phases.  The partial row at the
+// end of a tile flows into the next tile of the same workgroup as the
+// carry-in of its first thread; only the last partial row of each
+// workgroup is left for the fix-up kernel, which adds them in workgroup
+// order (deterministic; cf. spmv_vnec_L_for_mxv.c:160-164).
+#include <hip/hip_runtime.h>
+
+#include "vnec_common.h"
+#include "vnec_hip.h"
+
+template <int ENC>
+__device__ __forceinline__ double ld_value(const vnec_merge_args &a, int64_t j)
+{
+    if (ENC == 0) return __builtin_nontemporal_load(a.val + j);
+    if (ENC == 1)
+        return (double)__builtin_nontemporal_load(a.hi + j) +
+               (double)__builtin_nontemporal_load(a.lo + j);
+    return (double)__builtin_nontemporal_load(a.hi + j);
+}
+
+// Registers holding one tile's streamed data (striped: item u * 256 + tid).
+template <int IPT, bool WIN>
+struct TileRegs {
+    double v[IPT];
+    int c[IPT];
+    int re[IPT];
+};
+
+template <int IPT, bool WIN, int ENC>
+__device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int2 c1,
+                                          TileRegs<IPT, WIN> &R)
+{
+    const int tid = threadIdx.x;
+    const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        const int k = u * VNEC_BLOCK + tid;
+        if (k < nz) {
+            const int64_t j = (int64_t)c0.y + k;
+            R.v[u] = ld_value<ENC>(a, j);
+            R.c[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
+                         : __builtin_nontemporal_load(a.col + j);
+        }
+        if (k < nrow) R.re[u] = a.ptr[c0.x + 1 + k];
+    }
+}
+
+template <int IPT, bool WIN, int ENC>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, int nwg,
+                                                            int32_t *__restrict__ wg_row,
+                                                            double *__restrict__ wg_carry)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_prod[T];
+    __shared__ int s_rowend[T];
+    __shared__ double s_win[WIN ? T : 1];
+    __shared__ int s_f[VNEC_NWAVES + 1];
+    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ double s_carry;
+
+    const int tid = threadIdx.x;
+    const int rank = (int)xcd_tile(blockIdx.x, nwg);
+    const int64_t tb = (int64_t)rank * a.ntiles / nwg;
+    const int64_t te = (int64_t)(rank + 1) * a.ntiles / nwg;
+    const int2 *co = (const int2 *)a.coords;
+
+    if (tid == 0) s_carry = 0.0;
+    if (tb >= te) {
+        // An empty run reports the row at its boundary with a zero carry, so
+        // the workgroups carrying into one row stay adjacent in rank order
+        // (the fix-up relies on it to add each row's carries exactly once).
+        if (tid == 0) {
+            wg_row[rank] = co[tb].x;
+            wg_carry[rank] = 0.0;
+        }
+        return;
+    }
+
+    // pipeline registers: current tile (A0, A1), next tile end (B1)
+    // (plain if / else, not ?: — a select between a load and a register
+    // would be lowered through a private-memory address)
+    int2 A0 = co[tb], A1 = co[tb + 1];
+    int2 B1 = A1;
+    if (tb + 1 < te) B1 = co[tb + 2];
+    int ra0 = 0, ra1 = 0, rb1 = 0, nec_a = 0, nec_b = 0;
+    if (WIN) {
+        ra0 = a.run_off[tb];
+        ra1 = a.run_off[tb + 1];
+        rb1 = ra1;
+        nec_a = a.nec[tb];
+        if (tb + 1 < te) {
+            rb1 = a.run_off[tb + 2];
+            nec_b = a.nec[tb + 1];
+        }
+    }
+    TileRegs<IPT, WIN> cur, nxt;
+    load_tile<IPT, WIN, ENC>(a, A0, A1, cur);
+    __syncthreads();
+
+    for (int64_t t = tb; t < te; t++) {
+        // --- issue the next tile's loads, and the coordinates after it
+        int2 B2 = B1;
+        int rb2 = rb1, nec_c = 0;
+        if (t + 2 < te) {
+            B2 = co[t + 3];
+            if (WIN) {
+                rb2 = a.run_off[t + 3];
+                nec_c = a.nec[t + 2];
+            }
+        }
+        if (t + 1 < te) load_tile<IPT, WIN, ENC>(a, A1, B1, nxt);
+
+        const int x0 = A0.x, y0 = A0.y;
+        const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
+
+        // --- (E) compressed x window of this tile into LDS
+        if (WIN) {
+            fill_window(a.x, a.run_start, a.run_wpos, ra0, ra1, nec_a, s_win, (int *)s_prod);
+            __syncthreads();
+        }
+
+        // --- row ends and products into LDS
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int k = u * VNEC_BLOCK + tid;
+            if (k < nrow) s_rowend[k] = cur.re[u];
+            if (k < nz) s_prod[k] = cur.v[u] * (WIN ? s_win[cur.c[u]] : a.x[cur.c[u]]);
+        }
+        __syncthreads();
+
+        // --- thread-level merge walk
+        const int items = nrow + nz;
+        const int d0 = tid * IPT < items ? tid * IPT : items;
+        const int d1 = d0 + IPT < items ? d0 + IPT : items;
+        int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+        int ny = d0 - rx;
+        double acc = 0.0, head_val = 0.0;
+        int flag = 0, head_row = 0;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            if (d0 + u < d1) {
+                if (rx < nrow && s_rowend[rx] <= y0 + ny) {
+                    if (!flag) {
+                        head_row = x0 + rx;
+                        head_val = acc;
+                        flag = 1;
+                    } else {
+                        a.y[x0 + rx] = acc;
+                    }
+                    acc = 0.0;
+                    rx++;
+                } else {
+                    acc += s_prod[ny];
+                    ny++;
+                }
+            }
+        }
+
+        // --- segmented join across threads, carry-in from the previous tile
+        int f = flag;
+        double v = acc;
+        wave_seg_scan_f64(f, v);
+        const int w = wave_id();
+        if (lane_id() == VNEC_WAVE - 1) {
+            s_f[w] = f;
+            s_v[w] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double c = s_carry;
+            for (int k = 0; k < VNEC_NWAVES; k++) {
+                double tw = s_v[k];
+                int fw = s_f[k];
+                s_v[k] = c;
+                c = fw ? tw : c + tw;
+            }
+            s_v[VNEC_NWAVES] = c;  // this tile's carry-out
+        }
+        __syncthreads();
+        const double cw = s_v[w];
+        const double blk = f ? v : v + cw;
+        const double cin = wave_shift_up1_f64(blk, cw);
+        if (flag) a.y[head_row] = head_val + cin;
+        if (tid == 0) s_carry = s_v[VNEC_NWAVES];
+        __syncthreads();
+
+        // --- rotate the pipeline
+        cur = nxt;
+        A0 = A1;
+        A1 = B1;
+        B1 = B2;
+        if (WIN) {
+            ra0 = ra1;
+            ra1 = rb1;
+            rb1 = rb2;
+            nec_a = nec_b;
+            nec_b = nec_c;
+        }
+    }
+    if (tid == 0) {
+        wg_row[rank] = A0.x;  // end row of the last tile
+        wg_carry[rank] = s_carry;
+    }
+}
+
+// ----------------------------------------------------------- variant E
+// Pipelined form of the compressed-window kernel.  In the iteration that
+// reduces tile t, the workgroup also
+//   - loads the run descriptors of tile t + 2 (one run per thread),
+//   - loads the values / local columns / row ends of tile t + 1,
+//   - gathers tile t + 1's x window into registers (each thread finds the
+//     run of its window positions in the descriptors staged in LDS one
+//     iteration earlier),
+// so that the only work on a tile's critical path is in LDS.  A tile may
+// hold at most VNEC_ERUNS runs (the plan falls back to the one-workgroup-
+// per-tile kernel otherwise).
+#define VNEC_ERUNS VNEC_BLOCK
+static_assert(VNEC_ERUNS == VNEC_E_STREAM_MAX_RUNS, "one run descriptor per thread");
+
+__device__ __forceinline__ int find_run(const int2 *runs, int nr, int p)
+{
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (runs[mid].y <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int IPT>
+struct ERegs {
+    double v[IPT];
+    double xw[IPT];
+    int c[IPT];
+    int re[IPT];
+};
+
+template <int IPT, int ENC>
+__device__ __forceinline__ void e_load(const vnec_merge_args &a, int2 c0, int2 c1, int nec,
+                                       const int2 *runs, int nr, ERegs<IPT> &R)
+{
+    const int tid = threadIdx.x;
+    const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        const int k = u * VNEC_BLOCK + tid;
+        if (k < nz) {
+            const int64_t j = (int64_t)c0.y + k;
+            R.v[u] = ld_value<ENC>(a, j);
+            R.c[u] = (int)__builtin_nontemporal_load(a.lcol + j);
+        }
+        if (k < nrow) R.re[u] = a.ptr[c0.x + 1 + k];
+        if (k < nec) {
+            const int2 r = runs[find_run(runs, nr, k)];
+            R.xw[u] = a.x[r.x + (k - r.y)];
+        }
+    }
+}
+
+template <int IPT, int ENC>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_estream(vnec_merge_args a, int nwg,
+                                                             int32_t *__restrict__ wg_row,
+                                                             double *__restrict__ wg_carry)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_prod[T];
+    __shared__ int s_rowend[T];
+    __shared__ double s_win[T];
+    __shared__ int2 s_runs[2][VNEC_ERUNS];
+    __shared__ int s_f[VNEC_NWAVES + 1];
+    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ double s_carry;
+
+    const int tid = threadIdx.x;
+    const int rank = (int)xcd_tile(blockIdx.x, nwg);
+    const int64_t tb = (int64_t)rank * a.ntiles / nwg;
+    const int64_t te = (int64_t)(rank + 1) * a.ntiles / nwg;
+    const int2 *co = (const int2 *)a.coords;
+
+    if (tid == 0) s_carry = 0.0;
+    if (tb >= te) {
+        if (tid == 0) {
+            wg_row[rank] = co[tb].x;
+            wg_carry[rank] = 0.0;
+        }
+        return;
+    }
+
+    // scalars of tiles t .. t + 3 (if / else only: see the L kernel)
+    int2 cA = co[tb], cB = co[tb + 1], cC = cB;
+    int roA = a.run_off[tb], roB = a.run_off[tb + 1], roC = roB, roD = roB;
+    int necA = a.nec[tb], necB = 0;
+    if (tb + 1 < te) {
+        cC = co[tb + 2];
+        roC = a.run_off[tb + 2];
+        roD = roC;
+        necB = a.nec[tb + 1];
+    }
+    if (tb + 2 < te) roD = a.run_off[tb + 3];
+    // run descriptors of tiles tb and tb + 1
+    if (tid < roB - roA) s_runs[tb & 1][tid] = make_int2(a.run_start[roA + tid], a.run_wpos[roA + tid]);
+    if (tb + 1 < te && tid < roC - roB)
+        s_runs[(tb + 1) & 1][tid] = make_int2(a.run_start[roB + tid], a.run_wpos[roB + tid]);
+    __syncthreads();
+    ERegs<IPT> cur, nxt;
+    e_load<IPT, ENC>(a, cA, cB, necA, s_runs[tb & 1], roB - roA, cur);
+
+    for (int64_t t = tb; t < te; t++) {
+        // scalars for the next iteration
+        int2 cD = cC;
+        int roE = roD, necC = 0;
+        if (t + 2 < te) {
+            cD = co[t + 3];
+            necC = a.nec[t + 2];
+        }
+        if (t + 3 < te) roE = a.run_off[t + 4];
+        // run descriptors of tile t + 2 (registers until the end of the iteration)
+        int2 run2 = make_int2(0, 0);
+        const bool has2 = t + 2 < te && tid < roD - roC;
+        if (has2) run2 = make_int2(a.run_start[roC + tid], a.run_wpos[roC + tid]);
+        // tile t + 1: streamed data and its x window
+        if (t + 1 < te) e_load<IPT, ENC>(a, cB, cC, necB, s_runs[(t + 1) & 1], roC - roB, nxt);
+
+        // --- reduce tile t
+        const int x0 = cA.x, y0 = cA.y;
+        const int nrow = cB.x - cA.x, nz = cB.y - cA.y;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int k = u * VNEC_BLOCK + tid;
+            if (k < necA) s_win[k] = cur.xw[u];
+            if (k < nrow) s_rowend[k] = cur.re[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int k = u * VNEC_BLOCK + tid;
+            if (k < nz) s_prod[k] = cur.v[u] * s_win[cur.c[u]];
+        }
+        __syncthreads();
+        const int items = nrow + nz;
+        const int d0 = tid * IPT < items ? tid * IPT : items;
+        const int d1 = d0 + IPT < items ? d0 + IPT : items;
+        int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+        int ny = d0 - rx;
+        double acc = 0.0, head_val = 0.0;
+        int flag = 0, head_row = 0;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            if (d0 + u < d1) {
+                if (rx < nrow && s_rowend[rx] <= y0 + ny) {
+                    if (!flag) {
+                        head_row = x0 + rx;
+                        head_val = acc;
+                        flag = 1;
+                    } else {
+                        a.y[x0 + rx] = acc;
+                    }
+                    acc = 0.0;
+                    rx++;
+                } else {
+                    acc += s_prod[ny];
+                    ny++;
+                }
+            }
+        }
+        int f = flag;
+        double v = acc;
+        wave_seg_scan_f64(f, v);
+        const int w = wave_id();
+        if (lane_id() == VNEC_WAVE - 1) {
+            s_f[w] = f;
+            s_v[w] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double c = s_carry;
+            for (int k = 0; k < VNEC_NWAVES; k++) {
+                double tw = s_v[k];
+                int fw = s_f[k];
+                s_v[k] = c;
+                c = fw ? tw : c + tw;
+            }
+            s_v[VNEC_NWAVES] = c;
+        }
+        __syncthreads();
+        const double cw = s_v[w];
+        const double blk = f ? v : v + cw;
+        const double cin = wave_shift_up1_f64(blk, cw);
+        if (flag) a.y[head_row] = head_val + cin;
+        if (tid == 0) s_carry = s_v[VNEC_NWAVES];
+        // tile t + 2's runs go to the buffer tile t used
+        if (has2) s_runs[t & 1][tid] = run2;
+        __syncthreads();
+
+        cur = nxt;
+        cA = cB;
+        cB = cC;
+        cC = cD;
+        roA = roB;
+        roB = roC;
+        roC = roD;
+        roD = roE;
+        necA = necB;
+        necB = necC;
+    }
+    if (tid == 0) {
+        wg_row[rank] = cA.x;
+        wg_carry[rank] = s_carry;
+    }
+}
+
+// Adds each workgroup's carry to its row: the first workgroup of each run
+// carrying into one row sums the run in order and adds it once.
+__global__ void k_wg_fixup(const int32_t *__restrict__ wg_row, const double *__restrict__ wg_carry,
+                           int nwg, double *__restrict__ y, int64_t nrows)
+{
+    int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwg) return;
+    int row = wg_row[w];
+    if (row >= nrows) return;
+    if (w > 0 && wg_row[w - 1] == row) return;
+    double s = 0.0;
+    for (int u = w; u < nwg && wg_row[u] == row; u++) s += wg_carry[u];
+    y[row] += s;
+}
+
+typedef void (*stream_kernel_t)(vnec_merge_args, int, int32_t *, double *);
+
+static stream_kernel_t pick_stream(int variant, int ipt, int enc)
+{
+#define SCASE(I, E)                                                  \
+    if (ipt == I && enc == E) {                                      \
+        if (variant == 0) return k_spmv_stream<I, false, E>;         \
+        if (variant == 1) return k_spmv_estream<I, E>;               \
+    }
+#define SIPT(I) SCASE(I, 0) SCASE(I, 1) SCASE(I, 2)
+    SIPT(3) SIPT(5) SIPT(7) SIPT(9) SIPT(11) SIPT(13) SIPT(15)
+    return nullptr;
+#undef SIPT
+#undef SCASE
+}
+
+extern "C" int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg)
+{
+    stream_kernel_t k = (variant == 0 || variant == 1) ? pick_stream(variant, ipt, encoding) : nullptr;
+    if (!k) return (int)hipErrorInvalidValue;
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, VNEC_BLOCK, 0);
+    if (e != hipSuccess) return (int)e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return (int)e;
+    int n = per_cu * cus;
+    n = n / VNEC_XCDS * VNEC_XCDS;
+    *nwg = n > 0 ? n : VNEC_XCDS;
+    return 0;
+}
+
+extern "C" int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *wg_row,
+                                    double *wg_carry, void *stream)
+{
+    stream_kernel_t k = pick_stream(a->variant, a->ipt, a->encoding);
+    if (!k || a->nrhs != 1 || nwg < VNEC_XCDS || nwg % VNEC_XCDS) return (int)hipErrorInvalidValue;
+    if (a->ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, nwg,
+                       wg_row, wg_carry);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, double *y,
+                                 int64_t nrows, void *stream)
+{
+    if (nwg <= 0) return 0;
+    hipLaunchKernelGGL(k_wg_fixup, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, wg_row, wg_carry, nwg, y, nrows);
+    return (int)hipGetLastError();
+}

@@ -71,7 +71,10 @@ struct vnec_plan_s {
     int64_t nrows, ncols, nnz;
     int64_t ntiles;
     int32_t *coords;   /* 2 (ntiles + 1) */
-    double *carry;     /* ntiles * nrhs  */
+    double *carry;     /* ntiles * nrhs  (one workgroup per tile)      */
+    int nwg;           /* persistent workgroups (0: one per tile)       */
+    int32_t *wg_row;   /* nwg: row of each workgroup's trailing carry   */
+    double *wg_carry;  /* nwg                                           */
     /* variant E */
     uint16_t *lcol;
     int32_t *run_off, *run_start, *run_wpos, *nec;

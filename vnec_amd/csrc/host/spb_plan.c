@@ -18,6 +18,8 @@ static void plan_release(struct vnec_plan_s *p)
     if (p->gexec) hipGraphExecDestroy(p->gexec);
     vnec_free_dev(p->coords);
     vnec_free_dev(p->carry);
+    vnec_free_dev(p->wg_row);
+    vnec_free_dev(p->wg_carry);
     vnec_free_dev(p->lcol);
     vnec_free_dev(p->run_off);
     vnec_free_dev(p->run_start);
@@ -40,8 +42,9 @@ SpB_Info vnec_plan_free_ext(vnec_plan *plan)
     return SpB_SUCCESS;
 }
 
-/* Per-tile ECR windows for variant E. */
-static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a)
+/* Per-tile ECR windows for variant E; *max_runs receives the largest run
+ * count of any tile. */
+static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *max_runs)
 {
     hipStream_t s = vnec_stream();
     int64_t P = p->ntiles, nnz = p->nnz;
@@ -59,6 +62,18 @@ static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a)
                                                   p->nec, nruns, s), "ecr pass 1");
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(hipMemsetAsync(nruns + P, 0, sizeof(int32_t), s), "memset");
+    *max_runs = 0;
+    if (rc == SpB_SUCCESS) {
+        int32_t *h = (int32_t *)malloc(sizeof(int32_t) * (size_t)P);
+        if (!h) rc = vnec_fail("out of memory");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipMemcpyAsync(h, nruns, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+        if (rc == SpB_SUCCESS)
+            for (int64_t t = 0; t < P; t++)
+                if (h[t] > *max_runs) *max_runs = h[t];
+        free(h);
+    }
     int64_t total = 0;
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(vnec_hip_scan_excl_i32(nruns, p->run_off, P + 1, &total, s), "run scan");
@@ -121,12 +136,41 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     SpB_Info rc = SpB_SUCCESS;
     if (p->ntiles > 0) {
         p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
-        p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(p->ntiles * tune.nrhs), &p->dev_bytes);
-        if (!p->coords || !p->carry) rc = SpB_FAILURE;
+        if (tune.nrhs == 1 && tune.grid >= 0) {
+            int nwg = tune.grid;
+            if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding, &nwg),
+                                        "occupancy")) rc = SpB_FAILURE;
+            nwg = nwg / 8 * 8;
+            if (nwg < 8) nwg = 8;
+            p->nwg = nwg;
+            p->tune.grid = nwg;
+            p->wg_row = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)nwg, &p->dev_bytes);
+            p->wg_carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)nwg, &p->dev_bytes);
+            if (!p->wg_row || !p->wg_carry) rc = SpB_FAILURE;
+        } else {
+            p->tune.grid = -1;
+            p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(p->ntiles * tune.nrhs), &p->dev_bytes);
+            if (!p->carry) rc = SpB_FAILURE;
+        }
+        if (!p->coords) rc = SpB_FAILURE;
         if (rc == SpB_SUCCESS)
             rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
                              "partition");
-        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) rc = build_windows(p, a);
+        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) {
+            int32_t max_runs = 0;
+            rc = build_windows(p, a, &max_runs);
+            if (rc == SpB_SUCCESS && p->nwg > 0 && max_runs > VNEC_E_STREAM_MAX_RUNS) {
+                /* too many runs in a tile for the pipelined kernel */
+                vnec_free_dev(p->wg_row);
+                vnec_free_dev(p->wg_carry);
+                p->wg_row = NULL;
+                p->wg_carry = NULL;
+                p->nwg = 0;
+                p->tune.grid = -1;
+                p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)p->ntiles, &p->dev_bytes);
+                if (!p->carry) rc = SpB_FAILURE;
+            }
+        }
         if (rc == SpB_SUCCESS && tune.encoding != VNEC_ENC_FP64 && p->nnz > 0) {
             p->hi = (float *)vnec_malloc_dev(sizeof(float) * (size_t)p->nnz, &p->dev_bytes);
             if (tune.encoding == VNEC_ENC_HILO)
@@ -155,16 +199,9 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
     return SpB_SUCCESS;
 }
 
-/* Enqueue one SpMV (kernel + fix-up) on the library stream. */
-SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev)
+static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *y_dev,
+                      vnec_merge_args *out)
 {
-    hipStream_t s = vnec_stream();
-    if (p->ntiles == 0) {
-        if (p->nrows > 0)
-            return vnec_hip_ok(hipMemsetAsync(y_dev, 0, sizeof(double) * (size_t)(p->nrows * p->tune.nrhs), s),
-                               "zero y");
-        return SpB_SUCCESS;
-    }
     vnec_merge_args m;
     memset(&m, 0, sizeof m);
     m.ptr = p->ptr;
@@ -187,9 +224,40 @@ SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev
     m.variant = p->tune.variant;
     m.encoding = p->tune.encoding;
     m.nrhs = p->tune.nrhs;
-    if (vnec_hip_ok(vnec_hip_spmv_merge(&m, s), "spmv kernel")) return SpB_FAILURE;
-    return vnec_hip_ok(vnec_hip_carry_fixup(p->coords, p->carry, p->ntiles, y_dev, p->nrows, p->tune.nrhs, s),
-                       "carry fix-up");
+    *out = m;
+}
+
+/* Launch the SpMV kernel only (its carries stay in the plan's scratch). */
+static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hipStream_t s)
+{
+    if (p->nwg > 0)
+        return vnec_hip_ok(vnec_hip_spmv_stream(m, p->nwg, p->wg_row, p->wg_carry, s), "spmv kernel");
+    return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
+}
+
+static SpB_Info launch_fixup(struct vnec_plan_s *p, double *y_dev, hipStream_t s)
+{
+    if (p->nwg > 0)
+        return vnec_hip_ok(vnec_hip_wg_fixup(p->wg_row, p->wg_carry, p->nwg, y_dev, p->nrows, s),
+                           "carry fix-up");
+    return vnec_hip_ok(vnec_hip_carry_fixup(p->coords, p->carry, p->ntiles, y_dev, p->nrows,
+                                            p->tune.nrhs, s), "carry fix-up");
+}
+
+/* Enqueue one SpMV (kernel + fix-up) on the library stream. */
+SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev)
+{
+    hipStream_t s = vnec_stream();
+    if (p->ntiles == 0) {
+        if (p->nrows > 0)
+            return vnec_hip_ok(hipMemsetAsync(y_dev, 0, sizeof(double) * (size_t)(p->nrows * p->tune.nrhs), s),
+                               "zero y");
+        return SpB_SUCCESS;
+    }
+    vnec_merge_args m;
+    fill_args(p, x_dev, y_dev, &m);
+    if (launch_main(p, &m, s) != SpB_SUCCESS) return SpB_FAILURE;
+    return launch_fixup(p, y_dev, s);
 }
 
 SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev)
@@ -315,37 +383,15 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
     for (int i = 0; i < 3; i++)
         if (vnec_hip_ok(hipEventCreate(&ev[i]), "event")) return SpB_FAILURE;
     vnec_merge_args m;
-    memset(&m, 0, sizeof m);
-    m.ptr = plan->ptr;
-    m.col = plan->col;
-    m.lcol = plan->lcol;
-    m.val = plan->val;
-    m.hi = plan->hi;
-    m.lo = plan->lo;
-    m.coords = plan->coords;
-    m.ntiles = plan->ntiles;
-    m.run_off = plan->run_off;
-    m.run_start = plan->run_start;
-    m.run_wpos = plan->run_wpos;
-    m.nec = plan->nec;
-    m.x = x_dev;
-    m.y = y_dev;
-    m.carry = plan->carry;
-    m.nrows = plan->nrows;
-    m.ipt = plan->tune.ipt;
-    m.variant = plan->tune.variant;
-    m.encoding = plan->tune.encoding;
-    m.nrhs = plan->tune.nrhs;
+    fill_args(plan, x_dev, y_dev, &m);
     SpB_Info rc = vnec_plan_run(plan, x_dev, y_dev);
     double t_main = 0.0, t_fix = 0.0;
     for (int i = 0; rc == SpB_SUCCESS && i < iters; i++) {
         float a = 0.f, b = 0.f;
         rc = vnec_hip_ok(hipEventRecord(ev[0], s), "record");
-        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_spmv_merge(&m, s), "spmv");
+        if (rc == SpB_SUCCESS) rc = launch_main(plan, &m, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[1], s), "record");
-        if (rc == SpB_SUCCESS)
-            rc = vnec_hip_ok(vnec_hip_carry_fixup(plan->coords, plan->carry, plan->ntiles, y_dev,
-                                                  plan->nrows, plan->tune.nrhs, s), "fixup");
+        if (rc == SpB_SUCCESS) rc = launch_fixup(plan, y_dev, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[2], s), "record");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(ev[2]), "sync");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
@@ -366,6 +412,7 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
 int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
 {
     memset(t, 0, sizeof *t);
+    t->grid = 0;
     t->variant = VNEC_VAR_L;
     t->ipt = 7;
     t->lanes = 64;
@@ -386,4 +433,32 @@ int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
         if (rows && spread / rows < 65536.0) t->variant = VNEC_VAR_E;
     }
     return 0;
+}
+
+SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps)
+{
+    if (!gbps || iters < 1 || bytes < 16) return vnec_fail("vnec_copy_bandwidth_ext: bad arguments");
+    if (!vnec_runtime_ready()) return SpB_FAILURE;
+    bytes = bytes / 16 * 16;
+    void *a = vnec_malloc_dev(bytes, NULL), *b = vnec_malloc_dev(bytes, NULL);
+    hipStream_t s = vnec_stream();
+    hipEvent_t e0, e1;
+    SpB_Info rc = (a && b) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemsetAsync(a, 1, bytes, s), "memset");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e0), "event");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e1), "event");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_stream_copy(a, b, bytes, s), "copy");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
+    for (int i = 0; rc == SpB_SUCCESS && i < iters; i++)
+        rc = vnec_hip_ok(vnec_hip_stream_copy(a, b, bytes, s), "copy");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e1, s), "record");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (rc == SpB_SUCCESS) *gbps = 2.0 * (double)bytes * iters / (ms / 1e3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    vnec_free_dev(a);
+    vnec_free_dev(b);
+    return rc;
 }

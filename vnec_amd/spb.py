@@ -182,6 +182,7 @@ class TuningInfo:
     lanes: int
     encoding: str
     nrhs: int
+    grid: int
     ntiles: int
     dev_bytes: int
 
@@ -190,9 +191,10 @@ class Plan:
     """GPU execution plan (vnec_plan_build_ext)."""
 
     def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
-                 encoding: str = "fp64", nrhs: int = 1):
+                 encoding: str = "fp64", nrhs: int = 1, grid: int = 0):
         init()
-        t = L.Tuning(VNEC_VAR[variant] if variant else -1, ipt, lanes, VNEC_ENC[encoding], nrhs)
+        t = L.Tuning(VNEC_VAR[variant] if variant else -1, ipt, lanes, VNEC_ENC[encoding], nrhs,
+                     grid)
         self.h = C.c_void_p()
         self.A = A
         L.check(L.load().vnec_plan_build_ext(C.byref(self.h), A.h, C.byref(t)),
@@ -205,8 +207,8 @@ class Plan:
                 "vnec_plan_info_ext")
         inv = {v: k for k, v in VNEC_VAR.items()}
         inve = {v: k for k, v in VNEC_ENC.items()}
-        return TuningInfo(inv[t.variant], t.ipt, t.lanes, inve[t.encoding], t.nrhs, nt.value,
-                          db.value)
+        return TuningInfo(inv[t.variant], t.ipt, t.lanes, inve[t.encoding], t.nrhs, t.grid,
+                          nt.value, db.value)
 
     def spmv(self, y: Vector, x: Vector):
         L.check(L.load().vnec_spmv_ext(self.h, y.h, x.h), "vnec_spmv_ext")
