@@ -27,13 +27,15 @@ int vnec_hip_partition(const int32_t *ptr, int64_t nrows, int64_t nnz, int64_t P
 int vnec_hip_scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, int64_t *total_host,
                            void *stream);
 
-/* ECR by per-tile LDS sort (tiles of at most VNEC_ECR_TILE_MAX nonzeros).
+/* ECR by per-tile LDS sort (tiles of at most VNEC_ECR_TILE_MAX_BIG
+ * nonzeros; tiles up to VNEC_ECR_TILE_MAX use the smaller-LDS form).
  * Pass 1 writes the tile-local rank of each nonzero (lcol16 and/or ecr32,
  * either may be NULL), the sorted distinct columns of tile t at
  * uniq[y_t .. y_t + nec[t]), nec[t] and the number of runs of consecutive
  * columns nruns[t].  Pass 2 writes each tile's runs (first column, window
  * position) at run_off[t]... */
 #define VNEC_ECR_TILE_MAX 4096
+#define VNEC_ECR_TILE_MAX_BIG 16384
 int vnec_hip_ecr_tiles_pass1(const int32_t *col, const int32_t *coords, int64_t ntiles,
                              uint16_t *lcol16, int32_t *ecr32, int32_t *uniq, int32_t *nec,
                              int32_t *nruns, void *stream);
@@ -110,6 +112,40 @@ int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 
 /* Streaming copy of bytes (multiple of 16) from src to dst. */
 int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
+
+/* ---- variant S: 64-row slices, [l][lane] layout ---- */
+typedef struct {
+    const int32_t *ptr;         /* nrows + 1 (row lengths)                 */
+    const int32_t *slice_len;   /* nslices: longest row of each slice      */
+    const int32_t *slice_ptr;   /* nslices + 1: padded offsets              */
+    const double *sv;           /* padded values, encoding FP64             */
+    const float *shi, *slo;     /* padded values, encodings HILO / HI       */
+    const int32_t *scol;        /* padded global columns (win == 0)         */
+    const uint16_t *slcol;      /* padded window ranks (win == 1)           */
+    const int32_t *tile_slices; /* 2 (ntiles + 1): (first slice, offset)    */
+    int64_t ntiles;
+    const int32_t *run_off, *run_start, *run_wpos, *nec; /* windows        */
+    int32_t max_nec;            /* LDS window entries per workgroup         */
+    int win;
+    const double *x;
+    double *y;
+    int64_t nrows;
+    int encoding;
+} vnec_slice_args;
+
+/* slice_len[s] = longest row of slice s, slice_sz[s] = 64 * slice_len[s],
+ * slice_sz[nslices] = 0 (its exclusive scan gives the padded offsets). */
+int vnec_hip_slice_len(const int32_t *ptr, int64_t nrows, int64_t nslices, int32_t *slice_len,
+                       int32_t *slice_sz, void *stream);
+/* (row, nonzero) coordinates of tile boundaries given in slice space. */
+int vnec_hip_slice_tile_rows(const int32_t *slice_coords, int64_t ntiles, const int32_t *ptr,
+                             int64_t nrows, int32_t *row_coords, void *stream);
+/* Transposed, padded slice arrays (any output may be NULL). */
+int vnec_hip_slice_fill(const int32_t *ptr, const int32_t *col, const uint16_t *lcol,
+                        const double *val, int64_t nrows, int64_t nslices,
+                        const int32_t *slice_len, const int32_t *slice_ptr, double *sv, float *shi,
+                        float *slo, int32_t *scol, uint16_t *slcol, void *stream);
+int vnec_hip_spmv_slice(const vnec_slice_args *a, void *stream);
 
 /* Overwrites n bytes (cache flush helper for timing). */
 int vnec_hip_scribble(void *p, size_t n, void *stream);

@@ -81,7 +81,20 @@ def test_spmv_fp64_matches_oracle(name, m, variant, ipt, grid):
     assert ok, worst
 
 
-@pytest.mark.parametrize("variant", ["L", "E"])
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("ipt", [1, 3, 7])
+def test_slice_variant_matches_oracle(name, m, ipt):
+    """Variant S (64-row slices; window or direct columns by plan)."""
+    A = _mat(m)
+    print(f"case {name} variant S ipt {ipt}", flush=True)
+    y = _run_plan(A, m.x, "S", ipt)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y0, ax, TOL)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("variant", ["L", "E", "S"])
 @pytest.mark.parametrize("enc", ["hilo", "hi"])
 def test_encodings_error_bounds(variant, enc):
     m = g.stencil27(20, seed=21)
@@ -116,8 +129,8 @@ def test_known_answer_is_exact():
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
     rows = np.repeat(np.arange(m.nrows), np.diff(m.ptr))
     expect = np.bincount(rows, weights=rows * m.col.astype(np.float64), minlength=m.nrows)
-    for variant in ("L", "E"):
-        assert np.array_equal(_run_plan(A, x, variant, 7), expect)
+    for variant, ipt in (("L", 7), ("E", 7), ("S", 3)):
+        assert np.array_equal(_run_plan(A, x, variant, ipt), expect)
 
 
 def test_empty_rows_and_empty_matrix():
@@ -126,8 +139,10 @@ def test_empty_rows_and_empty_matrix():
     m = g.from_pattern(6, 6, np.array([0, 0, 2, 2, 2, 3, 3]), np.array([1, 4, 0], np.int32), 3)
     A = _mat(m)
     y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
-    for variant in ("L", "E"):
+    for variant in ("L", "E", "S"):
         assert np.allclose(_run_plan(A, m.x, variant, 3), y0, rtol=0, atol=1e-15)
+    A = spb.Matrix.from_csr(5, 4, [0, 0, 0, 0, 0, 0], [], [])
+    assert np.array_equal(_run_plan(A, np.ones(4), "S", 1), np.zeros(5))
 
 
 # -------------------------------------------------- builder structures
@@ -265,10 +280,10 @@ def test_graph_timing_path_gives_same_result():
     assert O.check_rows(y, O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
 
 
-@pytest.mark.parametrize("variant", ["L", "E"])
-def test_full_size_stencil_per_row_gate(variant):
+@pytest.mark.parametrize("variant,ipt", [("L", 7), ("E", 7), ("S", 3)])
+def test_full_size_stencil_per_row_gate(variant, ipt):
     m = g.stencil27(128, seed=1)
     A = _mat(m)
-    y = _run_plan(A, m.x, variant, 7)
+    y = _run_plan(A, m.x, variant, ipt)
     ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     assert O.check_rows(y, yl, ax, TOL)[0]

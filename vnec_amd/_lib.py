@@ -136,6 +136,10 @@ SIGNATURES = {
     "vnec_hip_spmv_stream": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "vnec_hip_wg_fixup": (C.c_int, [vp, vp, C.c_int, vp, i64, vp]),
     "vnec_hip_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "vnec_hip_slice_len": (C.c_int, [vp, i64, i64, vp, vp, vp]),
+    "vnec_hip_slice_tile_rows": (C.c_int, [vp, i64, vp, i64, vp, vp]),
+    "vnec_hip_slice_fill": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
+    "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
 }
 

@@ -145,13 +145,14 @@ extern "C" int vnec_hip_scan_excl_i32(const int32_t *in, int32_t *out, int64_t n
 // tile's distinct columns (spmv_vnec_D_for_mxv.c:292-319 computes the same
 // ranks with dense flags).  Runs of consecutive distinct columns are
 // counted for the compact window description.
+template <int CAP>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_ecr_tile_pass1(
     const int32_t *__restrict__ col, const int2 *__restrict__ coords, int64_t ntiles,
     uint16_t *__restrict__ lcol16, int32_t *__restrict__ ecr32, int32_t *__restrict__ uniq,
     int32_t *__restrict__ nec, int32_t *__restrict__ nruns)
 {
-    __shared__ uint32_t s_key[VNEC_ECR_TILE_MAX];
-    __shared__ uint16_t s_idx[VNEC_ECR_TILE_MAX];
+    __shared__ uint32_t s_key[CAP];
+    __shared__ uint16_t s_idx[CAP];
     __shared__ int s_tmp[VNEC_NWAVES + 1];
     int64_t t = blockIdx.x;
     if (t >= ntiles) return;
@@ -245,9 +246,18 @@ extern "C" int vnec_hip_ecr_tiles_pass1(const int32_t *col, const int32_t *coord
                                         int32_t *nec, int32_t *nruns, void *stream)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_ecr_tile_pass1, dim3((unsigned)ntiles), dim3(VNEC_BLOCK), 0,
-                       (hipStream_t)stream, col, (const int2 *)coords, ntiles, lcol16, ecr32, uniq,
-                       nec, nruns);
+    int32_t mx = 0;
+    int rc = vnec_hip_max_tile_nnz(coords, ntiles, &mx, stream);
+    if (rc) return rc;
+    if (mx > VNEC_ECR_TILE_MAX_BIG) return (int)hipErrorInvalidValue;
+    if (mx <= VNEC_ECR_TILE_MAX)
+        hipLaunchKernelGGL(k_ecr_tile_pass1<VNEC_ECR_TILE_MAX>, dim3((unsigned)ntiles), dim3(VNEC_BLOCK),
+                           0, (hipStream_t)stream, col, (const int2 *)coords, ntiles, lcol16, ecr32,
+                           uniq, nec, nruns);
+    else
+        hipLaunchKernelGGL(k_ecr_tile_pass1<VNEC_ECR_TILE_MAX_BIG>, dim3((unsigned)ntiles),
+                           dim3(VNEC_BLOCK), 0, (hipStream_t)stream, col, (const int2 *)coords, ntiles,
+                           lcol16, ecr32, uniq, nec, nruns);
     return (int)hipGetLastError();
 }
 

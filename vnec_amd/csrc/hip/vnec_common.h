@@ -132,7 +132,8 @@ static inline int64_t xcd_grid(int64_t ntiles)
 // consecutive x entries (coalesced; run descriptors are wave-uniform, read
 // with scalar loads).  Short runs (the window of a scattered tile): one
 // lane per window position, finding its run by binary search over the run
-// descriptors staged in s_runs (2 * nr ints of LDS).
+// descriptors staged in s_runs (2 * nr ints of LDS; with s_runs == nullptr
+// the long-run form is always used).
 __device__ __forceinline__ void fill_window(const double *__restrict__ x,
                                             const int32_t *__restrict__ run_start,
                                             const int32_t *__restrict__ run_wpos, int r0, int r1,
@@ -140,7 +141,7 @@ __device__ __forceinline__ void fill_window(const double *__restrict__ x,
 {
     const int nr = r1 - r0;
     if (nr <= 0) return;
-    if (nec >= 16 * nr) {
+    if (nec >= 16 * nr || s_runs == nullptr) {
         const int w = __builtin_amdgcn_readfirstlane(wave_id());
         for (int r = r0 + w; r < r1; r += VNEC_NWAVES) {
             const int start = run_start[r], wpos = run_wpos[r];

@@ -80,6 +80,15 @@ struct vnec_plan_s {
     int32_t *run_off, *run_start, *run_wpos, *nec;
     /* encodings */
     float *hi, *lo;
+    /* variant S: 64-row slices */
+    int64_t nslices;
+    int32_t *slice_len, *slice_ptr, *tile_slices;
+    double *sv;
+    float *shi, *slo;
+    int32_t *scol;
+    uint16_t *slcol;
+    int32_t max_nec;
+    int win;
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;
@@ -109,3 +118,4 @@ SpB_Info spb_vector_device_alloc(spb_dense *v);
 /* ---- plans (spb_plan.c) ---- */
 SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev);
 int vnec_select_tuning(const spb_csr *a, vnec_tuning *t);
+double vnec_slice_padding(const spb_csr *a);

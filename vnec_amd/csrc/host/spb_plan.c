@@ -27,6 +27,14 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->nec);
     vnec_free_dev(p->hi);
     vnec_free_dev(p->lo);
+    vnec_free_dev(p->slice_len);
+    vnec_free_dev(p->slice_ptr);
+    vnec_free_dev(p->tile_slices);
+    vnec_free_dev(p->sv);
+    vnec_free_dev(p->shi);
+    vnec_free_dev(p->slo);
+    vnec_free_dev(p->scol);
+    vnec_free_dev(p->slcol);
     memset(p, 0, sizeof *p);
 }
 
@@ -91,6 +99,117 @@ static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *
     return rc;
 }
 
+/* Largest x window (entries) a variant-S tile may stage in LDS. */
+#define S_WIN_MAX 6144
+/* Shortest mean run length (window entries per run) for the S window. */
+#define S_RUN_MIN 16
+
+/* Variant S: slices, tiles of whole slices, windows or direct columns. */
+static SpB_Info build_slices(struct vnec_plan_s *p, const spb_csr *a)
+{
+    hipStream_t s = vnec_stream();
+    int64_t nr = p->nrows, nnz = p->nnz;
+    int64_t ns = (nr + 63) / 64;
+    p->nslices = ns;
+    SpB_Info rc = SpB_SUCCESS;
+    p->slice_len = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(ns + 1), &p->dev_bytes);
+    p->slice_ptr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(ns + 1), &p->dev_bytes);
+    if (!p->slice_len || !p->slice_ptr) return SpB_FAILURE;
+    rc = vnec_hip_ok(vnec_hip_slice_len(a->d_ptr, nr, ns, p->slice_len, p->slice_ptr, s), "slice len");
+    int64_t padded = 0;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_scan_excl_i32(p->slice_ptr, p->slice_ptr, ns + 1, &padded, s), "slice scan");
+    if (rc != SpB_SUCCESS) return rc;
+    if (padded + ns >= INT32_MAX) return vnec_fail("padded slice layout exceeds 2^31 entries");
+    /* tiles of whole slices, balanced over the padded layout */
+    int64_t M = ns + padded;
+    int64_t T = 2048 * (int64_t)p->tune.ipt;
+    p->ntiles = M > 0 ? (M + T - 1) / T : 0;
+    if (p->ntiles == 0) return SpB_SUCCESS;
+    p->tile_slices = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
+    p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
+    if (!p->tile_slices || !p->coords) return SpB_FAILURE;
+    rc = vnec_hip_ok(vnec_hip_partition(p->slice_ptr, ns, padded, p->ntiles, 1, p->tile_slices, s), "slice partition");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_slice_tile_rows(p->tile_slices, p->ntiles, a->d_ptr, nr, p->coords, s), "tile rows");
+    if (rc != SpB_SUCCESS) return rc;
+    /* windows: only when every tile's window is small and made of long runs */
+    int32_t mx = 0;
+    rc = vnec_hip_ok(vnec_hip_max_tile_nnz(p->coords, p->ntiles, &mx, s), "max tile nnz");
+    if (rc != SpB_SUCCESS) return rc;
+    uint16_t *lcol = NULL;
+    p->win = 0;
+    if (mx <= VNEC_ECR_TILE_MAX_BIG && nnz > 0) {
+        int64_t P = p->ntiles;
+        int32_t *uniq = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)nnz, NULL);
+        int32_t *nruns = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), NULL);
+        int32_t *nec = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), NULL);
+        lcol = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * (size_t)nnz, NULL);
+        int32_t *hn = (int32_t *)malloc(sizeof(int32_t) * (size_t)P);
+        int32_t *hr = (int32_t *)malloc(sizeof(int32_t) * (size_t)P);
+        if (!uniq || !nruns || !nec || !lcol || !hn || !hr) rc = vnec_fail("out of memory");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass1(a->d_col, p->coords, P, lcol, NULL, uniq, nec, nruns, s), "ecr");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipMemcpyAsync(hn, nec, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipMemcpyAsync(hr, nruns, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+        int ok = rc == SpB_SUCCESS;
+        int32_t mn = 0;
+        for (int64_t t = 0; ok && t < P; t++) {
+            if (hn[t] > mn) mn = hn[t];
+            if (hn[t] > S_WIN_MAX || (int64_t)hn[t] < (int64_t)S_RUN_MIN * hr[t]) ok = 0;
+        }
+        if (ok) {
+            int64_t total = 0;
+            p->max_nec = mn;
+            p->nec = nec;
+            nec = NULL;
+            p->run_off = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), &p->dev_bytes);
+            rc = p->run_off ? SpB_SUCCESS : SpB_FAILURE;
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemsetAsync(nruns + P, 0, sizeof(int32_t), s), "memset");
+            if (rc == SpB_SUCCESS)
+                rc = vnec_hip_ok(vnec_hip_scan_excl_i32(nruns, p->run_off, P + 1, &total, s), "run scan");
+            if (rc == SpB_SUCCESS) {
+                p->run_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(total ? total : 1), &p->dev_bytes);
+                p->run_wpos = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(total ? total : 1), &p->dev_bytes);
+                if (!p->run_start || !p->run_wpos) rc = SpB_FAILURE;
+            }
+            if (rc == SpB_SUCCESS)
+                rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass2(p->coords, P, uniq, p->nec, p->run_off,
+                                                          p->run_start, p->run_wpos, s), "ecr pass 2");
+            if (rc == SpB_SUCCESS) p->win = 1;
+        }
+        vnec_free_dev(uniq);
+        vnec_free_dev(nruns);
+        vnec_free_dev(nec);
+        free(hn);
+        free(hr);
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    }
+    /* transposed, padded slice arrays */
+    size_t np = (size_t)(padded ? padded : 1);
+    int enc = p->tune.encoding;
+    if (rc == SpB_SUCCESS) {
+        if (enc == VNEC_ENC_FP64) p->sv = (double *)vnec_malloc_dev(sizeof(double) * np, &p->dev_bytes);
+        else p->shi = (float *)vnec_malloc_dev(sizeof(float) * np, &p->dev_bytes);
+        if (enc == VNEC_ENC_HILO) p->slo = (float *)vnec_malloc_dev(sizeof(float) * np, &p->dev_bytes);
+        if (p->win) p->slcol = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * np, &p->dev_bytes);
+        else p->scol = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * np, &p->dev_bytes);
+        if ((enc == VNEC_ENC_FP64 && !p->sv) || (enc != VNEC_ENC_FP64 && !p->shi) ||
+            (enc == VNEC_ENC_HILO && !p->slo) || (p->win ? !p->slcol : !p->scol))
+            rc = SpB_FAILURE;
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_slice_fill(a->d_ptr, a->d_col, p->win ? lcol : NULL, a->d_val, nr, ns,
+                                             p->slice_len, p->slice_ptr, p->sv, p->shi, p->slo,
+                                             p->scol, p->slcol, s), "slice fill");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    vnec_free_dev(lcol);
+    return rc;
+}
+
 SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tuning *t)
 {
     if (!plan) return vnec_fail("vnec_plan_build_ext: NULL plan handle");
@@ -109,13 +228,18 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;
-    if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_E)
+    if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_E && tune.variant != VNEC_VAR_S)
         return vnec_fail("variant %d is not available", tune.variant);
-    if (!valid_ipt(tune.ipt)) return vnec_fail("ipt %d: use an odd value in [3, 15]", tune.ipt);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
-    if (!vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs))
-        return vnec_fail("no kernel for variant %d, ipt %d, encoding %d, nrhs %d", tune.variant,
-                         tune.ipt, tune.encoding, tune.nrhs);
+    if (tune.variant == VNEC_VAR_S) {
+        if (tune.ipt < 1 || tune.ipt > 7) return vnec_fail("variant S: ipt (tile = 2048 ipt items) in [1, 7]");
+        if (tune.nrhs != 1) return vnec_fail("variant S takes one right-hand side");
+    } else {
+        if (!valid_ipt(tune.ipt)) return vnec_fail("ipt %d: use an odd value in [3, 15]", tune.ipt);
+        if (!vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs))
+            return vnec_fail("no kernel for variant %d, ipt %d, encoding %d, nrhs %d", tune.variant,
+                             tune.ipt, tune.encoding, tune.nrhs);
+    }
     if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
 
     struct vnec_plan_s *p = (struct vnec_plan_s *)calloc(1, sizeof *p);
@@ -134,7 +258,11 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     p->ntiles = M > 0 ? (M + T - 1) / T : 0;
     hipStream_t s = vnec_stream();
     SpB_Info rc = SpB_SUCCESS;
-    if (p->ntiles > 0) {
+    if (tune.variant == VNEC_VAR_S) {
+        p->ntiles = 0;
+        p->tune.grid = -1;
+        rc = p->nrows > 0 ? build_slices(p, a) : SpB_SUCCESS;
+    } else if (p->ntiles > 0) {
         p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
         if (tune.nrhs == 1 && tune.grid >= 0) {
             int nwg = tune.grid;
@@ -227,9 +355,37 @@ static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *
     *out = m;
 }
 
+static SpB_Info launch_slice(struct vnec_plan_s *p, const double *x_dev, double *y_dev, hipStream_t s)
+{
+    vnec_slice_args a;
+    memset(&a, 0, sizeof a);
+    a.ptr = p->ptr;
+    a.slice_len = p->slice_len;
+    a.slice_ptr = p->slice_ptr;
+    a.sv = p->sv;
+    a.shi = p->shi;
+    a.slo = p->slo;
+    a.scol = p->scol;
+    a.slcol = p->slcol;
+    a.tile_slices = p->tile_slices;
+    a.ntiles = p->ntiles;
+    a.run_off = p->run_off;
+    a.run_start = p->run_start;
+    a.run_wpos = p->run_wpos;
+    a.nec = p->nec;
+    a.max_nec = p->max_nec;
+    a.win = p->win;
+    a.x = x_dev;
+    a.y = y_dev;
+    a.nrows = p->nrows;
+    a.encoding = p->tune.encoding;
+    return vnec_hip_ok(vnec_hip_spmv_slice(&a, s), "slice kernel");
+}
+
 /* Launch the SpMV kernel only (its carries stay in the plan's scratch). */
 static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hipStream_t s)
 {
+    if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, m->x, m->y, s);
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_spmv_stream(m, p->nwg, p->wg_row, p->wg_carry, s), "spmv kernel");
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
@@ -237,6 +393,7 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hip
 
 static SpB_Info launch_fixup(struct vnec_plan_s *p, double *y_dev, hipStream_t s)
 {
+    if (p->tune.variant == VNEC_VAR_S) return SpB_SUCCESS; /* row-aligned tiles: no carries */
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_wg_fixup(p->wg_row, p->wg_carry, p->nwg, y_dev, p->nrows, s),
                            "carry fix-up");
@@ -407,8 +564,23 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
 }
 
 /* -------------------------------------------------------------- selector
- * Placeholder rule until the learned selector is wired in: locality-rich
- * matrices (small mean column spread per row) use the LDS window. */
+ * Rule used when no learned model is loaded: 64-row slices (variant S) when
+ * padding them to their longest row costs at most 15 % extra slots,
+ * otherwise the persistent merge-path kernel (variant L). */
+double vnec_slice_padding(const spb_csr *a)
+{
+    int64_t nr = (int64_t)a->base.row, slots = 0;
+    for (int64_t s = 0; s < nr; s += 64) {
+        int64_t e = s + 64 < nr ? s + 64 : nr, m = 0;
+        for (int64_t r = s; r < e; r++) {
+            int64_t l = (int64_t)(a->ptr[r + 1] - a->ptr[r]);
+            m = l > m ? l : m;
+        }
+        slots += 64 * m;
+    }
+    return a->nnz ? (double)slots / (double)a->nnz : 1.0;
+}
+
 int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
 {
     memset(t, 0, sizeof *t);
@@ -418,19 +590,9 @@ int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
     t->lanes = 64;
     t->encoding = VNEC_ENC_FP64;
     t->nrhs = 1;
-    int64_t nr = (int64_t)a->base.row;
-    if (nr > 0 && a->nnz > 0) {
-        /* sampled bandwidth: mean |col - row| over up to 4096 rows */
-        int64_t step = nr / 4096 > 0 ? nr / 4096 : 1, rows = 0;
-        double spread = 0.0;
-        for (int64_t r = 0; r < nr; r += step) {
-            SpB_Index b = a->ptr[r], e = a->ptr[r + 1];
-            if (e > b) {
-                spread += (double)(a->indices[e - 1] - a->indices[b]);
-                rows++;
-            }
-        }
-        if (rows && spread / rows < 65536.0) t->variant = VNEC_VAR_E;
+    if (a->nnz > 0 && vnec_slice_padding(a) <= 1.15) {
+        t->variant = VNEC_VAR_S;
+        t->ipt = 3;
     }
     return 0;
 }

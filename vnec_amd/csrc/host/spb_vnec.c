@@ -252,7 +252,7 @@ static SpB_Info d_layout_device(spb_csr *a, int64_t P, int L, int want_vals, dla
         d->col_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(G ? G : 1), NULL);
         if (!d->col_start) rc = SpB_FAILURE;
         if (want_vals && rc == SpB_SUCCESS) {
-            d->val_align = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(G * L ? G * L : 1), NULL);
+            d->val_align = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(G > 0 ? G * L : 1), NULL);
             if (!d->val_align) rc = SpB_FAILURE;
             else rc = vnec_hip_ok(hipMemsetAsync(d->val_align, 0, sizeof(double) * (size_t)(G * L), s), "zero");
         }
