@@ -1,22 +1,26 @@
 #!/usr/bin/env python3
-"""bench.py — FP64 SpMV y = A x on one MI355X, on the 27-point stencil of a
-128^3 grid (2,097,152 rows, 55,742,968 nonzeros, CSR int32, FP64 values,
-seed 1): the configuration the headline metric is quoted on.
+"""bench.py — FP64 SpMV y = A x on one MI355X.
 
-A "step" is one SpMV over the whole matrix with A, x and y resident in HBM.
-`value` is algorithmic GB/s over the timed steps, with
-  B = (vb + 4) nnz + 4 (nrows + 1) + 8 ncols + 8 nrows   (vb = 8 for FP64)
+Default workload (the configuration the headline metric is quoted on): the
+27-point stencil of a 128^3 grid (2,097,152 rows, 55,742,968 nonzeros, CSR
+int32, FP64 values, seed 1).  Other workloads of the same suite are
+selectable with --workload (rmat18, skew, batch, batch8).
+
+A "step" is one SpMV over the whole workload with A, x and y resident in
+HBM.  `value` is algorithmic GB/s over the timed steps, with
+  B = (vb + 4) nnz + 4 (nrows + 1) + 8 k ncols + 8 k nrows   (vb = 8 for FP64)
 bytes per SpMV, whatever the on-device format moves.  The roofline object
 uses the same B over the SpMV kernel's mean duration (HIP events on the
 library's stream).  The CPU baseline is the reference VNEC code (compiled
 from its sources into oracle/_ref) with its own dispatch, timed like the
 reference harness (1 warm-up + 40 SpMVs), on this host's cores.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1]
+  python bench.py [--workload stencil] [--steps K] [--warmup W] [--gpus 1]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -30,6 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAME = {"S": "k_spmv_slice", "L": "k_spmv_stream", "E": "k_spmv_estream"}
 METRIC = "FP64 SpMV GB/s as % of HBM peak; learned-vs-plain speedup at equal error"
+R18_FLUSH = 1 << 29  # 512 MiB scribbled between SpMVs for an HBM (not cache) figure
 
 
 def log(*a):
@@ -41,8 +46,7 @@ class _QuietStdout:
     the bench's JSON line is the only line on stdout."""
 
     def __enter__(self):
-        import ctypes
-        self.libc = ctypes.CDLL(None)
+        self.libc = C.CDLL(None)
         sys.stdout.flush()
         self.saved = os.dup(1)
         self.null = os.open(os.devnull, os.O_WRONLY)
@@ -57,16 +61,38 @@ class _QuietStdout:
         return False
 
 
-def cpu_baseline(m, y_gpu):
+def workload(name, g):
+    """(matrix, description, right-hand sides)."""
+    if name == "stencil":
+        m = g.stencil27(128, seed=1)
+        return m, "27-point stencil 128^3 FP64 SpMV (CSR int32, seed 1)", 1
+    if name == "rmat18":
+        m = g.rmat(18, 16, 0.57, 0.19, 0.19, seed=1)
+        return m, "R-MAT scale 18, edge factor 16, (.57,.19,.19,.05), seed 1", 1
+    if name == "skew":
+        m = g.skew(22, 16, 8, 500_000, seed=1)
+        return m, "R-MAT scale 22 ef 16 + 8 dense rows x 500k distinct columns, seed 1", 1
+    if name in ("batch", "batch8"):
+        k = 8 if name == "batch8" else 1
+        m = g.batch(512, 1000, 200_000, 4.0, 64.0, nrhs=k, seed=1)
+        return m, (f"512 independent matrices, rows log-uniform [1k, 200k], nnz/row "
+                   f"Poisson(mu ~ U[4, 64]), per-matrix x, k = {k}, one launch"), k
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(m, k, y_gpu):
     """Reference VNEC CPU path on this host (the checker's leg)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402  (checker / baseline only)
 
-    B = m.algorithmic_bytes(8)
+    B = m.algorithmic_bytes(8, k)
     flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
-    out = {}
+    col = m.col if m.col_off is None else (
+        m.col + np.repeat(m.col_off[:-1], np.diff(m.ptr[m.row_off])).astype(np.int32))
+    xs = [np.ascontiguousarray(m.x[c::k]) for c in range(k)]
+    ys = []
     if O.ref_available() and "avx512f" in flags:
-        R = O.RefMatrix(m.nrows, m.ncols, m.ptr, m.col, m.val)
+        R = O.RefMatrix(m.nrows, m.ncols, m.ptr, col, m.val)
         threads = O.ref().ref_num_threads()
         t0 = time.perf_counter()
         if m.nrows < 3_000_000:  # dispatch of test_spmv_vnec.cpp:197-256
@@ -76,32 +102,42 @@ def cpu_baseline(m, y_gpu):
         plan = {"D": R.build_d, "S": R.build_s, "L": R.build_l}[kind]()
         build_s = time.perf_counter() - t0
         reps = 40
-        t_spmv, y_cpu = R.time_spmv(kind, plan, m.x, reps)
+        t_spmv = 0.0
+        for c in range(k):
+            t, y = R.time_spmv(kind, plan, xs[c], reps)
+            t_spmv += t
+            ys.append(y)
         R.free(kind, plan)
         R.close()
         out = {"value": B / t_spmv / 1e9, "unit": "GB/s", "cores": int(threads),
                "kind": "reference",
-               "sample": (f"full workload, reference VNEC-{kind} (IRD dispatch) compiled "
-                          f"-O3 -march=skylake-avx512, OpenMP {threads} threads "
-                          f"(NUM_THREADS) on {os.cpu_count()} visible host CPUs; "
-                          f"1 warm-up + {reps} SpMVs; build {build_s:.1f} s untimed"),
+               "sample": (f"full workload, reference VNEC-{kind} (its dispatch) compiled "
+                          f"-O3 -march=skylake-avx512, OpenMP {threads} threads (NUM_THREADS) "
+                          f"on {os.cpu_count()} visible host CPUs; 1 warm-up + {reps} SpMVs"
+                          + (f" per right-hand side (k = {k}, no multi-RHS in the reference)"
+                             if k > 1 else "") + f"; build {build_s:.1f} s untimed"),
                "ms_per_spmv": t_spmv * 1e3}
     else:
         nt = O.nthreads()
-        P = max(nt, 1)
-        O.spmv_L(m.ptr, m.col, m.val, m.x, P, nt)
         reps = 10
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            y_cpu = O.spmv_L(m.ptr, m.col, m.val, m.x, P, nt)
-        t_spmv = (time.perf_counter() - t0) / reps
+        t_spmv = 0.0
+        for c in range(k):
+            O.spmv_L(m.ptr, col, m.val, xs[c], nt, nt)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                y = O.spmv_L(m.ptr, col, m.val, xs[c], nt, nt)
+            t_spmv += (time.perf_counter() - t0) / reps
+            ys.append(y)
         out = {"value": B / t_spmv / 1e9, "unit": "GB/s", "cores": int(nt), "kind": "port",
                "sample": f"full workload, oracle merge-path CSR (VNEC-L restatement), "
                          f"{nt} OpenMP threads, {reps} SpMVs (reference build unavailable)",
                "ms_per_spmv": t_spmv * 1e3}
-    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
-    err = np.abs(y_gpu - y_cpu) / np.where(ax > 0, ax, 1.0)
-    out["gpu_vs_cpu_max_rel_row_err"] = float(err.max())
+    worst = 0.0
+    for c in range(k):
+        ax, _ = O.spmv_abs(m.ptr, col, m.val, xs[c])
+        err = np.abs(y_gpu[c::k] - ys[c]) / np.where(ax > 0, ax, 1.0)
+        worst = max(worst, float(err.max()))
+    out["gpu_vs_cpu_max_rel_row_err"] = worst
     return out
 
 
@@ -110,8 +146,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--workload", default="stencil",
+                    choices=["stencil", "rmat18", "skew", "batch", "batch8"])
+    ap.add_argument("--variant", default=None, help="force a variant (S, L, E); default selector")
+    ap.add_argument("--ipt", type=int, default=0)
+    ap.add_argument("--grid", type=int, default=0, help="with --variant: 0 persistent, -1 one WG per tile")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -124,88 +165,102 @@ def main():
 
     if spb.device_count() < 1:
         raise SystemExit("no GPU visible")
-    log(f"generating 27-point stencil {args.grid}^3 ...")
-    m = g.stencil27(args.grid, seed=1)
-    B = m.algorithmic_bytes(8)
-    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    log(f"generating {args.workload} ...")
+    m, desc, k = workload(args.workload, g)
+    B = m.algorithmic_bytes(8, k)
+    if m.row_off is not None:
+        A = spb.Matrix.from_batch(m.row_off, m.col_off, m.ptr, m.col, m.val)
+    else:
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
     dx = spb.DeviceArray.from_array(m.x)
-    dy = spb.DeviceArray(8 * m.nrows)
+    dy = spb.DeviceArray(8 * m.nrows * k)
 
-    log("building plan (selector) ...")
+    log("building plan ...")
     t0 = time.perf_counter()
-    plan = spb.Plan(A, variant=None)
+    if args.variant:
+        plan = spb.Plan(A, args.variant, args.ipt or (3 if args.variant == "S" else 7), nrhs=k,
+                        grid=args.grid)
+    else:
+        plan = spb.Plan(A, variant=None, nrhs=k)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = plan.info()
     log(f"plan: {info}")
 
+    lib = spb.L.load()
     for _ in range(args.warmup):
         plan.spmv_dev(dx.ptr, dy.ptr)
-    spb.L.load().vnec_sync_ext()
+    spb.L.check(lib.vnec_sync_ext(), "sync")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         plan.spmv_dev(dx.ptr, dy.ptr)
-    spb.L.check(spb.L.load().vnec_sync_ext(), "sync")
+    spb.L.check(lib.vnec_sync_ext(), "sync")
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     value = B / (ms_step / 1e3) / 1e9
-    y_gpu = dy.to_array(np.float64, m.nrows)
+    y_gpu = dy.to_array(np.float64, m.nrows * k)
 
     k_ms, fix_ms = plan.time_kernels(dx.ptr, dy.ptr, max(args.steps // 2, 20))
     graph_ms = plan.time(dx.ptr, dy.ptr, max(args.steps, 20), graph=True)
     achieved = B / (k_ms / 1e3) / 1e9
     log(f"wall {ms_step * 1e3:.1f} us/step  kernel {k_ms * 1e3:.1f} us  fixup "
         f"{fix_ms * 1e3:.1f} us  graph {graph_ms * 1e3:.1f} us")
+    extra = {}
+    if args.workload == "rmat18":
+        # 47 MB fits in the 256 MB Infinity Cache: also time with the cache flushed
+        fl = plan.time(dx.ptr, dy.ptr, 50, graph=False, flush_bytes=R18_FLUSH)
+        extra["flushed_spmv_us"] = round(fl * 1e3, 2)
+        extra["flushed_GBps"] = round(B / (fl / 1e3) / 1e9, 1)
 
     variants = {}
-    for name, var, ipt, enc, vb in (("S_fp64", "S", 3, "fp64", 8), ("S_hilo", "S", 3, "hilo", 8),
-                                    ("S_hi", "S", 3, "hi", 4), ("L_fp64", "L", 7, "fp64", 8),
-                                    ("E_fp64", "E", 5, "fp64", 8)):
-        p = spb.Plan(A, var, ipt, encoding=enc)
-        km, _ = p.time_kernels(dx.ptr, dy.ptr, 50)
-        gm = p.time(dx.ptr, dy.ptr, 100, graph=True)
-        bv = m.algorithmic_bytes(vb)
-        variants[name] = {"kernel_us": round(km * 1e3, 2), "spmv_us": round(gm * 1e3, 2),
-                          "GBps": round(bv / (gm / 1e3) / 1e9, 1), "bytes": bv}
-        p.free()
-    plan.spmv_dev(dx.ptr, dy.ptr)
-    spb.L.load().vnec_sync_ext()
+    if args.workload == "stencil" and not args.no_variants:
+        for name, var, ipt, enc, vb in (("S_fp64", "S", 3, "fp64", 8),
+                                        ("S_hilo", "S", 3, "hilo", 8),
+                                        ("S_hi", "S", 3, "hi", 4), ("L_fp64", "L", 7, "fp64", 8),
+                                        ("E_fp64", "E", 5, "fp64", 8)):
+            p = spb.Plan(A, var, ipt, encoding=enc)
+            km, _ = p.time_kernels(dx.ptr, dy.ptr, 50)
+            gm = p.time(dx.ptr, dy.ptr, 100, graph=True)
+            bv = m.algorithmic_bytes(vb)
+            variants[name] = {"kernel_us": round(km * 1e3, 2), "spmv_us": round(gm * 1e3, 2),
+                              "GBps": round(bv / (gm / 1e3) / 1e9, 1), "bytes": bv}
+            p.free()
 
-    # measured streaming-copy ceiling (also the byte-counter calibration kernel)
-    import ctypes as C
-    cp = C.c_double()
-    spb.L.check(spb.L.load().vnec_copy_bandwidth_ext(1 << 30, 10, C.byref(cp)), "copy bandwidth")
+    cp = C.c_double()  # measured streaming-copy ceiling (also the counter calibration)
+    spb.L.check(lib.vnec_copy_bandwidth_ext(1 << 30, 10, C.byref(cp)), "copy bandwidth")
 
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_stencil27_128.json")
+    tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        if tj.get("variant") == info.variant:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
         with _QuietStdout():
-            cpu = cpu_baseline(m, y_gpu)
+            cpu = cpu_baseline(m, k, y_gpu)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"27-point stencil {args.grid}^3 FP64 SpMV (CSR int32, seed 1)",
-                   "nrows": m.nrows, "nnz": m.nnz, "algorithmic_bytes": B,
-                   "variant": info.variant, "ipt": info.ipt, "tiles": info.ntiles,
-                   "parallelism": "single GPU"},
+        "config": {"workload": desc, "nrows": m.nrows, "nnz": m.nnz, "nrhs": k,
+                   "algorithmic_bytes": B, "variant": info.variant, "ipt": info.ipt,
+                   "tiles": info.ntiles, "parallelism": "single GPU"},
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
-        "gflops": round(2.0 * m.nnz / (ms_step / 1e3) / 1e9, 1),
+        "gflops": round(2.0 * m.nnz * k / (ms_step / 1e3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": KERNEL_NAME[info.variant],
+                     "traffic": traffic, "kernel": KERNEL_NAME.get(info.variant, "?"),
                      "kernel_us": round(k_ms * 1e3, 2), "fixup_us": round(fix_ms * 1e3, 2)},
         "graph_spmv_us": round(graph_ms * 1e3, 2),
         "copy_peak_gbs": round(cp.value, 1),
         "pct_of_copy_peak": round(100.0 * achieved / cp.value, 2),
         "plan_build_ms": round(build_ms, 1),
+        **extra,
         "variants": variants,
         "cpu_baseline": cpu,
     }

@@ -30,6 +30,14 @@ int vnec_device_count_ext(void);
 SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
                                    const double *val);
 SpB_Info SpB_Matrix_nvals_ext(SpB_Index *nvals, const SpB_Matrix A);
+/* Batch of nmat independent matrices as one block-diagonal CSR matrix
+ * (A made by SpB_Matrix_new with row_off[nmat] rows, col_off[nmat]
+ * columns): member m owns rows [row_off[m], row_off[m+1]) and columns
+ * [col_off[m], col_off[m+1]); col holds member-local columns.  A plan on A
+ * runs the whole batch, each member with its own x slice, in one launch. */
+SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_off,
+                               const int64_t *col_off, const int64_t *ptr, const int32_t *col,
+                               const double *val);
 /* Bulk copy of a dense FP64 vector's n values in / out. */
 SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values);
 SpB_Info SpB_Vector_export_ext(const SpB_Vector v, double *values);

@@ -90,6 +90,7 @@ SIGNATURES = {
     "vnec_device_count_ext": (C.c_int, []),
     "SpB_Matrix_import_csr_ext": (Info, [vp, vp, vp, vp]),
     "SpB_Matrix_nvals_ext": (Info, [C.POINTER(u64), vp]),
+    "vnec_batch_import_ext": (Info, [vp, i64, vp, vp, vp, vp, vp]),
     "SpB_Vector_import_ext": (Info, [vp, vp]),
     "SpB_Vector_export_ext": (Info, [vp, vp]),
     "SpB_Vector_device_resident_ext": (Info, [vp, C.c_bool]),

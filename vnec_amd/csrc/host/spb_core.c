@@ -308,7 +308,7 @@ SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32
     for (SpB_Index r = 0; r < nr; r++)
         if (ptr[r + 1] < ptr[r]) return vnec_fail("SpB_Matrix_import_csr_ext: ptr not monotone");
     if (matrix_reserve(a, (SpB_Index)nnz) != SpB_SUCCESS) return SpB_FAILURE;
-    for (int64_t j = 0; j < nnz; j++) {
+    for (int64_t j = 0; col && j < nnz; j++) {
         if (col[j] < 0 || (SpB_Index)col[j] >= nc) {
             a->nnz = 0;
             return vnec_fail("SpB_Matrix_import_csr_ext: column %d out of range", col[j]);
@@ -317,6 +317,39 @@ SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32
     }
     memcpy(a->val, val, sizeof(double) * (size_t)nnz);
     for (SpB_Index r = 0; r <= nr; r++) a->ptr[r] = (SpB_Index)ptr[r];
+    return SpB_SUCCESS;
+}
+
+/* Block-diagonal batch of nmat matrices as one CSR matrix: member m owns
+ * rows [row_off[m], row_off[m+1]) and columns [col_off[m], col_off[m+1]);
+ * `col` holds member-local columns, `ptr` global row offsets.  One SpMV
+ * over the batch is then one merge-path launch over all the nonzeros. */
+SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_off,
+                               const int64_t *col_off, const int64_t *ptr, const int32_t *col,
+                               const double *val)
+{
+    spb_csr *a = (spb_csr *)A;
+    if (!a || a->magic != SPB_MAGIC_MATRIX || nmat < 1 || !row_off || !col_off || !ptr)
+        return vnec_fail("vnec_batch_import_ext: bad arguments");
+    if ((SpB_Index)row_off[nmat] != a->base.row || (SpB_Index)col_off[nmat] != a->base.col)
+        return vnec_fail("vnec_batch_import_ext: matrix is %lux%lu, batch is %ldx%ld", a->base.row,
+                         a->base.col, (long)row_off[nmat], (long)col_off[nmat]);
+    if (row_off[0] != 0 || col_off[0] != 0) return vnec_fail("vnec_batch_import_ext: offsets must start at 0");
+    for (int64_t m = 0; m < nmat; m++)
+        if (row_off[m + 1] < row_off[m] || col_off[m + 1] < col_off[m])
+            return vnec_fail("vnec_batch_import_ext: offsets not monotone at member %ld", (long)m);
+    SpB_Info rc = SpB_Matrix_import_csr_ext(A, ptr, NULL, val);
+    if (rc != SpB_SUCCESS) return rc;
+    for (int64_t m = 0; m < nmat; m++) {
+        int64_t c0 = col_off[m], nc = col_off[m + 1] - col_off[m];
+        for (int64_t j = ptr[row_off[m]]; j < ptr[row_off[m + 1]]; j++) {
+            if (col[j] < 0 || col[j] >= nc) {
+                a->nnz = 0;
+                return vnec_fail("vnec_batch_import_ext: member %ld column %d out of range", (long)m, col[j]);
+            }
+            a->indices[j] = (SpB_Index)(c0 + col[j]);
+        }
+    }
     return SpB_SUCCESS;
 }
 

@@ -226,6 +226,11 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             tune.encoding = t->encoding;
             if (t->nrhs > 0) tune.nrhs = t->nrhs;
         }
+        if (tune.nrhs > 1) { /* several right-hand sides: the merge-path kernel */
+            tune.variant = VNEC_VAR_L;
+            tune.ipt = 5;
+            tune.grid = -1;
+        }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;
     if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_E && tune.variant != VNEC_VAR_S)

@@ -65,6 +65,20 @@ class Matrix:
         return m
 
     @classmethod
+    def from_batch(cls, row_off, col_off, ptr, col_local, val) -> "Matrix":
+        """Block-diagonal batch (vnec_batch_import_ext)."""
+        row_off = np.ascontiguousarray(row_off, dtype=np.int64)
+        col_off = np.ascontiguousarray(col_off, dtype=np.int64)
+        m = cls(int(row_off[-1]), int(col_off[-1]))
+        ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+        col = np.ascontiguousarray(col_local, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        L.check(L.load().vnec_batch_import_ext(m.h, len(row_off) - 1, _p(row_off), _p(col_off),
+                                               _p(ptr), _p(col), _p(val)), "vnec_batch_import_ext")
+        m.nnz = int(ptr[-1])
+        return m
+
+    @classmethod
     def from_coo(cls, nrows, ncols, rows, cols, vals) -> "Matrix":
         m = cls(nrows, ncols)
         r = np.ascontiguousarray(rows, dtype=np.uint64)
