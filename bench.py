@@ -32,7 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAME = {"S": "k_spmv_slice", "L": "k_spmv_stream", "E": "k_spmv_estream"}
+# dominant kernel of each plan shape: (variant, persistent schedule?)
+KERNEL_NAME = {("S", False): "k_spmv_slice", ("L", True): "k_spmv_stream",
+               ("E", True): "k_spmv_estream", ("L", False): "k_spmv_merge1",
+               ("E", False): "k_spmv_merge1"}
+PLAIN_CANDIDATE = 4  # L, 7 items per thread, persistent: the plain merge-path CSR kernel
 METRIC = "FP64 SpMV GB/s as % of HBM peak; learned-vs-plain speedup at equal error"
 R18_FLUSH = 1 << 29  # 512 MiB scribbled between SpMVs for an HBM (not cache) figure
 
@@ -80,8 +84,10 @@ def workload(name, g):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(m, k, y_gpu):
-    """Reference VNEC CPU path on this host (the checker's leg)."""
+def cpu_baseline(m, k, y_gpu, variant_y):
+    """Reference VNEC CPU path on this host (the checker's leg).  Also
+    reports, for each extra GPU result in variant_y (one right-hand side),
+    max_i |y_i - y_ref,i| / (|A||x|)_i against the reference's FP64 y."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O  # noqa: E402  (checker / baseline only)
 
@@ -138,7 +144,65 @@ def cpu_baseline(m, k, y_gpu):
         err = np.abs(y_gpu[c::k] - ys[c]) / np.where(ax > 0, ax, 1.0)
         worst = max(worst, float(err.max()))
     out["gpu_vs_cpu_max_rel_row_err"] = worst
+    ax, _ = O.spmv_abs(m.ptr, col, m.val, xs[0])
+    scale = np.where(ax > 0, ax, 1.0)
+    out["variant_errors"] = {name: float((np.abs(y - ys[0]) / scale).max())
+                             for name, y in variant_y.items()}
     return out
+
+
+def _abs_rows(ptr, col, val, x):
+    rows = np.repeat(np.arange(len(ptr) - 1), np.diff(ptr))
+    return np.bincount(rows, weights=np.abs(val) * np.abs(x[col]), minlength=len(ptr) - 1)
+
+
+def selector_eval(spb, n):
+    """Learned selector vs the plain kernel, the reference dispatch and the
+    exhaustive autotuner, live, on n held-out selector matrices."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from vnec_amd.selector import specs
+
+    idx = [specs.N_TRAIN + (8 * j) % specs.N_HELD for j in range(n)]
+    pool = ThreadPoolExecutor(max_workers=8)
+    futs = [pool.submit(lambda i: specs.generate(specs.spec(i)), i) for i in idx]
+    rows = []
+    worst = 0.0
+    for i, f in zip(idx, futs):
+        m = f.result()
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+        t, best = spb.autotune(A, 20)
+        learned = spb.select(A, "learned")
+        ref = spb.select(A, "reference")
+        rule = spb.select(A, "rule")
+        ys = []
+        for c in (learned, PLAIN_CANDIDATE):
+            ci = spb.candidates()[c]
+            p = spb.Plan(A, ci.variant, ci.ipt, grid=ci.grid)
+            xv, yv = spb.Vector.from_array(m.x), spb.Vector(m.nrows)
+            p.spmv(yv, xv)
+            ys.append(yv.to_array())
+            p.free()
+        ax = _abs_rows(m.ptr, m.col, m.val, m.x)
+        worst = max(worst, float((np.abs(ys[0] - ys[1]) / np.where(ax > 0, ax, 1.0)).max()))
+        rows.append((t[best], t[learned], t[PLAIN_CANDIDATE], t[ref], t[rule]))
+        log(f"selector [{i}] {specs.spec(i)['family']:9s} nnz {m.nnz:>9d} best {best:2d} "
+            f"learned {learned:2d} ({t[learned] / t[best]:.3f}x of best)")
+        A.free()
+    pool.shutdown()
+    r = np.array(rows)
+    geo = lambda a, b: float(np.exp(np.mean(np.log(a / b))))  # noqa: E731
+    return {"matrices": n,
+            "sample": (f"held-out selector matrices {idx[0]}..{max(idx)} (every 8th of 200: "
+                       "R-MAT, banded, block-diagonal, uniform, power-law), timed live with "
+                       "hipGraph replay; plain = merge-path CSR kernel, 7 items/thread, persistent"),
+            "learned_vs_plain_speedup_geomean": round(geo(r[:, 2], r[:, 1]), 4),
+            "learned_vs_plain_speedup_aggregate": round(float(r[:, 2].sum() / r[:, 1].sum()), 4),
+            "learned_vs_reference_dispatch_speedup_geomean": round(geo(r[:, 3], r[:, 1]), 4),
+            "learned_vs_rule_speedup_geomean": round(geo(r[:, 4], r[:, 1]), 4),
+            "learned_regret_vs_exhaustive_geomean": round(geo(r[:, 1], r[:, 0]), 4),
+            "learned_regret_vs_exhaustive_aggregate": round(float(r[:, 1].sum() / r[:, 0].sum()), 4),
+            "max_rel_row_diff_learned_vs_plain": worst}
 
 
 def main():
@@ -153,6 +217,8 @@ def main():
     ap.add_argument("--grid", type=int, default=0, help="with --variant: 0 persistent, -1 one WG per tile")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--selector-matrices", type=int, default=25,
+                    help="held-out matrices for the learned-vs-plain comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -212,12 +278,17 @@ def main():
         extra["flushed_GBps"] = round(B / (fl / 1e3) / 1e9, 1)
 
     variants = {}
+    variant_y = {}
     if args.workload == "stencil" and not args.no_variants:
-        for name, var, ipt, enc, vb in (("S_fp64", "S", 3, "fp64", 8),
-                                        ("S_hilo", "S", 3, "hilo", 8),
-                                        ("S_hi", "S", 3, "hi", 4), ("L_fp64", "L", 7, "fp64", 8),
-                                        ("E_fp64", "E", 5, "fp64", 8)):
+        for name, var, ipt, enc, vb in (("S3_fp64", "S", 3, "fp64", 8),
+                                        ("S5_fp64", "S", 5, "fp64", 8),
+                                        ("S3_hilo", "S", 3, "hilo", 8),
+                                        ("S3_hi", "S", 3, "hi", 4), ("L7_fp64", "L", 7, "fp64", 8),
+                                        ("E5_fp64", "E", 5, "fp64", 8)):
             p = spb.Plan(A, var, ipt, encoding=enc)
+            p.spmv_dev(dx.ptr, dy.ptr)
+            spb.L.check(lib.vnec_sync_ext(), "sync")
+            variant_y[name] = dy.to_array(np.float64, m.nrows)
             km, _ = p.time_kernels(dx.ptr, dy.ptr, 50)
             gm = p.time(dx.ptr, dy.ptr, 100, graph=True)
             bv = m.algorithmic_bytes(vb)
@@ -236,11 +307,18 @@ def main():
         if tj.get("variant") == info.variant:
             traffic = tj.get("hbm_bytes_per_launch")
 
+    sel = None
+    if args.selector_matrices > 0:
+        log("selector comparison ...")
+        sel = selector_eval(spb, args.selector_matrices)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
         with _QuietStdout():
-            cpu = cpu_baseline(m, k, y_gpu)
+            cpu = cpu_baseline(m, k, y_gpu, variant_y)
+        for name, err in cpu.pop("variant_errors").items():
+            variants[name]["max_rel_row_err_vs_reference"] = err
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1,
@@ -249,12 +327,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": desc, "nrows": m.nrows, "nnz": m.nnz, "nrhs": k,
                    "algorithmic_bytes": B, "variant": info.variant, "ipt": info.ipt,
-                   "tiles": info.ntiles, "parallelism": "single GPU"},
+                   "grid": info.grid, "tiles": info.ntiles,
+                   "selection": args.variant or "learned", "parallelism": "single GPU"},
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "gflops": round(2.0 * m.nnz * k / (ms_step / 1e3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": KERNEL_NAME.get(info.variant, "?"),
+                     "traffic": traffic,
+                     "kernel": KERNEL_NAME.get((info.variant, info.grid >= 0), "?"),
                      "kernel_us": round(k_ms * 1e3, 2), "fixup_us": round(fix_ms * 1e3, 2)},
         "graph_spmv_us": round(graph_ms * 1e3, 2),
         "copy_peak_gbs": round(cp.value, 1),
@@ -262,6 +342,7 @@ def main():
         "plan_build_ms": round(build_ms, 1),
         **extra,
         "variants": variants,
+        "selector": sel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

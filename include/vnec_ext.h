@@ -66,8 +66,20 @@ typedef enum {
     VNEC_ENC_N
 } vnec_encoding;
 
+/* Negative vnec_tuning.variant values: how the plan chooses its variant,
+ * tile granularity and schedule. */
+typedef enum {
+    VNEC_SELECT_LEARNED = -1,   /* the trained MLP over matrix statistics (the
+                                   rule below when no model is installed)    */
+    VNEC_SELECT_AUTOTUNE = -2,  /* exhaustive: build and time every candidate */
+    VNEC_SELECT_RULE = -3,      /* S when 64-row slice padding <= 1.15, else
+                                   persistent L at 7 items per thread        */
+    VNEC_SELECT_REFERENCE = -4, /* the reference dispatch (nrows >= 3e6: L;
+                                   else IRD at P = 36, L = 8: D -> E, S -> S) */
+} vnec_select_mode;
+
 typedef struct {
-    int variant;  /* vnec_variant, or -1: choose with the learned selector */
+    int variant;  /* vnec_variant, or a vnec_select_mode                    */
     int ipt;      /* merge items per thread (tile = 256 * ipt items)       */
     int lanes;    /* D block width / S slice height                        */
     int encoding; /* vnec_encoding                                         */
@@ -141,6 +153,89 @@ SpB_Info vnec_d_layout_ext(const SpB_Matrix A, int64_t P, int L, int32_t *v_row_
 /* IRD ratio (float)nnz / (float)(G * L) and the D / S decision at (P, L,
  * threshold); IRD_VNEC uses P = 36, L = 8, 0.477 (ird_calc.c:2-123). */
 SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *ird, int *decision);
+
+/* --------------------------------------------- autotuner and selector
+ * No counterpart in the reference (its only selector is IRD_VNEC plus the
+ * row-count gate of its test harness); these serve callers that tune,
+ * train or audit the variant choice. */
+
+/* Candidate configurations (single right-hand side, FP64).  Copies up to
+ * max of them to out (may be NULL); returns VNEC_NCAND. */
+#define VNEC_NCAND 14
+int vnec_candidates_ext(vnec_tuning *out, int max);
+/* Which candidates may run on A: bit c set when candidate c applies
+ * (slice candidates need padding <= 3 and fewer than 2^31 slots). */
+SpB_Info vnec_candidate_mask_ext(const SpB_Matrix A, uint32_t *mask);
+/* Exhaustive autotune: builds every applicable candidate, times it with
+ * hipGraph replay (best of 3 means over reps SpMVs) and returns the times
+ * (ms; INFINITY for candidates that do not apply) and the fastest. */
+SpB_Info vnec_autotune_ext(const SpB_Matrix A, int reps, double times_ms[VNEC_NCAND], int *best);
+
+/* Integer statistics of A computed on the GPU (order-independent integer
+ * sums, maxima and histograms, so identical to a CPU computation): */
+#define VNEC_NFEAT_INTS 32
+#define VNEC_NFEAT 32
+#define VNEC_FEAT_NBUCKETS 16
+enum {
+    VF_NROWS = 0,
+    VF_NCOLS,
+    VF_NNZ,
+    VF_SUM_LEN2,    /* sum of squared row lengths (mod 2^64)                   */
+    VF_SUM_LEN3,    /* sum of cubed row lengths (mod 2^64)                     */
+    VF_MAX_LEN,     /* longest row                                              */
+    VF_EMPTY_ROWS,  /* rows without entries                                     */
+    VF_HIST,        /* 16 buckets: rows of length in [2^b, 2^(b+1)), b = 15 open */
+    VF_GINI_NUM = VF_HIST + VNEC_FEAT_NBUCKETS,
+                    /* sum over row pairs i < j of |l_i - l_j|, l = min(len, 65535) */
+    VF_SUM_DIST,    /* sum over entries of |col - row * ncols / nrows|          */
+    VF_MAX_DIST,    /* bandwidth: largest such distance                         */
+    VF_NEAR_DIAG,   /* entries with distance <= 32                              */
+    VF_SLICE_SLOTS, /* slots of the 64-row slice layout (64 x longest row)      */
+    VF_NEC_SMALL,   /* distinct columns summed over merge-path windows of 4096 items */
+    VF_RUNS_SMALL,  /* runs of consecutive distinct columns, same windows       */
+    VF_NEC_LARGE,   /* as above, windows of 16384 items                         */
+    VF_RUNS_LARGE,
+};
+SpB_Info vnec_feature_ints_ext(const SpB_Matrix A, uint64_t ints[VNEC_NFEAT_INTS]);
+/* The selector's 32 FP32 inputs from the integer statistics. */
+void vnec_features_from_ints_ext(const uint64_t ints[VNEC_NFEAT_INTS], float f[VNEC_NFEAT]);
+/* Candidate mask from the integer statistics (same rule as
+ * vnec_candidate_mask_ext). */
+uint32_t vnec_candidate_mask_from_ints_ext(const uint64_t ints[VNEC_NFEAT_INTS]);
+
+/* MLP 32 -> 64 -> 64 -> VNEC_NCAND, ReLU, FP32.  Output c predicts
+ * log2(time of candidate c / time of the fastest candidate).  Parameters
+ * are one FP32 array: W1[32][64], b1[64], W2[64][64], b2[64],
+ * W3[64][VNEC_NCAND], b3[VNEC_NCAND] (input-major matrices). */
+#define VNEC_MLP_HIDDEN 64
+#define VNEC_MLP_NPARAMS                                                                        \
+    (VNEC_NFEAT * VNEC_MLP_HIDDEN + VNEC_MLP_HIDDEN + VNEC_MLP_HIDDEN * VNEC_MLP_HIDDEN +      \
+     VNEC_MLP_HIDDEN + VNEC_MLP_HIDDEN * VNEC_NCAND + VNEC_NCAND)
+/* The following run HIP kernels; all buffers are host memory. */
+/* out[B][VNEC_NCAND] = MLP(X[B][32]) (X already standardised). */
+SpB_Info vnec_mlp_forward_ext(const float *params, const float *X, int64_t B, float *out);
+/* Loss sum_{b,c} mask (out - T)^2 / sum mask and its gradient (NPARAMS). */
+SpB_Info vnec_mlp_grad_ext(const float *params, const float *X, const float *T, const float *mask,
+                           int64_t B, float *grad, float *loss);
+/* Adam training on the GPU: epochs over N samples in minibatches of
+ * `batch` (order reshuffled each epoch from seed); params updated in
+ * place; loss_hist (epochs entries, may be NULL) gets each epoch's mean
+ * loss. */
+SpB_Info vnec_mlp_train_ext(float *params, const float *X, const float *T, const float *mask,
+                            int64_t N, int epochs, int batch, float lr, float weight_decay,
+                            uint64_t seed, float *loss_hist);
+/* Inference kernel: choice[b] = argmin over candidates allowed by mask[b]
+ * of MLP((features[b] - mu) / sd) (lowest index on ties; -1 if none). */
+SpB_Info vnec_mlp_select_ext(const float *params, const float *mu, const float *sd,
+                             const float *features, const uint32_t *mask, int64_t B,
+                             int32_t *choice);
+/* Install the model VNEC_SELECT_LEARNED uses (copied; NULL params restores
+ * the model compiled into the library).  *trained reports whether a model
+ * is in force. */
+SpB_Info vnec_selector_install_ext(const float *params, const float *mu, const float *sd);
+SpB_Info vnec_selector_info_ext(int *trained, int64_t *train_samples);
+/* The candidate a selection mode picks for A (single right-hand side). */
+SpB_Info vnec_select_ext(const SpB_Matrix A, int mode, int *candidate);
 
 /* ------------------------------------------------------- plan lifetimes
  * The reference has no destroy call for its plans; these release both the

@@ -150,6 +150,31 @@ int vnec_hip_spmv_slice(const vnec_slice_args *a, void *stream);
 /* Overwrites n bytes (cache flush helper for timing). */
 int vnec_hip_scribble(void *p, size_t n, void *stream);
 
+/* ---- selector: integer statistics (stats: 32 uint64, caller-zeroed;
+ * lenhist: 65536 uint32 counts of min(row length, 65535), caller-zeroed) */
+int vnec_hip_feature_rows(const int32_t *ptr, int64_t nrows, uint64_t *stats, uint32_t *lenhist,
+                          void *stream);
+int vnec_hip_feature_cols(const int32_t *ptr, const int32_t *col, int64_t nrows, int64_t ncols,
+                          uint64_t *stats, void *stream);
+
+/* ---- selector: MLP kernels (FP32 parameters laid out as in vnec_ext.h) */
+int vnec_hip_mlp_forward(const float *params, const float *X, int64_t B, float *out, void *stream);
+/* Gradient partials of the masked squared loss scaled by inv_den over the
+ * samples idx[0..B) (idx may be NULL: samples 0..B-1), chunks of `chunk`
+ * samples: part[ceil(B / chunk)][NPARAMS], loss_part[ceil(B / chunk)]. */
+int vnec_hip_mlp_grad(const float *params, const float *X, const float *T, const float *mask,
+                      const int32_t *idx, int64_t B, int chunk, float inv_den, float *part,
+                      float *loss_part, void *stream);
+/* Sums nchunks partials in chunk order into grad (may be NULL) and, if
+ * lr > 0, applies one Adam step (bias corrections bc1 = 1 - beta1^t,
+ * bc2 = 1 - beta2^t) to params with state m, v. */
+int vnec_hip_mlp_reduce_adam(const float *part, int nchunks, float *grad, float *params, float *m,
+                             float *v, float lr, float weight_decay, float bc1, float bc2,
+                             void *stream);
+int vnec_hip_mlp_select(const float *params, const float *mu, const float *sd,
+                        const float *features, const uint32_t *mask, int64_t B, int32_t *choice,
+                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

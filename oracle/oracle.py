@@ -56,6 +56,7 @@ def lib():
             "orc_spmv_S": (None, [vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                   C.c_int]),
             "orc_num_threads": (C.c_int, []),
+            "orc_feature_ints": (None, [vp, vp, i64, i64, vp]),
         }
         for k, (res, args) in sig.items():
             f = getattr(o, k)
@@ -67,6 +68,16 @@ def lib():
 
 def nthreads() -> int:
     return int(lib().orc_num_threads())
+
+
+# ------------------------------------------------------- selector statistics
+def feature_ints(ptr, col, ncols: int) -> np.ndarray:
+    """The selector's 32 integer statistics (selector_oracle.c)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    out = np.zeros(32, dtype=np.uint64)
+    lib().orc_feature_ints(_ptr(ptr), _ptr(col), len(ptr) - 1, ncols, _ptr(out))
+    return out
 
 
 # ------------------------------------------------------------------ a1/a2

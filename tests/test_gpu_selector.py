@@ -1,0 +1,211 @@
+"""GPU checks of the selector: statistics kernels bit-exact against the CPU
+oracle, the HIP MLP forward / backward / training / inference kernels
+against the FP64 oracle, autotuner invariants, and SpMV parity of plans
+built through every selection mode.
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as MO
+import oracle as O
+from conftest import load_golden
+from vnec_amd import _lib as L
+from vnec_amd import matgen as g
+from vnec_amd import spb
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if spb.device_count() < 1:
+        pytest.fail("no GPU visible: the gpu tests need an MI355X")
+    spb.init()
+
+
+def _mat(m):
+    return spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+
+
+def _long_row_case():
+    rng = np.random.default_rng(9)
+    n, ncols = 1000, 100000
+    lens = rng.integers(0, 30, n)
+    lens[17] = 70000                          # longer than the 65535-bin histogram
+    lens[900:] = 0                            # empty tail rows
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(rng.choice(ncols, k, replace=False)) for k in lens]).astype(np.int32)
+    return g.from_pattern(n, ncols, ptr, col, seed=3)
+
+
+def _stat_cases():
+    d = load_golden("rmat10")
+    return [("rmat10", g.from_pattern(len(d["ptr"]) - 1, len(d["ptr"]) - 1, d["ptr"], d["col"])),
+            ("rmat13", g.rmat(13, 8, 0.5, 0.2, 0.2, seed=21)),
+            ("stencil12", g.stencil27(12, seed=22)),
+            ("banded", g.banded(10001, 17, seed=23)),
+            ("blockdiag", g.blockdiag(5000, 100, seed=24)),
+            ("powerlaw", g.powerlaw(20000, 1.6, 1, 20000, seed=25)),
+            ("uniform_rect", g.from_pattern(3000, 777, np.arange(0, 3001 * 5, 5),
+                                            np.random.default_rng(1).integers(0, 777, 15000)
+                                            .astype(np.int32))),
+            ("long_row", _long_row_case())]
+
+
+@pytest.mark.parametrize("name,m", _stat_cases(), ids=lambda v: v if isinstance(v, str) else "")
+def test_feature_ints_bit_exact(name, m):
+    A = _mat(m)
+    gpu = spb.feature_ints(A)
+    cpu = O.feature_ints(m.ptr, m.col, m.ncols)
+    np.testing.assert_array_equal(gpu, cpu)
+    A.free()
+
+
+def _toy(seed, B):
+    rng = np.random.default_rng(seed)
+    p = MO.init_params(seed)
+    X = rng.normal(size=(B, MO.NF)).astype(np.float32)
+    T = rng.normal(size=(B, MO.NC)).astype(np.float32)
+    M = (rng.random((B, MO.NC)) < 0.8).astype(np.float32)
+    return p, X, T, M
+
+
+def test_mlp_forward_matches_fp64():
+    p, X, _, _ = _toy(1, 300)
+    out = spb.mlp_forward(p, X)
+    ref, _ = MO.forward(p.astype(np.float64), X.astype(np.float64))
+    np.testing.assert_allclose(out, ref, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("B", [1, 37, 500])
+def test_mlp_gradient_matches_fp64(B):
+    p, X, T, M = _toy(2 + B, B)
+    M[0, 0] = 1.0
+    g_gpu, loss = spb.mlp_grad(p, X, T, M)
+    loss_ref, g_ref = MO.loss_grad(p.astype(np.float64), X, T, M)
+    assert abs(loss - loss_ref) <= 1e-4 * abs(loss_ref)
+    err = np.abs(g_gpu - g_ref).max() / np.abs(g_ref).max()
+    assert err < 1e-4, err
+    # and the FP64 gradient is itself the derivative of the loss
+    idx = np.random.default_rng(B).choice(MO.NPARAMS, 40, replace=False)
+    np.testing.assert_allclose(g_ref[idx], MO.fd_grad(p, X, T, M, idx), rtol=1e-5, atol=1e-8)
+
+
+def test_mlp_training_fits_and_is_deterministic():
+    rng = np.random.default_rng(5)
+    B = 600
+    X = rng.normal(size=(B, MO.NF)).astype(np.float32)
+    W = rng.normal(size=(MO.NF, MO.NC)) / np.sqrt(MO.NF)
+    T = np.tanh(X @ W).astype(np.float32)                  # learnable target
+    M = np.ones((B, MO.NC), np.float32)
+    p0 = MO.init_params(7)
+    p1, h1 = spb.mlp_train(p0, X, T, M, epochs=60, batch=64, lr=3e-3, seed=11)
+    p2, h2 = spb.mlp_train(p0, X, T, M, epochs=60, batch=64, lr=3e-3, seed=11)
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_array_equal(h1, h2)
+    l0 = MO.loss_only(p0, X, T, M)
+    l1 = MO.loss_only(p1, X, T, M)
+    assert l1 < 0.1 * l0, (l0, l1)
+    assert h1[-1] < 0.2 * h1[0]
+
+
+def test_mlp_select_matches_fp64():
+    rng = np.random.default_rng(6)
+    B = 400
+    p = MO.init_params(8)
+    F = rng.normal(size=(B, MO.NF)).astype(np.float32) * 3 + 1
+    mu = rng.normal(size=MO.NF).astype(np.float32)
+    sd = (rng.random(MO.NF) + 0.5).astype(np.float32)
+    masks = rng.integers(0, 1 << MO.NC, B).astype(np.uint32)
+    masks[:5] = 0
+    masks[5:10] = (1 << MO.NC) - 1
+    ch = spb.mlp_select(p, mu, sd, F, masks)
+    ref, o = MO.select(p, mu, sd, F, masks)
+    assert (ch[:5] == -1).all()
+    for b in range(B):
+        if ref[b] < 0:
+            assert ch[b] == -1
+            continue
+        assert (masks[b] >> ch[b]) & 1
+        # equal, or a near-tie in FP32
+        assert ch[b] == ref[b] or abs(o[b, ch[b]] - o[b, ref[b]]) < 1e-4
+
+
+def test_autotune_invariants():
+    m = g.powerlaw(30000, 1.7, 2, 20000, seed=31)   # slice padding well above 3
+    A = _mat(m)
+    ints = spb.feature_ints(A)
+    mask = spb.mask_from_ints(ints)
+    assert mask == spb.candidate_mask(A)
+    t, best = spb.autotune(A, 10)
+    cand = spb.candidates()
+    for c in range(L.NCAND):
+        assert np.isfinite(t[c]) == bool((mask >> c) & 1)
+        if cand[c].variant == "S":
+            assert not np.isfinite(t[c])
+    assert best == int(np.argmin(t)) and (t[np.isfinite(t)] >= t[best]).all()
+    t2, best2 = spb.autotune(A, 10)
+    assert t2[best2] <= 1.5 * t[best] and t[best] <= 1.5 * t2[best2]
+    A.free()
+
+
+@pytest.mark.parametrize("mode", ["learned", "autotune", "rule", "reference"])
+def test_selection_modes_parity(mode):
+    for m in (g.stencil27(20, seed=41), g.rmat(12, 16, seed=42), g.powerlaw(8000, 1.8, 2, 4000, seed=43)):
+        A = _mat(m)
+        c = spb.select(A, mode)
+        assert 0 <= c < L.NCAND and (spb.candidate_mask(A) >> c) & 1
+        p = spb.Plan(A, mode)
+        info = p.info()
+        want = spb.candidates()[c]
+        assert (info.variant, info.ipt) == (want.variant, want.ipt)
+        xv = spb.Vector.from_array(m.x)
+        yv = spb.Vector(m.nrows)
+        p.spmv(yv, xv)
+        y = yv.to_array()
+        yr = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+        ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+        ok, worst = O.check_rows(y, yr, ax)
+        assert ok, (mode, worst)
+        p.free()
+        A.free()
+
+
+def test_installed_model_drives_learned_mode():
+    """A model that prefers candidate c everywhere makes the learned mode pick c."""
+    p = np.zeros(MO.NPARAMS, np.float32)
+    b3 = MO.NPARAMS - MO.NC
+    p[b3:] = 1.0
+    p[b3 + 8] = -1.0                                   # L, 7 items/thread, one workgroup per tile
+    mu = np.zeros(MO.NF, np.float32)
+    sd = np.ones(MO.NF, np.float32)
+    spb.selector_install(p, mu, sd)
+    try:
+        m = g.rmat(11, 8, seed=44)
+        A = _mat(m)
+        assert spb.select(A, "learned") == 8
+        A.free()
+    finally:
+        spb.selector_install(None)
+
+
+def test_learned_mode_follows_compiled_model_on_held_out():
+    """The library's learned mode (GPU statistics -> host features -> HIP
+    inference kernel with the compiled-in model) picks what the FP64 model
+    picks on the same matrices: six regenerated held-out matrices."""
+    from vnec_amd.selector import model_file, specs
+    trained, _ = spb.selector_info()
+    assert trained
+    params, mu, sd = model_file.load()
+    for i in (2000, 2001, 2003, 2005, 2007, 2009):
+        s = specs.spec(i)
+        if s.get("n", 0) > 300000 or s.get("scale", 0) > 17:
+            continue
+        m = specs.generate(s)
+        A = _mat(m)
+        ints = spb.feature_ints(A)
+        f = spb.features_from_ints(ints)
+        ch, o = MO.select(params, mu, sd, f[None, :], np.array([spb.mask_from_ints(ints)]))
+        got = spb.select(A, "learned")
+        assert got == ch[0] or abs(o[0, got] - o[0, ch[0]]) < 1e-4, (i, got, ch[0])
+        A.free()

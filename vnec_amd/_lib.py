@@ -116,6 +116,20 @@ SIGNATURES = {
     "vnec_d_layout_ext": (Info, [vp, i64, C.c_int, vp, vp, vp, C.POINTER(i64)]),
     "vnec_ird_ext": (Info, [vp, i64, C.c_int, C.c_double, C.POINTER(C.c_float),
                             C.POINTER(C.c_int)]),
+    "vnec_candidates_ext": (C.c_int, [vp, C.c_int]),
+    "vnec_candidate_mask_ext": (Info, [vp, C.POINTER(C.c_uint32)]),
+    "vnec_autotune_ext": (Info, [vp, C.c_int, vp, C.POINTER(C.c_int)]),
+    "vnec_feature_ints_ext": (Info, [vp, vp]),
+    "vnec_features_from_ints_ext": (None, [vp, vp]),
+    "vnec_candidate_mask_from_ints_ext": (C.c_uint32, [vp]),
+    "vnec_mlp_forward_ext": (Info, [vp, vp, i64, vp]),
+    "vnec_mlp_grad_ext": (Info, [vp, vp, vp, vp, i64, vp, C.POINTER(C.c_float)]),
+    "vnec_mlp_train_ext": (Info, [vp, vp, vp, vp, i64, C.c_int, C.c_int, C.c_float, C.c_float,
+                                  u64, vp]),
+    "vnec_mlp_select_ext": (Info, [vp, vp, vp, vp, vp, i64, vp]),
+    "vnec_selector_install_ext": (Info, [vp, vp, vp]),
+    "vnec_selector_info_ext": (Info, [C.POINTER(C.c_int), C.POINTER(i64)]),
+    "vnec_select_ext": (Info, [vp, C.c_int, C.POINTER(C.c_int)]),
     "VNEC_D_free_fp64_ext": (Info, [vp]),
     "VNEC_S_free_fp64_ext": (Info, [vp]),
     "VNEC_L_free_ext": (Info, [vp]),
@@ -142,7 +156,23 @@ SIGNATURES = {
     "vnec_hip_slice_fill": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
+    "vnec_hip_feature_rows": (C.c_int, [vp, i64, vp, vp, vp]),
+    "vnec_hip_feature_cols": (C.c_int, [vp, vp, i64, i64, vp, vp]),
+    "vnec_hip_mlp_forward": (C.c_int, [vp, vp, i64, vp, vp]),
+    "vnec_hip_mlp_grad": (C.c_int, [vp, vp, vp, vp, vp, i64, C.c_int, C.c_float, vp, vp, vp]),
+    "vnec_hip_mlp_reduce_adam": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_float, C.c_float,
+                                           C.c_float, C.c_float, vp]),
+    "vnec_hip_mlp_select": (C.c_int, [vp, vp, vp, vp, vp, i64, vp, vp]),
 }
+
+# Constants of include/vnec_ext.h used by the selector bindings.
+NCAND = 14
+NFEAT = 32
+NFEAT_INTS = 32
+MLP_HIDDEN = 64
+MLP_NPARAMS = NFEAT * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN \
+    + MLP_HIDDEN * NCAND + NCAND
+SELECT_LEARNED, SELECT_AUTOTUNE, SELECT_RULE, SELECT_REFERENCE = -1, -2, -3, -4
 
 _lib = None
 

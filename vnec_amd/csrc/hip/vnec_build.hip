@@ -113,8 +113,8 @@ static int scan_rec(const int32_t *in, int32_t *out, int64_t n, hipStream_t s)
         rc = (int)hipGetLastError();
     }
     hipError_t e = hipStreamSynchronize(s);
-    hipFree(sums);
-    return rc ? rc : (int)e;
+    hipError_t f = hipFree(sums);
+    return rc ? rc : (e ? (int)e : (int)f);
 }
 
 extern "C" int vnec_hip_scan_excl_i32(const int32_t *in, int32_t *out, int64_t n,
@@ -303,8 +303,8 @@ extern "C" int vnec_hip_max_tile_nnz(const int32_t *coords, int64_t ntiles, int3
     int rc = (int)hipGetLastError();
     if (!rc) rc = (int)hipMemcpyAsync(max_host, d, 4, hipMemcpyDeviceToHost, s);
     if (!rc) rc = (int)hipStreamSynchronize(s);
-    hipFree(d);
-    return rc;
+    hipError_t f = hipFree(d);
+    return rc ? rc : (int)f;
 }
 
 // ------------------------------------------------------ ECR, dense form

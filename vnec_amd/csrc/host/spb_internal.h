@@ -52,6 +52,7 @@ typedef struct spb_csr {
     int32_t *d_col;     /* nnz       */
     double *d_val;      /* nnz       */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
+    int tuned_cand;     /* autotuner's best candidate, -1 until measured */
 } spb_csr;
 
 typedef struct spb_dense {
@@ -117,5 +118,16 @@ SpB_Info spb_vector_device_alloc(spb_dense *v);
 
 /* ---- plans (spb_plan.c) ---- */
 SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev);
-int vnec_select_tuning(const spb_csr *a, vnec_tuning *t);
 double vnec_slice_padding(const spb_csr *a);
+
+/* ---- autotuner and selector (spb_tune.c) ---- */
+/* Tuning chosen by a selection mode (vnec_select_mode) for one FP64
+ * right-hand side. */
+SpB_Info vnec_select_tuning(spb_csr *a, int mode, vnec_tuning *t);
+
+/* ---- compiled-in selector model (spb_selector_model.c, generated) ---- */
+extern const int vnec_model_trained;
+extern const int64_t vnec_model_train_samples;
+extern const float vnec_model_params[VNEC_MLP_NPARAMS];
+extern const float vnec_model_mu[VNEC_NFEAT];
+extern const float vnec_model_sd[VNEC_NFEAT];

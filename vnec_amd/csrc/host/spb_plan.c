@@ -221,7 +221,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (t && t->variant >= 0) {
         tune = *t;
     } else {
-        if (vnec_select_tuning(a, &tune)) return SpB_FAILURE;
+        if (vnec_select_tuning(a, t ? t->variant : VNEC_SELECT_LEARNED, &tune) != SpB_SUCCESS)
+            return SpB_FAILURE;
         if (t) {
             tune.encoding = t->encoding;
             if (t->nrhs > 0) tune.nrhs = t->nrhs;
@@ -568,10 +569,7 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
     return SpB_SUCCESS;
 }
 
-/* -------------------------------------------------------------- selector
- * Rule used when no learned model is loaded: 64-row slices (variant S) when
- * padding them to their longest row costs at most 15 % extra slots,
- * otherwise the persistent merge-path kernel (variant L). */
+/* Slots of the 64-row slice layout per stored entry (1.0: no padding). */
 double vnec_slice_padding(const spb_csr *a)
 {
     int64_t nr = (int64_t)a->base.row, slots = 0;
@@ -584,22 +582,6 @@ double vnec_slice_padding(const spb_csr *a)
         slots += 64 * m;
     }
     return a->nnz ? (double)slots / (double)a->nnz : 1.0;
-}
-
-int vnec_select_tuning(const spb_csr *a, vnec_tuning *t)
-{
-    memset(t, 0, sizeof *t);
-    t->grid = 0;
-    t->variant = VNEC_VAR_L;
-    t->ipt = 7;
-    t->lanes = 64;
-    t->encoding = VNEC_ENC_FP64;
-    t->nrhs = 1;
-    if (a->nnz > 0 && vnec_slice_padding(a) <= 1.15) {
-        t->variant = VNEC_VAR_S;
-        t->ipt = 3;
-    }
-    return 0;
 }
 
 SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps)

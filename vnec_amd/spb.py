@@ -19,6 +19,9 @@ SpB_MATRIX_FORMAT = 1
 SpB_CSR = 1
 VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3}
 VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2}
+# Selection modes (negative vnec_tuning.variant values).
+SELECT = {"learned": L.SELECT_LEARNED, "autotune": L.SELECT_AUTOTUNE, "rule": L.SELECT_RULE,
+          "reference": L.SELECT_REFERENCE}
 
 _init = False
 
@@ -207,8 +210,9 @@ class Plan:
     def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
                  encoding: str = "fp64", nrhs: int = 1, grid: int = 0):
         init()
-        t = L.Tuning(VNEC_VAR[variant] if variant else -1, ipt, lanes, VNEC_ENC[encoding], nrhs,
-                     grid)
+        code = -1 if variant is None else (VNEC_VAR[variant] if variant in VNEC_VAR
+                                            else SELECT[variant])
+        t = L.Tuning(code, ipt, lanes, VNEC_ENC[encoding], nrhs, grid)
         self.h = C.c_void_p()
         self.A = A
         L.check(L.load().vnec_plan_build_ext(C.byref(self.h), A.h, C.byref(t)),
@@ -325,3 +329,123 @@ def d_layout(A: Matrix, P: int, L_: int, with_values: bool = True):
                                        _p(va) if with_values else None, C.byref(G)),
             "vnec_d_layout_ext")
     return vrp, cs[:g], (va[: g * L_] if with_values else None)
+
+
+# ------------------------------------------------ autotuner and selector
+def candidates() -> list[TuningInfo]:
+    """The candidate configurations (vnec_candidates_ext)."""
+    arr = (L.Tuning * L.NCAND)()
+    n = L.load().vnec_candidates_ext(arr, L.NCAND)
+    inv = {v: k for k, v in VNEC_VAR.items()}
+    return [TuningInfo(inv[t.variant], t.ipt, t.lanes, "fp64", t.nrhs, t.grid, 0, 0)
+            for t in arr[:n]]
+
+
+def feature_ints(A: Matrix) -> np.ndarray:
+    """The 32 integer statistics computed on the GPU (vnec_feature_ints_ext)."""
+    init()
+    out = np.zeros(L.NFEAT_INTS, np.uint64)
+    L.check(L.load().vnec_feature_ints_ext(A.h, _p(out)), "vnec_feature_ints_ext")
+    return out
+
+
+def features_from_ints(ints) -> np.ndarray:
+    ints = np.ascontiguousarray(ints, dtype=np.uint64)
+    f = np.zeros(L.NFEAT, np.float32)
+    L.load().vnec_features_from_ints_ext(_p(ints), _p(f))
+    return f
+
+
+def mask_from_ints(ints) -> int:
+    ints = np.ascontiguousarray(ints, dtype=np.uint64)
+    return int(L.load().vnec_candidate_mask_from_ints_ext(_p(ints)))
+
+
+def candidate_mask(A: Matrix) -> int:
+    init()
+    m = C.c_uint32()
+    L.check(L.load().vnec_candidate_mask_ext(A.h, C.byref(m)), "vnec_candidate_mask_ext")
+    return int(m.value)
+
+
+def autotune(A: Matrix, reps: int = 20) -> tuple[np.ndarray, int]:
+    """(per-candidate ms, inf where not applicable; index of the fastest)."""
+    init()
+    t = np.zeros(L.NCAND, np.float64)
+    b = C.c_int()
+    L.check(L.load().vnec_autotune_ext(A.h, reps, _p(t), C.byref(b)), "vnec_autotune_ext")
+    return t, int(b.value)
+
+
+def select(A: Matrix, mode: str = "learned") -> int:
+    init()
+    c = C.c_int()
+    L.check(L.load().vnec_select_ext(A.h, SELECT[mode], C.byref(c)), "vnec_select_ext")
+    return int(c.value)
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def mlp_forward(params, X) -> np.ndarray:
+    """HIP forward pass: (B, NCAND) outputs of standardised inputs X (B, 32)."""
+    init()
+    X = _f32(X, (-1, L.NFEAT))
+    out = np.zeros((X.shape[0], L.NCAND), np.float32)
+    L.check(L.load().vnec_mlp_forward_ext(_p(_f32(params)), _p(X), X.shape[0], _p(out)),
+            "vnec_mlp_forward_ext")
+    return out
+
+
+def mlp_grad(params, X, T, M) -> tuple[np.ndarray, float]:
+    """HIP backward pass: gradient of sum M (out - T)^2 / sum M, and the loss."""
+    init()
+    X = _f32(X, (-1, L.NFEAT))
+    g = np.zeros(L.MLP_NPARAMS, np.float32)
+    loss = C.c_float()
+    L.check(L.load().vnec_mlp_grad_ext(_p(_f32(params)), _p(X), _p(_f32(T)), _p(_f32(M)),
+                                       X.shape[0], _p(g), C.byref(loss)), "vnec_mlp_grad_ext")
+    return g, float(loss.value)
+
+
+def mlp_train(params, X, T, M, epochs: int, batch: int, lr: float, weight_decay: float = 0.0,
+              seed: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Adam training with the HIP kernels; returns (params, per-epoch loss)."""
+    init()
+    p = np.array(params, dtype=np.float32, copy=True)
+    X = _f32(X, (-1, L.NFEAT))
+    hist = np.zeros(max(epochs, 1), np.float32)
+    L.check(L.load().vnec_mlp_train_ext(_p(p), _p(X), _p(_f32(T)), _p(_f32(M)), X.shape[0],
+                                        epochs, batch, lr, weight_decay, seed, _p(hist)),
+            "vnec_mlp_train_ext")
+    return p, hist[:epochs]
+
+
+def mlp_select(params, mu, sd, F, masks) -> np.ndarray:
+    """HIP inference kernel: chosen candidate per row of raw features F."""
+    init()
+    F = _f32(F, (-1, L.NFEAT))
+    masks = np.ascontiguousarray(masks, dtype=np.uint32)
+    out = np.zeros(F.shape[0], np.int32)
+    L.check(L.load().vnec_mlp_select_ext(_p(_f32(params)), _p(_f32(mu)), _p(_f32(sd)), _p(F),
+                                         _p(masks), F.shape[0], _p(out)), "vnec_mlp_select_ext")
+    return out
+
+
+def selector_install(params=None, mu=None, sd=None):
+    init()
+    if params is None:
+        L.check(L.load().vnec_selector_install_ext(None, None, None), "vnec_selector_install_ext")
+    else:
+        L.check(L.load().vnec_selector_install_ext(_p(_f32(params)), _p(_f32(mu)), _p(_f32(sd))),
+                "vnec_selector_install_ext")
+
+
+def selector_info() -> tuple[bool, int]:
+    t, n = C.c_int(), C.c_int64()
+    L.check(L.load().vnec_selector_info_ext(C.byref(t), C.byref(n)), "vnec_selector_info_ext")
+    return bool(t.value), int(n.value)
