@@ -54,13 +54,17 @@ int vnec_hip_ecr_dense(const int32_t *col, int64_t nnz, int64_t ncols, const int
 
 /* VNEC-D block layout at (P, L) from ECR ranks ecr (nnz): pass 1 writes
  * each row's group count to cnt (nrows + 1 entries, cnt[nrows] = 0; its
- * exclusive scan is v_row_ptr); pass 2 writes col_start (G) and, if
- * val_align is not NULL, val_align (G * L, caller-zeroed). */
+ * exclusive scan is v_row_ptr) and sets *unsorted (device int, caller-
+ * zeroed, may be NULL) when some row's columns descend (the greedy
+ * grouping needs them ascending; pass 2 skips such entries); pass 2 writes col_start (G) and, if
+ * val_align is not NULL, val_align (G * L, caller-zeroed).  A column
+ * repeated within a row overwrites its slot (accumulate = 0, the
+ * reference's layout) or is added to it (accumulate = 1, for SpMV). */
 int vnec_hip_d_count(const int32_t *ptr, const int32_t *ecr, int64_t nrows, const int32_t *coords,
-                     int64_t P, int L, int32_t *cnt, void *stream);
+                     int64_t P, int L, int32_t *cnt, int32_t *unsorted, void *stream);
 int vnec_hip_d_fill(const int32_t *ptr, const int32_t *ecr, const double *val, int64_t nrows,
                     const int32_t *coords, int64_t P, int L, const int32_t *v_row_ptr,
-                    int32_t *col_start, double *val_align, void *stream);
+                    int32_t *col_start, double *val_align, int accumulate, void *stream);
 
 /* FP64 -> FP32 hi + FP32 lo split: hi = fl32(v), lo = fl32(v - hi). */
 int vnec_hip_split_hilo(const double *val, int64_t n, float *hi, float *lo, void *stream);
@@ -146,6 +150,34 @@ int vnec_hip_slice_fill(const int32_t *ptr, const int32_t *col, const uint16_t *
                         const int32_t *slice_len, const int32_t *slice_ptr, double *sv, float *shi,
                         float *slo, int32_t *scol, uint16_t *slcol, void *stream);
 int vnec_hip_spmv_slice(const vnec_slice_args *a, void *stream);
+
+/* ---- variant D: dense column blocks of the ECR window ---- */
+typedef struct {
+    const int32_t *ptr;       /* nrows + 1                                     */
+    const int32_t *vrp;       /* nrows + 1: groups of row i are [vrp[i], vrp[i+1]) */
+    const int32_t *col_start; /* groups: first window position of each group   */
+    const double *vals;       /* groups * L, blocks of 64 groups stored [j][lane] */
+    const double *val;        /* nnz: CSR values (each tile's carried row)     */
+    const uint16_t *lcol;     /* nnz: tile-local window ranks                  */
+    const int32_t *coords;    /* 2 (ntiles + 1)                                */
+    int64_t ntiles;
+    const int32_t *run_off, *run_start, *run_wpos, *nec; /* windows            */
+    const double *x;
+    double *y;
+    double *carry;            /* ntiles                                        */
+    int64_t nrows;
+    int ipt;
+    int lanes;                /* L: 4, 8 or 16                                 */
+} vnec_d_args;
+
+/* Wave64 layout of a D value array: in[g * L + j] -> out[((g / 64) * L + j)
+ * * 64 + g % 64]; out holds ceil(G / 64) * 64 * L doubles. */
+int vnec_hip_d_interleave(const double *in, int64_t G, int L, double *out, void *stream);
+int vnec_hip_spmv_d(const vnec_d_args *a, void *stream);
+int vnec_hip_d_supported(int ipt, int lanes);
+
+/* y[0 .. n) = 0 (a kernel, safe inside captured graphs). */
+int vnec_hip_zero_f64(double *y, int64_t n, void *stream);
 
 /* Overwrites n bytes (cache flush helper for timing). */
 int vnec_hip_scribble(void *p, size_t n, void *stream);

@@ -31,8 +31,10 @@ def _mat(m):
     return spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
 
 
-def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1, grid=0):
-    p = spb.Plan(A, variant, ipt, encoding=enc, nrhs=nrhs, grid=grid)
+def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1, grid=0, lanes=None):
+    if lanes is None:
+        lanes = 8 if variant == "D" else 64
+    p = spb.Plan(A, variant, ipt, lanes=lanes, encoding=enc, nrhs=nrhs, grid=grid)
     xv = spb.Vector.from_array(x)
     yv = spb.Vector(A.nrows * nrhs)
     p.spmv(yv, xv)
@@ -94,6 +96,26 @@ def test_slice_variant_matches_oracle(name, m, ipt):
     assert ok, worst
 
 
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("ipt,lanes", [(5, 4), (7, 8), (9, 16), (5, 16)])
+def test_d_variant_matches_oracle(name, m, ipt, lanes):
+    """Variant D (dense groups of `lanes` window positions, wave64 layout).
+    Its greedy grouping needs ascending columns in every row (the CSR
+    builder sorts them); a matrix with a descending row is refused."""
+    A = _mat(m)
+    print(f"case {name} variant D ipt {ipt} lanes {lanes}", flush=True)
+    rows_sorted = all((np.diff(m.col[m.ptr[i]:m.ptr[i + 1]]) >= 0).all() for i in range(m.nrows))
+    if not rows_sorted:
+        with pytest.raises(RuntimeError, match="ascending"):
+            _run_plan(A, m.x, "D", ipt, lanes=lanes)
+        return
+    y = _run_plan(A, m.x, "D", ipt, lanes=lanes)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y0, ax, TOL)
+    assert ok, worst
+
+
 @pytest.mark.parametrize("variant", ["L", "E", "S"])
 @pytest.mark.parametrize("enc", ["hilo", "hi"])
 def test_encodings_error_bounds(variant, enc):
@@ -129,7 +151,7 @@ def test_known_answer_is_exact():
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
     rows = np.repeat(np.arange(m.nrows), np.diff(m.ptr))
     expect = np.bincount(rows, weights=rows * m.col.astype(np.float64), minlength=m.nrows)
-    for variant, ipt in (("L", 7), ("E", 7), ("S", 3)):
+    for variant, ipt in (("L", 7), ("E", 7), ("S", 3), ("D", 7)):
         assert np.array_equal(_run_plan(A, x, variant, ipt), expect)
 
 
@@ -141,8 +163,10 @@ def test_empty_rows_and_empty_matrix():
     y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
     for variant in ("L", "E", "S"):
         assert np.allclose(_run_plan(A, m.x, variant, 3), y0, rtol=0, atol=1e-15)
+    assert np.allclose(_run_plan(A, m.x, "D", 5), y0, rtol=0, atol=1e-15)
     A = spb.Matrix.from_csr(5, 4, [0, 0, 0, 0, 0, 0], [], [])
     assert np.array_equal(_run_plan(A, np.ones(4), "S", 1), np.zeros(5))
+    assert np.array_equal(_run_plan(A, np.ones(4), "D", 7), np.zeros(5))
 
 
 # -------------------------------------------------- builder structures
@@ -280,10 +304,30 @@ def test_graph_timing_path_gives_same_result():
     assert O.check_rows(y, O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
 
 
-@pytest.mark.parametrize("variant,ipt", [("L", 7), ("E", 7), ("S", 3)])
+@pytest.mark.parametrize("variant,ipt", [("L", 7), ("E", 7), ("S", 3), ("D", 7)])
 def test_full_size_stencil_per_row_gate(variant, ipt):
     m = g.stencil27(128, seed=1)
     A = _mat(m)
     y = _run_plan(A, m.x, variant, ipt)
     ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     assert O.check_rows(y, yl, ax, TOL)[0]
+
+
+def test_repeated_columns_are_summed_by_every_variant():
+    """Duplicates are kept by the CSR builder (SpB_Matrix_build_FP64), so a
+    row may name a column twice; every kernel must add both entries."""
+    rng = np.random.default_rng(5)
+    n = 900
+    lens = rng.integers(0, 12, n)
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.sort(np.concatenate([c, c[:2]])) if len(c) > 2 else np.sort(c)
+                          for c in (rng.choice(n, k, replace=False) for k in lens)]).astype(np.int32)
+    lens2 = np.array([k + 2 if k > 2 else k for k in lens])
+    ptr = np.concatenate([[0], np.cumsum(lens2)]).astype(np.int64)
+    m = g.from_pattern(n, n, ptr, col, seed=6)
+    A = _mat(m)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    for variant, ipt in (("L", 7), ("E", 5), ("S", 3), ("D", 5), ("D", 9)):
+        ok, worst = O.check_rows(_run_plan(A, m.x, variant, ipt), y0, ax, TOL)
+        assert ok, (variant, worst)

@@ -141,8 +141,8 @@ SIGNATURES = {
     "vnec_hip_ecr_tiles_pass2": (C.c_int, [vp, i64, vp, vp, vp, vp, vp, vp]),
     "vnec_hip_max_tile_nnz": (C.c_int, [vp, i64, vp, vp]),
     "vnec_hip_ecr_dense": (C.c_int, [vp, i64, i64, vp, i64, vp, vp, vp, vp, vp, vp]),
-    "vnec_hip_d_count": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp, vp]),
-    "vnec_hip_d_fill": (C.c_int, [vp, vp, vp, i64, vp, i64, C.c_int, vp, vp, vp, vp]),
+    "vnec_hip_d_count": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp, vp, vp]),
+    "vnec_hip_d_fill": (C.c_int, [vp, vp, vp, i64, vp, i64, C.c_int, vp, vp, vp, C.c_int, vp]),
     "vnec_hip_split_hilo": (C.c_int, [vp, i64, vp, vp, vp]),
     "vnec_hip_spmv_merge": (C.c_int, [vp, vp]),
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),
@@ -156,6 +156,10 @@ SIGNATURES = {
     "vnec_hip_slice_fill": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
+    "vnec_hip_zero_f64": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_d_interleave": (C.c_int, [vp, i64, C.c_int, vp, vp]),
+    "vnec_hip_spmv_d": (C.c_int, [vp, vp]),
+    "vnec_hip_d_supported": (C.c_int, [C.c_int, C.c_int]),
     "vnec_hip_feature_rows": (C.c_int, [vp, i64, vp, vp, vp]),
     "vnec_hip_feature_cols": (C.c_int, [vp, vp, i64, i64, vp, vp]),
     "vnec_hip_mlp_forward": (C.c_int, [vp, vp, i64, vp, vp]),

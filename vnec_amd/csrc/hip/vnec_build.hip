@@ -414,6 +414,24 @@ __global__ void k_scribble(uint4 *p, int64_t n16, uint32_t salt)
         p[i] = make_uint4((uint32_t)i ^ salt, salt, (uint32_t)(i >> 32), ~salt);
 }
 
+// y[0 .. n) = 0 with a kernel (also inside captured graphs, where memset
+// nodes are avoided).
+__global__ void k_zero_f64(double *__restrict__ y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = 0.0;
+}
+
+extern "C" int vnec_hip_zero_f64(double *y, int64_t n, void *stream)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, y, n);
+    return (int)hipGetLastError();
+}
+
 extern "C" int vnec_hip_scribble(void *p, size_t n, void *stream)
 {
     static uint32_t salt = 0x9E3779B9u;

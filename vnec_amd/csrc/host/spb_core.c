@@ -18,6 +18,8 @@ static __thread char g_err[512];
 
 const char *vnec_last_error(void) { return g_err; }
 
+void vnec_clear_error(void) { g_err[0] = '\0'; }
+
 SpB_Info vnec_fail(const char *fmt, ...)
 {
     va_list ap;

@@ -90,6 +90,11 @@ struct vnec_plan_s {
     uint16_t *slcol;
     int32_t max_nec;
     int win;
+    /* variant D: groups of `lanes` window positions */
+    int32_t *vrp;       /* nrows + 1 */
+    int32_t *col_start; /* groups    */
+    double *dvals;      /* groups * lanes, wave64 layout */
+    int64_t groups;
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;
@@ -105,6 +110,9 @@ struct vnec_plan_s {
 /* ---- library runtime (spb_core.c) ---- */
 hipStream_t vnec_stream(void);
 SpB_Info vnec_fail(const char *fmt, ...);
+/* Entry points that return void (the reference's SpMV_VNEC_*) clear the
+ * error slot first, so vnec_last_error() afterwards describes that call. */
+void vnec_clear_error(void);
 SpB_Info vnec_hip_ok(int rc, const char *what);
 int vnec_runtime_ready(void);
 void *vnec_malloc_dev(size_t bytes, int64_t *acct);

@@ -35,6 +35,9 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->slo);
     vnec_free_dev(p->scol);
     vnec_free_dev(p->slcol);
+    vnec_free_dev(p->vrp);
+    vnec_free_dev(p->col_start);
+    vnec_free_dev(p->dvals);
     memset(p, 0, sizeof *p);
 }
 
@@ -50,9 +53,11 @@ SpB_Info vnec_plan_free_ext(vnec_plan *plan)
     return SpB_SUCCESS;
 }
 
-/* Per-tile ECR windows for variant E; *max_runs receives the largest run
- * count of any tile. */
-static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *max_runs)
+/* Per-tile ECR windows for variants E and D; *max_runs receives the
+ * largest run count of any tile; ecr32 (nnz, may be NULL) receives the
+ * tile-local ranks as int32. */
+static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *max_runs,
+                              int32_t *ecr32)
 {
     hipStream_t s = vnec_stream();
     int64_t P = p->ntiles, nnz = p->nnz;
@@ -66,7 +71,7 @@ static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *
     p->run_off = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(P + 1), &p->dev_bytes);
     SpB_Info rc = (uniq && nruns && p->lcol && p->nec && p->run_off) ? SpB_SUCCESS : SpB_FAILURE;
     if (rc == SpB_SUCCESS)
-        rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass1(a->d_col, p->coords, P, p->lcol, NULL, uniq,
+        rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass1(a->d_col, p->coords, P, p->lcol, ecr32, uniq,
                                                   p->nec, nruns, s), "ecr pass 1");
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(hipMemsetAsync(nruns + P, 0, sizeof(int32_t), s), "memset");
@@ -210,6 +215,55 @@ static SpB_Info build_slices(struct vnec_plan_s *p, const spb_csr *a)
     return rc;
 }
 
+/* Variant D: windows, then the D layout of every tile (greedy groups of
+ * `lanes` window positions per row segment), re-laid for wave64. */
+static SpB_Info build_d(struct vnec_plan_s *p, const spb_csr *a)
+{
+    hipStream_t s = vnec_stream();
+    int64_t nr = p->nrows, nnz = p->nnz, P = p->ntiles;
+    int L = p->tune.lanes;
+    int32_t max_runs = 0;
+    int32_t *ecr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
+    int32_t *cnt = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nr + 2), NULL);
+    int32_t *unsorted = cnt ? cnt + nr + 1 : NULL, h_unsorted = 0;
+    p->vrp = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nr + 1), &p->dev_bytes);
+    SpB_Info rc = (ecr && cnt && p->vrp) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS) rc = build_windows(p, a, &max_runs, ecr);
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemsetAsync(unsorted, 0, sizeof(int32_t), s), "memset");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_d_count(a->d_ptr, ecr, nr, p->coords, P, L, cnt, unsorted, s), "d count");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(&h_unsorted, unsorted, sizeof(int32_t), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    if (rc == SpB_SUCCESS && h_unsorted)
+        rc = vnec_fail("variant D needs every row's columns in ascending order (as SpB_Matrix_build_FP64 leaves them)");
+    int64_t G = 0;
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_scan_excl_i32(cnt, p->vrp, nr + 1, &G, s), "d scan");
+    if (rc == SpB_SUCCESS && G * L >= INT32_MAX) rc = vnec_fail("variant D: %lld groups too many", (long long)G);
+    double *va = NULL;
+    int64_t nblk = (G + 63) / 64;
+    if (rc == SpB_SUCCESS) {
+        p->groups = G;
+        p->col_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(G ? G : 1), &p->dev_bytes);
+        p->dvals = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nblk ? nblk * 64 * L : 1), &p->dev_bytes);
+        va = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(G ? G * L : 1), NULL);
+        if (!p->col_start || !p->dvals || !va) rc = SpB_FAILURE;
+    }
+    if (rc == SpB_SUCCESS && G)
+        rc = vnec_hip_ok(hipMemsetAsync(va, 0, sizeof(double) * (size_t)(G * L), s), "memset");
+    if (rc == SpB_SUCCESS && nblk)
+        rc = vnec_hip_ok(hipMemsetAsync(p->dvals, 0, sizeof(double) * (size_t)(nblk * 64 * L), s), "memset");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_d_fill(a->d_ptr, ecr, a->d_val, nr, p->coords, P, L, p->vrp,
+                                         p->col_start, va, 1, s), "d fill");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_d_interleave(va, G, L, p->dvals, s), "d interleave");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    vnec_free_dev(va);
+    vnec_free_dev(ecr);
+    vnec_free_dev(cnt);
+    return rc;
+}
+
 SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tuning *t)
 {
     if (!plan) return vnec_fail("vnec_plan_build_ext: NULL plan handle");
@@ -234,10 +288,16 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;
-    if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_E && tune.variant != VNEC_VAR_S)
+    if (tune.variant < 0 || tune.variant >= VNEC_VAR_N)
         return vnec_fail("variant %d is not available", tune.variant);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
-    if (tune.variant == VNEC_VAR_S) {
+    if (tune.variant == VNEC_VAR_D) {
+        if (!vnec_hip_d_supported(tune.ipt, tune.lanes))
+            return vnec_fail("variant D: ipt in {5, 7, 9} and lanes in {4, 8, 16}");
+        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant D stores FP64 values only");
+        if (tune.nrhs != 1) return vnec_fail("variant D takes one right-hand side");
+        tune.grid = -1;
+    } else if (tune.variant == VNEC_VAR_S) {
         if (tune.ipt < 1 || tune.ipt > 7) return vnec_fail("variant S: ipt (tile = 2048 ipt items) in [1, 7]");
         if (tune.nrhs != 1) return vnec_fail("variant S takes one right-hand side");
     } else {
@@ -270,7 +330,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         rc = p->nrows > 0 ? build_slices(p, a) : SpB_SUCCESS;
     } else if (p->ntiles > 0) {
         p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
-        if (tune.nrhs == 1 && tune.grid >= 0) {
+        if (tune.nrhs == 1 && tune.grid >= 0 && tune.variant != VNEC_VAR_D) {
             int nwg = tune.grid;
             if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding, &nwg),
                                         "occupancy")) rc = SpB_FAILURE;
@@ -290,9 +350,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (rc == SpB_SUCCESS)
             rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
                              "partition");
+        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_D) rc = build_d(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) {
             int32_t max_runs = 0;
-            rc = build_windows(p, a, &max_runs);
+            rc = build_windows(p, a, &max_runs, NULL);
             if (rc == SpB_SUCCESS && p->nwg > 0 && max_runs > VNEC_E_STREAM_MAX_RUNS) {
                 /* too many runs in a tile for the pipelined kernel */
                 vnec_free_dev(p->wg_row);
@@ -389,9 +450,35 @@ static SpB_Info launch_slice(struct vnec_plan_s *p, const double *x_dev, double 
 }
 
 /* Launch the SpMV kernel only (its carries stay in the plan's scratch). */
+static SpB_Info launch_d(struct vnec_plan_s *p, const double *x_dev, double *y_dev, hipStream_t s)
+{
+    vnec_d_args a;
+    memset(&a, 0, sizeof a);
+    a.ptr = p->ptr;
+    a.vrp = p->vrp;
+    a.col_start = p->col_start;
+    a.vals = p->dvals;
+    a.val = p->val;
+    a.lcol = p->lcol;
+    a.coords = p->coords;
+    a.ntiles = p->ntiles;
+    a.run_off = p->run_off;
+    a.run_start = p->run_start;
+    a.run_wpos = p->run_wpos;
+    a.nec = p->nec;
+    a.x = x_dev;
+    a.y = y_dev;
+    a.carry = p->carry;
+    a.nrows = p->nrows;
+    a.ipt = p->tune.ipt;
+    a.lanes = p->tune.lanes;
+    return vnec_hip_ok(vnec_hip_spmv_d(&a, s), "d kernel");
+}
+
 static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hipStream_t s)
 {
     if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, m->x, m->y, s);
+    if (p->tune.variant == VNEC_VAR_D) return launch_d(p, m->x, m->y, s);
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_spmv_stream(m, p->nwg, p->wg_row, p->wg_carry, s), "spmv kernel");
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
@@ -412,10 +499,7 @@ SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev
 {
     hipStream_t s = vnec_stream();
     if (p->ntiles == 0) {
-        if (p->nrows > 0)
-            return vnec_hip_ok(hipMemsetAsync(y_dev, 0, sizeof(double) * (size_t)(p->nrows * p->tune.nrhs), s),
-                               "zero y");
-        return SpB_SUCCESS;
+        return vnec_hip_ok(vnec_hip_zero_f64(y_dev, p->nrows * p->tune.nrhs, s), "zero y");
     }
     vnec_merge_args m;
     fill_args(p, x_dev, y_dev, &m);

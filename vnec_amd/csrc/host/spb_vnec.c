@@ -244,7 +244,7 @@ static SpB_Info d_layout_device(spb_csr *a, int64_t P, int L, int want_vals, dla
         rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, nr, nnz, P, 0, d->coords, s), "partition");
     if (rc == SpB_SUCCESS) rc = ecr_device(a, P, d->coords, 0, d->ecr, d->win, d->win_off);
     if (rc == SpB_SUCCESS)
-        rc = vnec_hip_ok(vnec_hip_d_count(a->d_ptr, d->ecr, nr, d->coords, P, L, d->vrp, s), "d count");
+        rc = vnec_hip_ok(vnec_hip_d_count(a->d_ptr, d->ecr, nr, d->coords, P, L, d->vrp, NULL, s), "d count");
     int64_t G = 0;
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_scan_excl_i32(d->vrp, d->vrp, nr + 1, &G, s), "d scan");
     if (rc == SpB_SUCCESS) {
@@ -259,7 +259,7 @@ static SpB_Info d_layout_device(spb_csr *a, int64_t P, int L, int want_vals, dla
     }
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(vnec_hip_d_fill(a->d_ptr, d->ecr, a->d_val, nr, d->coords, P, L, d->vrp,
-                                         d->col_start, d->val_align, s), "d fill");
+                                         d->col_start, d->val_align, 0, s), "d fill");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
     if (rc != SpB_SUCCESS) dlay_free(d);
     return rc;
@@ -316,10 +316,35 @@ SpB_VNEC_type IRD_VNEC(const SpB_Matrix A, SpB_Type type)
 }
 
 /* ------------------------------------------------------ public plans */
-static struct vnec_plan_s *gpu_plan_for(spb_csr *a)
+/* The device plan behind a Build_VNEC_* result runs the matching GPU
+ * kernel: VNEC-L the merge-path CSR kernel (7 items per thread,
+ * persistent), VNEC-D the wave64 D-block kernel (groups of 8 window
+ * positions, the reference's nLanes), VNEC-S the 64-row slice kernel when
+ * the slice layout applies (padding <= 3) and the learned selector's
+ * choice otherwise. */
+static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
 {
     vnec_plan p = NULL;
-    if (vnec_plan_build_ext(&p, (SpB_Matrix)a, NULL) != SpB_SUCCESS) return NULL;
+    vnec_tuning t = {VNEC_SELECT_LEARNED, 7, 64, VNEC_ENC_FP64, 1, -1};
+    if (kind == KIND_L) {
+        t.variant = VNEC_VAR_L;
+        t.grid = 0;
+    } else if (kind == KIND_D) {
+        t.variant = VNEC_VAR_D;
+        t.lanes = REF_L;
+    } else {
+        uint32_t mask = 0;
+        if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return NULL;
+        vnec_tuning c[VNEC_NCAND];
+        vnec_candidates_ext(c, VNEC_NCAND);
+        for (int i = 0; i < VNEC_NCAND; i++)
+            if (c[i].variant == VNEC_VAR_S && ((mask >> i) & 1u)) {
+                t.variant = VNEC_VAR_S;
+                t.ipt = 3;
+                break;
+            }
+    }
+    if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &t) != SpB_SUCCESS) return NULL;
     return p;
 }
 
@@ -361,7 +386,7 @@ struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
     }
     for (int64_t j = 0; j < nnz; j++) l->nz_indices[j] = (int)j;
     free(co);
-    struct vnec_plan_s *g = gpu_plan_for(a);
+    struct vnec_plan_s *g = gpu_plan_for(a, KIND_L);
     if (!g) {
         VNEC_L_free_ext(l);
         return NULL;
@@ -427,7 +452,7 @@ struct VNEC_D_FP64 *Build_VNEC_D_FP64(const SpB_Matrix A)
     free(off);
     free(win);
     dlay_free(&d);
-    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a) : NULL;
+    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a, KIND_D) : NULL;
     if (!g) {
         if (p) VNEC_D_free_fp64_ext(p);
         return NULL;
@@ -471,7 +496,7 @@ struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
     free(nec);
     free(off);
     free(win);
-    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a) : NULL;
+    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a, KIND_S) : NULL;
     if (!g) {
         if (p) VNEC_S_free_fp64_ext(p);
         return NULL;
@@ -484,6 +509,7 @@ struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
 static SpB_Info run_wrapped(const void *pub, plan_kind kind, SpB_Vector y, const SpB_Matrix A,
                             const SpB_Vector x)
 {
+    vnec_clear_error();
     wrap *w = find_wrap(pub);
     if (!w || w->kind != kind) return vnec_fail("plan was not made by this library's Build_VNEC_*");
     if ((const spb_csr *)A != w->A) return vnec_fail("plan was built for another matrix");
