@@ -150,6 +150,15 @@ SpB_Info vnec_ecr_ext(const SpB_Matrix A, int64_t P, const int32_t *coords, int 
 SpB_Info vnec_d_layout_ext(const SpB_Matrix A, int64_t P, int L, int32_t *v_row_ptr,
                            int32_t *col_start, double *val_align, int64_t *groups);
 
+/* VNEC-S task order and reordered matrix at (P, L, cross-row cut) over the
+ * snapped partitions (reference: P = 36, L = 8, cut 32): tasks (nrows,
+ * concatenated per partition), spvv_len (P), row_begin / row_end (nrows),
+ * s_val (nnz, may be NULL), s_col (nnz, ECR ranks), task_start / task_end
+ * (P).  Replaces spmv_vnec_S_for_mxv.c:442-585. */
+SpB_Info vnec_s_order_ext(const SpB_Matrix A, int64_t P, int L, int cut, int32_t *tasks,
+                          int32_t *spvv_len, int32_t *row_begin, int32_t *row_end, double *s_val,
+                          int32_t *s_col, int32_t *task_start, int32_t *task_end);
+
 /* IRD ratio (float)nnz / (float)(G * L) and the D / S decision at (P, L,
  * threshold); IRD_VNEC uses P = 36, L = 8, 0.477 (ird_calc.c:2-123). */
 SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *ird, int *decision);

@@ -176,6 +176,22 @@ int vnec_hip_d_interleave(const double *in, int64_t G, int L, double *out, void 
 int vnec_hip_spmv_d(const vnec_d_args *a, void *stream);
 int vnec_hip_d_supported(int ipt, int lanes);
 
+/* ---- VNEC-S task order and reordered matrix (builder) ---- */
+/* Stable sort of n (key, value) pairs by the low key_bits bits of the keys
+ * (LSD radix, 8-bit digits); allocates its own scratch. */
+int vnec_hip_sort_pairs_u64(uint64_t *keys, int32_t *vals, int64_t n, int key_bits, void *stream);
+/* Task list (nrows) and cross-row counts (P) of the snapped partitions in
+ * coords (2 (P + 1) ints): per partition, rows by length then row id, the
+ * first c - c % L rows of each length <= cut first. */
+int vnec_hip_s_order(const int32_t *ptr, int64_t nrows, const int32_t *coords, int64_t P, int L,
+                     int cut, int32_t *tasks, int32_t *spvv_len, void *stream);
+/* Rows copied in task order (row_begin / row_end: nrows each; s_val may be
+ * NULL; s_col gets the ECR ranks ecr[]), cross-row groups as [l][row]. */
+int vnec_hip_s_reorder(const int32_t *ptr, const double *val, const int32_t *ecr,
+                       const int32_t *tasks, int64_t nrows, int64_t nnz, const int32_t *coords,
+                       const int32_t *spvv_len, int64_t P, int L, int32_t *row_begin,
+                       int32_t *row_end, double *s_val, int32_t *s_col, void *stream);
+
 /* y[0 .. n) = 0 (a kernel, safe inside captured graphs). */
 int vnec_hip_zero_f64(double *y, int64_t n, void *stream);
 

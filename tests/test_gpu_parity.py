@@ -223,6 +223,38 @@ def test_d_layout_and_ird_bit_exact_with_reference(name):
     assert spb.ird_vnec(A) == int(d["ird"])
 
 
+@pytest.mark.parametrize("name", golden_names())
+def test_s_order_bit_exact_with_reference(name):
+    """GPU S task order and reordered matrix at the reference's P = 36,
+    L = 8, cut 32 against the reference build's own structures (fixtures
+    whose reference S build is well defined carry them)."""
+    d = load_golden(name)
+    if "s_tasks" not in d:
+        pytest.skip("no reference S structures in this fixture")
+    n = int(d["nrows"])
+    A = spb.Matrix.from_csr(n, int(d["ncols"]), d["ptr"], d["col"], d["val"])
+    r = spb.s_order(A, 36, 8, 32)
+    assert np.array_equal(r["tasks"], d["s_tasks"])
+    assert np.array_equal(r["spvv_len"], d["s_spvv_len"])
+    assert np.array_equal(r["row_begin"], d["s_row_begin"])
+    assert np.array_equal(r["row_end"], d["s_row_end"])
+    assert np.array_equal(r["s_col"], d["s_s_col"])
+    assert np.array_equal(r["s_val"], d["s_s_val"])
+    assert np.array_equal(np.c_[r["task_start"], r["task_end"]], d["s_task_se"])
+
+
+@pytest.mark.parametrize("P,L_,cut", [(1, 8, 32), (7, 4, 8), (36, 8, 32), (300, 16, 64), (2000, 2, 3)])
+def test_s_order_bit_exact_with_oracle(P, L_, cut):
+    m = g.powerlaw(12000, 1.9, 1, 400, seed=P + L_)
+    A = _mat(m)
+    r = spb.s_order(A, P, L_, cut)
+    o = O.s_build(m.ptr, m.col, m.val, P, L_, cut)
+    assert np.array_equal(r["tasks"], o.tasks) and np.array_equal(r["spvv_len"], o.spvv_len)
+    assert np.array_equal(r["row_begin"], o.row_begin) and np.array_equal(r["row_end"], o.row_end)
+    assert np.array_equal(r["s_col"], o.s_col) and np.array_equal(r["s_val"], o.s_val)
+    assert np.array_equal(r["task_start"], o.task_start) and np.array_equal(r["task_end"], o.task_end)
+
+
 @pytest.mark.parametrize("P,L_", [(1, 1), (7, 4), (64, 64), (300, 8)])
 def test_d_layout_bit_exact_with_oracle(P, L_):
     m = g.banded(4000, 9, seed=P + L_)
@@ -264,6 +296,22 @@ def test_reference_api_build_and_spmv(name):
                 assert np.array_equal(np.ctypeslib.as_array(p.col_start, (G,)), d["col_start"])
             nec = np.ctypeslib.as_array(p.NEC_NUM, (36,))
             assert np.array_equal(nec, d["nec"])
+        if kind == "S" and "s_tasks" in d:
+            nz = int(d["ptr"][-1])
+            assert np.array_equal(np.ctypeslib.as_array(p.NEC_NUM, (36,)), d["s_nec"])
+            assert np.array_equal(np.ctypeslib.as_array(p.spvv8_len, (36,)), d["s_spvv_len"])
+            rm = p.reorder_mat
+            ts = np.ctypeslib.as_array(rm.task_start, (36,))
+            te = np.ctypeslib.as_array(rm.task_end, (36,))
+            assert np.array_equal(np.c_[ts, te], d["s_task_se"])
+            tasks = np.concatenate([np.ctypeslib.as_array(p.tasks[t], (int(te[t] - ts[t]),))
+                                    for t in range(36) if te[t] > ts[t]] or [np.zeros(0, np.int32)])
+            assert np.array_equal(tasks, d["s_tasks"])
+            assert np.array_equal(np.ctypeslib.as_array(rm.row_begin, (n,)), d["s_row_begin"])
+            assert np.array_equal(np.ctypeslib.as_array(rm.row_end, (n,)), d["s_row_end"])
+            if nz:
+                assert np.array_equal(np.ctypeslib.as_array(rm.col, (nz,)), d["s_s_col"])
+                assert np.array_equal(np.ctypeslib.as_array(rm.nnz, (nz,)), d["s_s_val"])
         spb.free_vnec(kind, plan)
     # SpB_mxv with no plan builds and caches one
     y = spb.Vector(n)

@@ -114,6 +114,7 @@ SIGNATURES = {
     "vnec_partition_ext": (Info, [vp, i64, C.c_int, vp]),
     "vnec_ecr_ext": (Info, [vp, i64, vp, C.c_int, vp, vp, vp, vp]),
     "vnec_d_layout_ext": (Info, [vp, i64, C.c_int, vp, vp, vp, C.POINTER(i64)]),
+    "vnec_s_order_ext": (Info, [vp, i64, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vnec_ird_ext": (Info, [vp, i64, C.c_int, C.c_double, C.POINTER(C.c_float),
                             C.POINTER(C.c_int)]),
     "vnec_candidates_ext": (C.c_int, [vp, C.c_int]),
@@ -157,6 +158,10 @@ SIGNATURES = {
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
     "vnec_hip_zero_f64": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_sort_pairs_u64": (C.c_int, [vp, vp, i64, C.c_int, vp]),
+    "vnec_hip_s_order": (C.c_int, [vp, i64, vp, i64, C.c_int, C.c_int, vp, vp, vp]),
+    "vnec_hip_s_reorder": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, i64, C.c_int, vp, vp, vp,
+                                     vp, vp]),
     "vnec_hip_d_interleave": (C.c_int, [vp, i64, C.c_int, vp, vp]),
     "vnec_hip_spmv_d": (C.c_int, [vp, vp]),
     "vnec_hip_d_supported": (C.c_int, [C.c_int, C.c_int]),

@@ -7,9 +7,8 @@
  * partitions, L = 8 lanes, src/operation/include/SpB_mxv.h:26-33), computed
  * by the GPU builders, and keeps behind it a GPU execution plan laid out
  * for wave64 tiles.  SpMV_VNEC_* runs that GPU plan; y is valid on return.
- * The heavy FP64 payload fields (val_align, reorder_mat.nnz) stay NULL:
- * the device plan holds the values, and the reference's val_align would
- * take 64 bytes per nonzero of host memory.
+ * The D plan's val_align stays NULL: the device plan holds the values, and
+ * the reference's val_align would take 64 bytes per nonzero of host memory.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -19,6 +18,7 @@
 #define REF_P 36     /* NUM_THREADS, SpB_mxv.h:28 */
 #define REF_L 8      /* nLanes_f64,  SpB_mxv.h:30 */
 #define REF_IRD_THR 0.477 /* IRD_thr_fp64, SpB_mxv.h:32 */
+#define REF_CROSS_MAX 32  /* longest row sent to the cross-row kernel, spmv_vnec_S_for_mxv.c:489 */
 
 typedef enum { KIND_L = 1, KIND_D = 2, KIND_S = 3 } plan_kind;
 
@@ -203,6 +203,74 @@ SpB_Info vnec_ecr_ext(const SpB_Matrix A, int64_t P, const int32_t *coords, int 
     vnec_free_dev(decr);
     vnec_free_dev(dwin);
     vnec_free_dev(doff);
+    return rc;
+}
+
+/* VNEC-S task order and reordered matrix on the device at (P, L, cut), over
+ * the snapped partitions (spmv_vnec_S_for_mxv.c:336-585), copied to the host
+ * buffers given (any of s_val, task_start, task_end may be NULL). */
+SpB_Info vnec_s_order_ext(const SpB_Matrix A, int64_t P, int L, int cut, int32_t *tasks,
+                          int32_t *spvv_len, int32_t *row_begin, int32_t *row_end, double *s_val,
+                          int32_t *s_col, int32_t *task_start, int32_t *task_end)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a || P < 1 || L < 1 || !tasks || !spvv_len || !row_begin || !row_end || !s_col)
+        return vnec_fail("vnec_s_order_ext: bad arguments");
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
+    size_t nrb = sizeof(int32_t) * (size_t)(nr ? nr : 1), nzb = (size_t)(nnz ? nnz : 1);
+    int32_t *dco = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(P + 1), NULL);
+    int32_t *decr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * nzb, NULL);
+    int32_t *dwin = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * nzb, NULL);
+    int64_t *doff = (int64_t *)vnec_malloc_dev(sizeof(int64_t) * (size_t)(P + 1), NULL);
+    int32_t *dtasks = (int32_t *)vnec_malloc_dev(nrb, NULL);
+    int32_t *dspvv = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)P, NULL);
+    int32_t *drb = (int32_t *)vnec_malloc_dev(nrb, NULL);
+    int32_t *dre = (int32_t *)vnec_malloc_dev(nrb, NULL);
+    int32_t *dcol = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * nzb, NULL);
+    double *dval = s_val ? (double *)vnec_malloc_dev(sizeof(double) * nzb, NULL) : NULL;
+    int32_t *hco = (int32_t *)malloc(sizeof(int32_t) * 2 * (size_t)(P + 1));
+    SpB_Info rc = (dco && decr && dwin && doff && dtasks && dspvv && drb && dre && dcol && hco &&
+                   (!s_val || dval)) ? SpB_SUCCESS : vnec_fail("out of memory");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, nr, nnz, P, 1, dco, s), "partition");
+    if (rc == SpB_SUCCESS) rc = ecr_device(a, P, dco, 0, decr, dwin, doff);
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_s_order(a->d_ptr, nr, dco, P, L, cut, dtasks, dspvv, s), "s order");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_s_reorder(a->d_ptr, a->d_val, decr, dtasks, nr, nnz, dco, dspvv, P, L, drb, dre,
+                                            dval, dcol, s), "s reorder");
+    if (rc == SpB_SUCCESS && nr) {
+        rc = vnec_hip_ok(hipMemcpyAsync(tasks, dtasks, nrb, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemcpyAsync(row_begin, drb, nrb, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemcpyAsync(row_end, dre, nrb, hipMemcpyDeviceToHost, s), "d2h");
+    }
+    if (rc == SpB_SUCCESS && nnz) {
+        rc = vnec_hip_ok(hipMemcpyAsync(s_col, dcol, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+        if (rc == SpB_SUCCESS && s_val)
+            rc = vnec_hip_ok(hipMemcpyAsync(s_val, dval, sizeof(double) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(spvv_len, dspvv, sizeof(int32_t) * (size_t)P, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(hipMemcpyAsync(hco, dco, sizeof(int32_t) * 2 * (size_t)(P + 1), hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    /* tasks of partition t occupy [start_t.x, end_t.x) (snapped rows are contiguous) */
+    for (int64_t t = 0; rc == SpB_SUCCESS && t < P; t++) {
+        if (task_start) task_start[t] = hco[2 * t];
+        if (task_end) task_end[t] = hco[2 * t + 2];
+    }
+    vnec_free_dev(dco);
+    vnec_free_dev(decr);
+    vnec_free_dev(dwin);
+    vnec_free_dev(doff);
+    vnec_free_dev(dtasks);
+    vnec_free_dev(dspvv);
+    vnec_free_dev(drb);
+    vnec_free_dev(dre);
+    vnec_free_dev(dcol);
+    vnec_free_dev(dval);
+    free(hco);
     return rc;
 }
 
@@ -491,6 +559,36 @@ struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
             if (p->use_x_indices[t]) memcpy(p->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
         }
     }
+    /* task order and reordered matrix (spmv_vnec_S_for_mxv.c:442-585) */
+    int64_t nr = (int64_t)a->base.row;
+    int32_t *tasks = NULL;
+    if (rc == SpB_SUCCESS) {
+        size_t rb = sizeof(int) * (size_t)(nr ? nr : 1), zb = (size_t)(nnz ? nnz : 1);
+        tasks = (int32_t *)malloc(rb);
+        p->spvv8_len = (int *)calloc(REF_P + 1, sizeof(int));
+        p->tasks = (int **)calloc(REF_P + 1, sizeof(int *));
+        p->reorder_mat.nnz = (double *)malloc(sizeof(double) * zb);
+        p->reorder_mat.col = (int *)malloc(sizeof(int) * zb);
+        p->reorder_mat.row_begin = (int *)malloc(rb);
+        p->reorder_mat.row_end = (int *)malloc(rb);
+        p->reorder_mat.task_start = (int *)calloc(REF_P + 1, sizeof(int));
+        p->reorder_mat.task_end = (int *)calloc(REF_P + 1, sizeof(int));
+        if (!tasks || !p->spvv8_len || !p->tasks || !p->reorder_mat.nnz || !p->reorder_mat.col ||
+            !p->reorder_mat.row_begin || !p->reorder_mat.row_end || !p->reorder_mat.task_start ||
+            !p->reorder_mat.task_end)
+            rc = vnec_fail("Build_VNEC_S_FP64: out of memory");
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_s_order_ext(A, REF_P, REF_L, REF_CROSS_MAX, tasks, p->spvv8_len, p->reorder_mat.row_begin,
+                              p->reorder_mat.row_end, p->reorder_mat.nnz, p->reorder_mat.col,
+                              p->reorder_mat.task_start, p->reorder_mat.task_end);
+    for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
+        int n = p->reorder_mat.task_end[t] - p->reorder_mat.task_start[t];
+        p->tasks[t] = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        if (!p->tasks[t]) rc = vnec_fail("Build_VNEC_S_FP64: out of memory");
+        else if (n > 0) memcpy(p->tasks[t], tasks + p->reorder_mat.task_start[t], sizeof(int) * (size_t)n);
+    }
+    free(tasks);
     free(co);
     free(ecr);
     free(nec);

@@ -317,6 +317,23 @@ def ecr(A: Matrix, coords: np.ndarray, method: int = 0):
     return e[:nnz], nec, off, win[: int(off[-1])]
 
 
+def s_order(A: Matrix, P: int, L_: int, cut: int = 32) -> dict:
+    """VNEC-S task order and reordered matrix at (P, L, cut) (vnec_s_order_ext)."""
+    n, nz = A.nrows, A.nnz
+    out = {"tasks": np.zeros(max(n, 1), np.int32), "spvv_len": np.zeros(P, np.int32),
+           "row_begin": np.zeros(max(n, 1), np.int32), "row_end": np.zeros(max(n, 1), np.int32),
+           "s_val": np.zeros(max(nz, 1), np.float64), "s_col": np.zeros(max(nz, 1), np.int32),
+           "task_start": np.zeros(P, np.int32), "task_end": np.zeros(P, np.int32)}
+    L.check(L.load().vnec_s_order_ext(A.h, P, L_, cut, *(_p(out[k]) for k in
+                                      ("tasks", "spvv_len", "row_begin", "row_end", "s_val",
+                                       "s_col", "task_start", "task_end"))), "vnec_s_order_ext")
+    for k in ("tasks", "row_begin", "row_end"):
+        out[k] = out[k][:n]
+    for k in ("s_val", "s_col"):
+        out[k] = out[k][:nz]
+    return out
+
+
 def d_layout(A: Matrix, P: int, L_: int, with_values: bool = True):
     G = C.c_int64()
     L.check(L.load().vnec_d_layout_ext(A.h, P, L_, None, None, None, C.byref(G)),
