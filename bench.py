@@ -212,7 +212,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="stencil",
                     choices=["stencil", "rmat18", "skew", "batch", "batch8"])
-    ap.add_argument("--variant", default=None, help="force a variant (S, L, E); default selector")
+    ap.add_argument("--variant", default=None, help="force a variant (S, L, E, D); default selector")
+    ap.add_argument("--select", default=None, choices=["learned", "autotune", "rule", "reference"],
+                    help="selection mode when no variant is forced (default: autotune for the "
+                         "batched workloads, whose tile shapes come from the autotuner; learned "
+                         "otherwise)")
     ap.add_argument("--ipt", type=int, default=0)
     ap.add_argument("--grid", type=int, default=0, help="with --variant: 0 persistent, -1 one WG per tile")
     ap.add_argument("--no-cpu", action="store_true")
@@ -242,12 +246,14 @@ def main():
     dy = spb.DeviceArray(8 * m.nrows * k)
 
     log("building plan ...")
+    mode = None
     t0 = time.perf_counter()
     if args.variant:
         plan = spb.Plan(A, args.variant, args.ipt or (3 if args.variant == "S" else 7), nrhs=k,
                         grid=args.grid)
     else:
-        plan = spb.Plan(A, variant=None, nrhs=k)
+        mode = args.select or ("autotune" if args.workload.startswith("batch") else "learned")
+        plan = spb.Plan(A, mode, nrhs=k)
     build_ms = (time.perf_counter() - t0) * 1e3
     info = plan.info()
     log(f"plan: {info}")
@@ -304,7 +310,8 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             tj = json.load(f)
-        if tj.get("variant") == info.variant:
+        # the counters were read for one plan shape: use them only for that shape
+        if (tj.get("variant"), tj.get("ipt"), tj.get("grid")) == (info.variant, info.ipt, info.grid):
             traffic = tj.get("hbm_bytes_per_launch")
 
     sel = None
@@ -328,7 +335,7 @@ def main():
         "config": {"workload": desc, "nrows": m.nrows, "nnz": m.nnz, "nrhs": k,
                    "algorithmic_bytes": B, "variant": info.variant, "ipt": info.ipt,
                    "grid": info.grid, "tiles": info.ntiles,
-                   "selection": args.variant or "learned", "parallelism": "single GPU"},
+                   "selection": args.variant or mode, "parallelism": "single GPU"},
         "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "gflops": round(2.0 * m.nnz * k / (ms_step / 1e3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -209,3 +209,27 @@ def test_learned_mode_follows_compiled_model_on_held_out():
         got = spb.select(A, "learned")
         assert got == ch[0] or abs(o[0, got] - o[0, ch[0]]) < 1e-4, (i, got, ch[0])
         A.free()
+
+
+@pytest.mark.parametrize("mode", ["autotune", "learned"])
+def test_multi_rhs_selection(mode):
+    """Several right-hand sides run the merge-path kernel; the autotuner
+    picks its granularity among the compiled ones."""
+    k = 4
+    m = g.rmat(12, 8, seed=45)
+    A = _mat(m)
+    p = spb.Plan(A, mode, nrhs=k)
+    info = p.info()
+    assert info.variant == "L" and info.nrhs == k and info.ipt in (3, 5, 7)
+    X = np.random.default_rng(k).uniform(-1, 1, m.ncols * k)
+    xv = spb.Vector.from_array(X)
+    yv = spb.Vector(m.nrows * k)
+    p.spmv(yv, xv)
+    Y = yv.to_array()
+    Y0 = O.spmm_serial(m.ptr, m.col, m.val, X, k)
+    ax = np.empty(m.nrows * k)
+    for c in range(k):
+        ax[c::k] = O.spmv_abs(m.ptr, m.col, m.val, X[c::k])[0]
+    assert O.check_rows(Y, Y0, ax)[0]
+    p.free()
+    A.free()

@@ -129,9 +129,11 @@ SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev
 double vnec_slice_padding(const spb_csr *a);
 
 /* ---- autotuner and selector (spb_tune.c) ---- */
-/* Tuning chosen by a selection mode (vnec_select_mode) for one FP64
- * right-hand side. */
-SpB_Info vnec_select_tuning(spb_csr *a, int mode, vnec_tuning *t);
+/* Tuning chosen by a selection mode (vnec_select_mode) for FP64 values
+ * and nrhs right-hand sides (several: the merge-path kernel, its
+ * granularity autotuned in VNEC_SELECT_AUTOTUNE mode, else 5 items per
+ * thread). */
+SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t);
 
 /* ---- compiled-in selector model (spb_selector_model.c, generated) ---- */
 extern const int vnec_model_trained;

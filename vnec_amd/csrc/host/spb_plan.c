@@ -275,16 +275,14 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (t && t->variant >= 0) {
         tune = *t;
     } else {
-        if (vnec_select_tuning(a, t ? t->variant : VNEC_SELECT_LEARNED, &tune) != SpB_SUCCESS)
+        int nrhs = t && t->nrhs > 0 ? t->nrhs : 1;
+        if (vnec_select_tuning(a, t ? t->variant : VNEC_SELECT_LEARNED, nrhs, &tune) != SpB_SUCCESS)
             return SpB_FAILURE;
-        if (t) {
-            tune.encoding = t->encoding;
-            if (t->nrhs > 0) tune.nrhs = t->nrhs;
-        }
-        if (tune.nrhs > 1) { /* several right-hand sides: the merge-path kernel */
-            tune.variant = VNEC_VAR_L;
-            tune.ipt = 5;
-            tune.grid = -1;
+        if (t) tune.encoding = t->encoding;
+        if (tune.variant != VNEC_VAR_S &&
+            !vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs)) {
+            tune.variant = VNEC_VAR_L; /* the encodings every merge kernel has */
+            tune.ipt = 7;
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;

@@ -580,8 +580,54 @@ SpB_Info vnec_select_ext(const SpB_Matrix A, int mode, int *candidate)
     }
 }
 
-SpB_Info vnec_select_tuning(spb_csr *a, int mode, vnec_tuning *t)
+/* Several right-hand sides run on the merge-path kernel with one workgroup
+ * per tile; the autotuner picks its granularity among the compiled ones. */
+static const int MULTI_IPT[] = {3, 5, 7};
+
+static SpB_Info autotune_multi(spb_csr *a, int nrhs, int *ipt)
 {
+    int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
+    size_t xb = sizeof(double) * (size_t)(nc * nrhs > 0 ? nc * nrhs : 1);
+    double *hx = (double *)malloc(xb);
+    double *dx = (double *)vnec_malloc_dev(xb, NULL);
+    double *dy = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nr * nrhs > 0 ? nr * nrhs : 1), NULL);
+    SpB_Info rc = (hx && dx && dy) ? SpB_SUCCESS : vnec_fail("out of memory");
+    for (int64_t i = 0; rc == SpB_SUCCESS && i < nc * nrhs; i++) hx[i] = 1.0 / (double)(1 + (i % 7));
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemcpy(dx, hx, xb, hipMemcpyHostToDevice), "h2d");
+    double best = INFINITY;
+    *ipt = 5;
+    for (size_t c = 0; rc == SpB_SUCCESS && c < sizeof MULTI_IPT / sizeof MULTI_IPT[0]; c++) {
+        vnec_tuning t = {VNEC_VAR_L, MULTI_IPT[c], 64, VNEC_ENC_FP64, nrhs, -1};
+        vnec_plan p = NULL;
+        if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &t) != SpB_SUCCESS) continue;
+        double tb = INFINITY;
+        for (int r = 0; rc == SpB_SUCCESS && r < 3; r++) {
+            double ms = 0.0;
+            rc = vnec_time_dev_ext(p, dx, dy, 10, 1, 0, &ms);
+            if (rc == SpB_SUCCESS && ms < tb) tb = ms;
+        }
+        vnec_plan_free_ext(&p);
+        if (tb < best) {
+            best = tb;
+            *ipt = MULTI_IPT[c];
+        }
+    }
+    free(hx);
+    vnec_free_dev(dx);
+    vnec_free_dev(dy);
+    return rc;
+}
+
+SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t)
+{
+    if (nrhs > 1) {
+        int ipt = 5;
+        if (mode == VNEC_SELECT_AUTOTUNE && autotune_multi(a, nrhs, &ipt) != SpB_SUCCESS) return SpB_FAILURE;
+        vnec_tuning m = {VNEC_VAR_L, ipt, 64, VNEC_ENC_FP64, nrhs, -1};
+        *t = m;
+        return SpB_SUCCESS;
+    }
     int c = 0;
     if (vnec_select_ext((SpB_Matrix)a, mode, &c) != SpB_SUCCESS) return SpB_FAILURE;
     *t = CANDIDATES[c];
