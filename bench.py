@@ -271,6 +271,17 @@ def main():
     value = B / (ms_step / 1e3) / 1e9
     y_gpu = dy.to_array(np.float64, m.nrows * k)
 
+    # host vectors (the reference API's semantics: x uploaded, y downloaded per call)
+    xh, yh = spb.Vector.from_array(m.x), spb.Vector(m.nrows * k)
+    plan.spmv(yh, xh)
+    reps_h = 10
+    t0 = time.perf_counter()
+    for _ in range(reps_h):
+        plan.spmv(yh, xh)
+    host_ms = (time.perf_counter() - t0) / reps_h * 1e3
+    xh.free()
+    yh.free()
+
     k_ms, fix_ms = plan.time_kernels(dx.ptr, dy.ptr, max(args.steps // 2, 20))
     graph_ms = plan.time(dx.ptr, dy.ptr, max(args.steps, 20), graph=True)
     achieved = B / (k_ms / 1e3) / 1e9
@@ -344,6 +355,8 @@ def main():
                      "kernel": KERNEL_NAME.get((info.variant, info.grid >= 0), "?"),
                      "kernel_us": round(k_ms * 1e3, 2), "fixup_us": round(fix_ms * 1e3, 2)},
         "graph_spmv_us": round(graph_ms * 1e3, 2),
+        "host_vectors_spmv_us": round(host_ms * 1e3, 1),
+        "host_vectors_GBps": round(B / (host_ms / 1e3) / 1e9, 1),
         "copy_peak_gbs": round(cp.value, 1),
         "pct_of_copy_peak": round(100.0 * achieved / cp.value, 2),
         "plan_build_ms": round(build_ms, 1),
