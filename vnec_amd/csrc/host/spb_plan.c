@@ -640,7 +640,8 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[2], s), "record");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(ev[2]), "sync");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");
-        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&b, ev[1], ev[2]), "elapsed");
+        if (rc == SpB_SUCCESS && plan->tune.variant != VNEC_VAR_S) /* S has no fix-up kernel */
+            rc = vnec_hip_ok(hipEventElapsedTime(&b, ev[1], ev[2]), "elapsed");
         t_main += a;
         t_fix += b;
     }
