@@ -56,6 +56,8 @@ typedef enum {
     VNEC_VAR_E = 1, /* merge-path CSR, ECR x window staged in LDS, u16 cols */
     VNEC_VAR_S = 2, /* row-aligned tiles, 64-row equal-length slices        */
     VNEC_VAR_D = 3, /* dense column blocks of width `lanes`                 */
+    VNEC_VAR_H = 4, /* persistent merge-path CSR with the `lanes` (4096 or
+                       8192) most used x entries held in LDS                */
     VNEC_VAR_N
 } vnec_variant;
 
@@ -81,7 +83,7 @@ typedef enum {
 typedef struct {
     int variant;  /* vnec_variant, or a vnec_select_mode                    */
     int ipt;      /* merge items per thread (tile = 256 * ipt items)       */
-    int lanes;    /* D block width / S slice height                        */
+    int lanes;    /* D block width / H hot x entries in LDS                */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV (1, or 2..16)               */
     int grid;     /* 0: persistent, one workgroup per resident slot;

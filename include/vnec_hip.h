@@ -90,6 +90,8 @@ typedef struct {
     int variant;
     int encoding;
     int nrhs;
+    const double *xhot;       /* nhot packed hot x entries (variant H) */
+    int nhot;
 } vnec_merge_args;
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
@@ -111,6 +113,27 @@ int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *wg_row, dou
                          void *stream);
 int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, double *y,
                       int64_t nrows, void *stream);
+/* Variant H: the persistent kernel with the hot x entries in LDS (hot =
+ * 4096 or 8192 slots at 5 or 7 items per thread; FP64, one right-hand
+ * side).  col holds -1 - slot for hot columns.  Workgroups of
+ * 1024 threads run four tile pipelines over one LDS table: nwg workgroups
+ * own 4 nwg runs of tiles, and wg_row / wg_carry hold 4 nwg entries. */
+int vnec_hip_hot_supported(int ipt, int hot);
+int vnec_hip_hot_grid(int ipt, int hot, int *nwg);
+int vnec_hip_spmv_hot(const vnec_merge_args *a, int hot, int nwg, int32_t *wg_row, double *wg_carry,
+                      void *stream);
+/* Hot-column set of a matrix: deg (ncols, caller-zeroed) gets each column's
+ * use count; vnec_hip_hot_order sorts columns by descending count (ties by
+ * column) into order (ncols); vnec_hip_hot_encode writes slot[] (ncols, -1
+ * for cold columns) from order[0 .. nhot) and the encoded columns hcol
+ * (nnz). */
+int vnec_hip_col_degree(const int32_t *col, int64_t nnz, int32_t *deg, void *stream);
+int vnec_hip_hot_order(const int32_t *deg, int64_t ncols, int32_t *order, void *stream);
+int vnec_hip_hot_encode(const int32_t *col, int64_t nnz, const int32_t *order, int64_t nhot,
+                        int64_t ncols, int32_t *slot, int32_t *hcol, void *stream);
+/* xhot[h] = x[order[h]], h < nhot (before each variant-H SpMV). */
+int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, double *xhot,
+                        void *stream);
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 

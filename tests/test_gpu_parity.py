@@ -33,7 +33,7 @@ def _mat(m):
 
 def _run_plan(A, x, variant, ipt, enc="fp64", nrhs=1, grid=0, lanes=None):
     if lanes is None:
-        lanes = 8 if variant == "D" else 64
+        lanes = {"D": 8, "H": 4096}.get(variant, 64)
     p = spb.Plan(A, variant, ipt, lanes=lanes, encoding=enc, nrhs=nrhs, grid=grid)
     xv = spb.Vector.from_array(x)
     yv = spb.Vector(A.nrows * nrhs)
@@ -116,6 +116,20 @@ def test_d_variant_matches_oracle(name, m, ipt, lanes):
     assert ok, worst
 
 
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("ipt,hot", [(5, 4096), (7, 8192), (5, 8192)])
+def test_h_variant_matches_oracle(name, m, ipt, hot):
+    """Variant H (persistent merge path, most used x entries in LDS); the
+    R-MAT and dense-row cases have hot columns, the others none."""
+    A = _mat(m)
+    print(f"case {name} variant H ipt {ipt} hot {hot}", flush=True)
+    y = _run_plan(A, m.x, "H", ipt, lanes=hot)
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y0, ax, TOL)
+    assert ok, worst
+
+
 @pytest.mark.parametrize("variant", ["L", "E", "S"])
 @pytest.mark.parametrize("enc", ["hilo", "hi"])
 def test_encodings_error_bounds(variant, enc):
@@ -151,7 +165,7 @@ def test_known_answer_is_exact():
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
     rows = np.repeat(np.arange(m.nrows), np.diff(m.ptr))
     expect = np.bincount(rows, weights=rows * m.col.astype(np.float64), minlength=m.nrows)
-    for variant, ipt in (("L", 7), ("E", 7), ("S", 3), ("D", 7)):
+    for variant, ipt in (("L", 7), ("E", 7), ("S", 3), ("D", 7), ("H", 5)):
         assert np.array_equal(_run_plan(A, x, variant, ipt), expect)
 
 
@@ -164,9 +178,11 @@ def test_empty_rows_and_empty_matrix():
     for variant in ("L", "E", "S"):
         assert np.allclose(_run_plan(A, m.x, variant, 3), y0, rtol=0, atol=1e-15)
     assert np.allclose(_run_plan(A, m.x, "D", 5), y0, rtol=0, atol=1e-15)
+    assert np.allclose(_run_plan(A, m.x, "H", 5), y0, rtol=0, atol=1e-15)
     A = spb.Matrix.from_csr(5, 4, [0, 0, 0, 0, 0, 0], [], [])
     assert np.array_equal(_run_plan(A, np.ones(4), "S", 1), np.zeros(5))
     assert np.array_equal(_run_plan(A, np.ones(4), "D", 7), np.zeros(5))
+    assert np.array_equal(_run_plan(A, np.ones(4), "H", 7), np.zeros(5))
 
 
 # -------------------------------------------------- builder structures
@@ -352,7 +368,7 @@ def test_graph_timing_path_gives_same_result():
     assert O.check_rows(y, O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
 
 
-@pytest.mark.parametrize("variant,ipt", [("L", 7), ("E", 7), ("S", 3), ("D", 7)])
+@pytest.mark.parametrize("variant,ipt", [("L", 7), ("E", 7), ("S", 3), ("D", 7), ("H", 7)])
 def test_full_size_stencil_per_row_gate(variant, ipt):
     m = g.stencil27(128, seed=1)
     A = _mat(m)
@@ -376,6 +392,6 @@ def test_repeated_columns_are_summed_by_every_variant():
     A = _mat(m)
     y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
     ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
-    for variant, ipt in (("L", 7), ("E", 5), ("S", 3), ("D", 5), ("D", 9)):
+    for variant, ipt in (("L", 7), ("E", 5), ("S", 3), ("D", 5), ("D", 9), ("H", 7)):
         ok, worst = O.check_rows(_run_plan(A, m.x, variant, ipt), y0, ax, TOL)
         assert ok, (variant, worst)

@@ -158,6 +158,13 @@ SIGNATURES = {
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
     "vnec_hip_zero_f64": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_hot_supported": (C.c_int, [C.c_int, C.c_int]),
+    "vnec_hip_hot_grid": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "vnec_hip_spmv_hot": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
+    "vnec_hip_col_degree": (C.c_int, [vp, i64, vp, vp]),
+    "vnec_hip_hot_order": (C.c_int, [vp, i64, vp, vp]),
+    "vnec_hip_hot_encode": (C.c_int, [vp, i64, vp, i64, i64, vp, vp, vp]),
+    "vnec_hip_gather_hot": (C.c_int, [vp, vp, i64, vp, vp]),
     "vnec_hip_sort_pairs_u64": (C.c_int, [vp, vp, i64, C.c_int, vp]),
     "vnec_hip_s_order": (C.c_int, [vp, i64, vp, i64, C.c_int, C.c_int, vp, vp, vp]),
     "vnec_hip_s_reorder": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, i64, C.c_int, vp, vp, vp,

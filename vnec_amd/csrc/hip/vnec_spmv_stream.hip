@@ -33,11 +33,11 @@ struct TileRegs {
     int re[IPT];
 };
 
+// tid: the thread's index within its 256-thread tile pipeline.
 template <int IPT, bool WIN, int ENC>
 __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int2 c1,
-                                          TileRegs<IPT, WIN> &R)
+                                          TileRegs<IPT, WIN> &R, int tid)
 {
-    const int tid = threadIdx.x;
     const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
 #pragma unroll
     for (int u = 0; u < IPT; u++) {
@@ -52,7 +52,18 @@ __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int
     }
 }
 
-template <int IPT, bool WIN, int ENC>
+// HOT > 0 (variant H): the a.nhot <= HOT most used columns of x, packed
+// into a.xhot by k_gather_hot before the launch, are held in LDS for the
+// whole launch; a.col then stores -1 - slot for those columns, so their
+// gathers never leave the compute unit.
+template <int HOT>
+__device__ __forceinline__ double gather_x(const vnec_merge_args &a, const double *s_hot, int c)
+{
+    if (HOT > 0 && c < 0) return s_hot[-1 - c];
+    return a.x[c];
+}
+
+template <int IPT, bool WIN, int ENC, int HOT = 0>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, int nwg,
                                                             int32_t *__restrict__ wg_row,
                                                             double *__restrict__ wg_carry)
@@ -61,6 +72,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
     __shared__ double s_prod[T];
     __shared__ int s_rowend[T];
     __shared__ double s_win[WIN ? T : 1];
+    __shared__ double s_hot[HOT > 0 ? HOT : 1];
     __shared__ int s_f[VNEC_NWAVES + 1];
     __shared__ double s_v[VNEC_NWAVES + 1];
     __shared__ double s_carry;
@@ -101,7 +113,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         }
     }
     TileRegs<IPT, WIN> cur, nxt;
-    load_tile<IPT, WIN, ENC>(a, A0, A1, cur);
+    load_tile<IPT, WIN, ENC>(a, A0, A1, cur, tid);
+    if (HOT > 0)
+        for (int i = tid; i < a.nhot; i += VNEC_BLOCK) s_hot[i] = a.xhot[i];
     __syncthreads();
 
     for (int64_t t = tb; t < te; t++) {
@@ -115,7 +129,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                 nec_c = a.nec[t + 2];
             }
         }
-        if (t + 1 < te) load_tile<IPT, WIN, ENC>(a, A1, B1, nxt);
+        if (t + 1 < te) load_tile<IPT, WIN, ENC>(a, A1, B1, nxt, tid);
 
         const int x0 = A0.x, y0 = A0.y;
         const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
@@ -131,7 +145,8 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         for (int u = 0; u < IPT; u++) {
             const int k = u * VNEC_BLOCK + tid;
             if (k < nrow) s_rowend[k] = cur.re[u];
-            if (k < nz) s_prod[k] = cur.v[u] * (WIN ? s_win[cur.c[u]] : a.x[cur.c[u]]);
+            if (k < nz)
+                s_prod[k] = cur.v[u] * (WIN ? s_win[cur.c[u]] : gather_x<HOT>(a, s_hot, cur.c[u]));
         }
         __syncthreads();
 
@@ -207,6 +222,148 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
     if (tid == 0) {
         wg_row[rank] = A0.x;  // end row of the last tile
         wg_carry[rank] = s_carry;
+    }
+}
+
+// ----------------------------------------------------------- variant H
+// Four independent tile pipelines of 256 threads share one workgroup of
+// 1024 threads and one LDS copy of the hot x entries, so the hot table
+// costs no occupancy: per compute unit the same 16 wavefronts run as with
+// the plain persistent kernel.  Pipeline g of workgroup b owns the run of
+// tiles of rank 4 b' + g (b' the XCD-ordered workgroup rank); runs differ
+// by at most one tile, and a pipeline whose run is over keeps meeting the
+// workgroup barriers with empty tiles.
+#define HOT_SUB 4
+
+template <int IPT, int HOT>
+__global__ __launch_bounds__(VNEC_BLOCK * HOT_SUB) void k_spmv_hot4(vnec_merge_args a, int nwg,
+                                                                    int32_t *__restrict__ wg_row,
+                                                                    double *__restrict__ wg_carry)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_hot[HOT];
+    __shared__ double s_prod[HOT_SUB][T];
+    __shared__ int s_rowend[HOT_SUB][T];
+    __shared__ int s_f[HOT_SUB][VNEC_NWAVES + 1];
+    __shared__ double s_v[HOT_SUB][VNEC_NWAVES + 1];
+    __shared__ double s_carry[HOT_SUB];
+
+    const int g = threadIdx.x / VNEC_BLOCK;
+    const int tid = threadIdx.x % VNEC_BLOCK;
+    const int w = tid / VNEC_WAVE;
+    const int64_t nr = (int64_t)nwg * HOT_SUB;
+    const int64_t base = xcd_tile(blockIdx.x, nwg) * HOT_SUB;
+    const int64_t rank = base + g;
+    const int64_t tb = rank * a.ntiles / nr, te = (rank + 1) * a.ntiles / nr;
+    int64_t nmax = 0;
+    for (int k = 0; k < HOT_SUB; k++) {
+        const int64_t n = (base + k + 1) * a.ntiles / nr - (base + k) * a.ntiles / nr;
+        nmax = n > nmax ? n : nmax;
+    }
+    const int2 *co = (const int2 *)a.coords;
+
+    if (tid == 0) s_carry[g] = 0.0;
+    for (int i = threadIdx.x; i < a.nhot; i += VNEC_BLOCK * HOT_SUB) s_hot[i] = a.xhot[i];
+
+    int2 A0 = make_int2(0, 0), A1 = A0, B1 = A0;
+    TileRegs<IPT, false> cur, nxt;
+    if (tb < te) {
+        A0 = co[tb];
+        A1 = co[tb + 1];
+        B1 = A1;
+        if (tb + 1 < te) B1 = co[tb + 2];
+        load_tile<IPT, false, 0>(a, A0, A1, cur, tid);
+    }
+    __syncthreads();
+
+    for (int64_t step = 0; step < nmax; step++) {
+        const int64_t t = tb + step;
+        const bool act = t < te;
+        int2 B2 = B1;
+        if (act && t + 2 < te) B2 = co[t + 3];
+        if (act && t + 1 < te) load_tile<IPT, false, 0>(a, A1, B1, nxt, tid);
+
+        const int x0 = A0.x, y0 = A0.y;
+        const int nrow = act ? A1.x - A0.x : 0, nz = act ? A1.y - A0.y : 0;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int k = u * VNEC_BLOCK + tid;
+            if (k < nrow) s_rowend[g][k] = cur.re[u];
+            if (k < nz) {
+                const int c = cur.c[u];
+                s_prod[g][k] = cur.v[u] * (c < 0 ? s_hot[-1 - c] : a.x[c]);
+            }
+        }
+        __syncthreads();
+
+        const int items = nrow + nz;
+        const int d0 = tid * IPT < items ? tid * IPT : items;
+        const int d1 = d0 + IPT < items ? d0 + IPT : items;
+        int rx = (int)merge_search(s_rowend[g], nrow, y0, nz, d0);
+        int ny = d0 - rx;
+        double acc = 0.0, head_val = 0.0;
+        int flag = 0, head_row = 0;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            if (d0 + u < d1) {
+                if (rx < nrow && s_rowend[g][rx] <= y0 + ny) {
+                    if (!flag) {
+                        head_row = x0 + rx;
+                        head_val = acc;
+                        flag = 1;
+                    } else {
+                        a.y[x0 + rx] = acc;
+                    }
+                    acc = 0.0;
+                    rx++;
+                } else {
+                    acc += s_prod[g][ny];
+                    ny++;
+                }
+            }
+        }
+
+        int f = flag;
+        double v = acc;
+        wave_seg_scan_f64(f, v);
+        if (lane_id() == VNEC_WAVE - 1) {
+            s_f[g][w] = f;
+            s_v[g][w] = v;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double c = s_carry[g];
+            for (int k = 0; k < VNEC_NWAVES; k++) {
+                const double tw = s_v[g][k];
+                const int fw = s_f[g][k];
+                s_v[g][k] = c;
+                c = fw ? tw : c + tw;
+            }
+            s_v[g][VNEC_NWAVES] = c;
+        }
+        __syncthreads();
+        const double cw = s_v[g][w];
+        const double blk = f ? v : v + cw;
+        const double cin = wave_shift_up1_f64(blk, cw);
+        if (flag) a.y[head_row] = head_val + cin;
+        if (tid == 0 && act) s_carry[g] = s_v[g][VNEC_NWAVES];
+        __syncthreads();
+
+        if (act) {
+            cur = nxt;
+            A0 = A1;
+            A1 = B1;
+            B1 = B2;
+        }
+    }
+    if (tid == 0) {
+        if (tb < te) {
+            wg_row[rank] = A0.x;
+            wg_carry[rank] = s_carry[g];
+        } else {
+            wg_row[rank] = co[tb].x;
+            wg_carry[rank] = 0.0;
+        }
     }
 }
 
@@ -472,6 +629,49 @@ extern "C" int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *
     if (a->ntiles <= 0) return 0;
     hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, nwg,
                        wg_row, wg_carry);
+    return (int)hipGetLastError();
+}
+
+// ----------------------------------------------------------- variant H
+static stream_kernel_t pick_hot(int ipt, int hot)
+{
+#define HCASE(I, H) \
+    if (ipt == I && hot == H) return k_spmv_hot4<I, H>;
+    HCASE(5, 4096) HCASE(7, 4096) HCASE(5, 8192) HCASE(7, 8192)
+#undef HCASE
+    return nullptr;
+}
+
+extern "C" int vnec_hip_hot_supported(int ipt, int hot) { return pick_hot(ipt, hot) != nullptr; }
+
+extern "C" int vnec_hip_hot_grid(int ipt, int hot, int *nwg)
+{
+    stream_kernel_t k = pick_hot(ipt, hot);
+    if (!k) return (int)hipErrorInvalidValue;
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k,
+                                                                VNEC_BLOCK * HOT_SUB, 0);
+    if (e != hipSuccess) return (int)e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return (int)e;
+    int n = per_cu * cus;
+    n = n / VNEC_XCDS * VNEC_XCDS;
+    *nwg = n > 0 ? n : VNEC_XCDS;
+    return 0;
+}
+
+extern "C" int vnec_hip_spmv_hot(const vnec_merge_args *a, int hot, int nwg, int32_t *wg_row,
+                                 double *wg_carry, void *stream)
+{
+    stream_kernel_t k = pick_hot(a->ipt, hot);
+    if (!k || a->nrhs != 1 || a->encoding != 0 || a->nhot > hot || nwg < VNEC_XCDS ||
+        nwg % VNEC_XCDS)
+        return (int)hipErrorInvalidValue;
+    if (a->ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK * HOT_SUB), 0, (hipStream_t)stream, *a,
+                       nwg, wg_row, wg_carry);
     return (int)hipGetLastError();
 }
 

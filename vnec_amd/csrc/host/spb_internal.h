@@ -95,6 +95,11 @@ struct vnec_plan_s {
     int32_t *col_start; /* groups    */
     double *dvals;      /* groups * lanes, wave64 layout */
     int64_t groups;
+    /* variant H: hot x entries */
+    int32_t *hcol;      /* nnz: column, or -1 - slot for a hot column */
+    int32_t *hot_order; /* ncols: columns by descending use; first nhot are hot */
+    double *xhot;       /* nhot: packed before each SpMV */
+    int nhot;
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;

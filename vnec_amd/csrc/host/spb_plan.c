@@ -38,6 +38,9 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->vrp);
     vnec_free_dev(p->col_start);
     vnec_free_dev(p->dvals);
+    vnec_free_dev(p->hcol);
+    vnec_free_dev(p->hot_order);
+    vnec_free_dev(p->xhot);
     memset(p, 0, sizeof *p);
 }
 
@@ -264,6 +267,49 @@ static SpB_Info build_d(struct vnec_plan_s *p, const spb_csr *a)
     return rc;
 }
 
+/* Variant H: the hot-column set.  A column earns an LDS slot when it is
+ * used more often than the cost of loading the slot into every persistent
+ * workgroup (workgroups / 16 requests: 16 entries per 128-byte line). */
+static SpB_Info build_hot(struct vnec_plan_s *p, const spb_csr *a)
+{
+    hipStream_t s = vnec_stream();
+    int64_t nc = p->ncols, nnz = p->nnz, H = p->tune.lanes;
+    if (H > nc) H = nc;
+    int32_t *deg = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), NULL);
+    int32_t *slot = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), NULL);
+    int32_t *hdeg = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nc ? nc : 1));
+    int32_t *hord = (int32_t *)malloc(sizeof(int32_t) * (size_t)(H ? H : 1));
+    p->hot_order = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), &p->dev_bytes);
+    p->hcol = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), &p->dev_bytes);
+    SpB_Info rc = (deg && slot && hdeg && hord && p->hot_order && p->hcol) ? SpB_SUCCESS
+                                                                           : vnec_fail("out of memory");
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemsetAsync(deg, 0, sizeof(int32_t) * (size_t)nc, s), "memset");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_col_degree(a->d_col, nnz, deg, s), "column uses");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_hot_order(deg, nc, p->hot_order, s), "hot order");
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemcpyAsync(hdeg, deg, sizeof(int32_t) * (size_t)nc, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && H)
+        rc = vnec_hip_ok(hipMemcpyAsync(hord, p->hot_order, sizeof(int32_t) * (size_t)H, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    int64_t wgs = p->tune.grid, nhot = 0, thr = wgs / 16 > 1 ? wgs / 16 : 1;
+    while (rc == SpB_SUCCESS && nhot < H && hdeg[hord[nhot]] > thr) nhot++;
+    p->nhot = (int)nhot;
+    if (rc == SpB_SUCCESS) {
+        p->xhot = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nhot ? nhot : 1), &p->dev_bytes);
+        if (!p->xhot) rc = SpB_FAILURE;
+    }
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_hot_encode(a->d_col, nnz, p->hot_order, nhot, nc, slot, p->hcol, s),
+                         "hot encode");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    vnec_free_dev(deg);
+    vnec_free_dev(slot);
+    free(hdeg);
+    free(hord);
+    return rc;
+}
+
 SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tuning *t)
 {
     if (!plan) return vnec_fail("vnec_plan_build_ext: NULL plan handle");
@@ -289,7 +335,13 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (tune.variant < 0 || tune.variant >= VNEC_VAR_N)
         return vnec_fail("variant %d is not available", tune.variant);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
-    if (tune.variant == VNEC_VAR_D) {
+    if (tune.variant == VNEC_VAR_H) {
+        if (!vnec_hip_hot_supported(tune.ipt, tune.lanes))
+            return vnec_fail("variant H: ipt 5 or 7 and lanes (hot entries) 4096 or 8192");
+        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant H stores FP64 values only");
+        if (tune.nrhs != 1) return vnec_fail("variant H takes one right-hand side");
+        if (tune.grid < 0) tune.grid = 0; /* persistent only: the hot set loads once per workgroup */
+    } else if (tune.variant == VNEC_VAR_D) {
         if (!vnec_hip_d_supported(tune.ipt, tune.lanes))
             return vnec_fail("variant D: ipt in {5, 7, 9} and lanes in {4, 8, 16}");
         if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant D stores FP64 values only");
@@ -330,12 +382,17 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
         if (tune.nrhs == 1 && tune.grid >= 0 && tune.variant != VNEC_VAR_D) {
             int nwg = tune.grid;
-            if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding, &nwg),
-                                        "occupancy")) rc = SpB_FAILURE;
+            if (nwg == 0 && tune.variant == VNEC_VAR_H) {
+                if (vnec_hip_ok(vnec_hip_hot_grid(tune.ipt, tune.lanes, &nwg), "occupancy")) rc = SpB_FAILURE;
+            } else if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding, &nwg),
+                                               "occupancy")) {
+                rc = SpB_FAILURE;
+            }
             nwg = nwg / 8 * 8;
             if (nwg < 8) nwg = 8;
-            p->nwg = nwg;
             p->tune.grid = nwg;
+            if (tune.variant == VNEC_VAR_H) nwg *= 4; /* four tile pipelines per workgroup */
+            p->nwg = nwg;
             p->wg_row = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)nwg, &p->dev_bytes);
             p->wg_carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)nwg, &p->dev_bytes);
             if (!p->wg_row || !p->wg_carry) rc = SpB_FAILURE;
@@ -349,6 +406,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
                              "partition");
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_D) rc = build_d(p, a);
+        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_H) rc = build_hot(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) {
             int32_t max_runs = 0;
             rc = build_windows(p, a, &max_runs, NULL);
@@ -417,6 +475,11 @@ static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *
     m.variant = p->tune.variant;
     m.encoding = p->tune.encoding;
     m.nrhs = p->tune.nrhs;
+    if (p->tune.variant == VNEC_VAR_H) {
+        m.col = p->hcol;
+        m.xhot = p->xhot;
+        m.nhot = p->nhot;
+    }
     *out = m;
 }
 
@@ -477,6 +540,12 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hip
 {
     if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, m->x, m->y, s);
     if (p->tune.variant == VNEC_VAR_D) return launch_d(p, m->x, m->y, s);
+    if (p->tune.variant == VNEC_VAR_H) {
+        if (vnec_hip_ok(vnec_hip_gather_hot(m->x, p->hot_order, p->nhot, p->xhot, s), "hot gather"))
+            return SpB_FAILURE;
+        return vnec_hip_ok(vnec_hip_spmv_hot(m, p->tune.lanes, p->tune.grid, p->wg_row, p->wg_carry, s),
+                           "spmv kernel");
+    }
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_spmv_stream(m, p->nwg, p->wg_row, p->wg_carry, s), "spmv kernel");
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");

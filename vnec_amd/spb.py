@@ -17,7 +17,7 @@ from . import _lib as L
 SpB_FP64 = 11
 SpB_MATRIX_FORMAT = 1
 SpB_CSR = 1
-VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3}
+VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4}
 VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2}
 # Selection modes (negative vnec_tuning.variant values).
 SELECT = {"learned": L.SELECT_LEARNED, "autotune": L.SELECT_AUTOTUNE, "rule": L.SELECT_RULE,
