@@ -111,3 +111,31 @@ def test_plan_without_device_fails_loudly():
     A = spb.Matrix.from_csr(2, 2, [0, 1, 2], [0, 1], [1.0, 2.0])
     with pytest.raises(RuntimeError, match="no HIP device"):
         spb.Plan(A)
+
+
+@pytest.mark.parametrize("ptr,col,msg", [
+    ([0, 1, 2], [0, 5], "out of range"),
+    ([0, 2, 1, 3], [0, 1, 2], "monotone"),
+])
+def test_invalid_structure_is_refused_at_import(ptr, col, msg):
+    n = len(ptr) - 1
+    with pytest.raises(RuntimeError, match=msg):
+        spb.Matrix.from_csr(n, 3, np.array(ptr, np.int64), np.array(col, np.int32),
+                            np.ones(len(col)))
+
+
+@pytest.mark.parametrize("which,msg", [("col", "not below ncols"), ("ptr", "decrease")])
+def test_invalid_structure_is_refused_before_any_kernel(which, msg):
+    """Callers that fill the public CSR fields themselves (as the reference
+    harness downcasts handles) bypass import: the device mirror re-checks
+    every column and row pointer on the host before any kernel reads them."""
+    A = spb.Matrix.from_csr(3, 3, np.array([0, 1, 2, 3], np.int64), np.array([0, 1, 2], np.int32),
+                            np.ones(3))
+    s = _view(A.h, RefMatrixLayout)
+    if which == "col":
+        s.indices[1] = 7
+    else:
+        s.ptr[1] = 2
+        s.ptr[2] = 1
+    with pytest.raises(RuntimeError, match=msg):
+        spb.partition(A, 2)

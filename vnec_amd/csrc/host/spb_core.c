@@ -361,7 +361,6 @@ SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_of
 SpB_Info spb_matrix_upload(spb_csr *a)
 {
     if (a->dev_ready) return SpB_SUCCESS;
-    if (!vnec_runtime_ready()) return SpB_FAILURE;
     int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
     if ((int64_t)a->ptr[nr] != nnz) return vnec_fail("matrix row pointers do not end at nnz");
     if (nnz + nr >= INT32_MAX) return vnec_fail("nnz + nrows must stay below 2^31");
@@ -371,8 +370,36 @@ SpB_Info spb_matrix_upload(spb_csr *a)
         free(hp); free(hc);
         return vnec_fail("device mirror: out of host memory");
     }
-    for (int64_t r = 0; r <= nr; r++) hp[r] = (int32_t)a->ptr[r];
-    for (int64_t j = 0; j < nnz; j++) hc[j] = (int32_t)a->indices[j];
+    /* Every kernel indexes x by these columns and walks these row pointers:
+     * refuse a structure that would make them read out of bounds. */
+    int64_t nc = (int64_t)a->base.col;
+    if (a->ptr[0] != 0) {
+        free(hp);
+        free(hc);
+        return vnec_fail("matrix row pointers must start at 0");
+    }
+    for (int64_t r = 0; r <= nr; r++) {
+        if (r > 0 && a->ptr[r] < a->ptr[r - 1]) {
+            free(hp);
+            free(hc);
+            return vnec_fail("matrix row pointers decrease at row %lld", (long long)r);
+        }
+        hp[r] = (int32_t)a->ptr[r];
+    }
+    for (int64_t j = 0; j < nnz; j++) {
+        if (a->indices[j] >= (SpB_Index)nc) {
+            free(hp);
+            free(hc);
+            return vnec_fail("column index %llu at entry %lld is not below ncols = %lld",
+                             (unsigned long long)a->indices[j], (long long)j, (long long)nc);
+        }
+        hc[j] = (int32_t)a->indices[j];
+    }
+    if (!vnec_runtime_ready()) {
+        free(hp);
+        free(hc);
+        return SpB_FAILURE;
+    }
     a->d_ptr = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nr + 1), NULL);
     a->d_col = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), NULL);
     a->d_val = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nnz ? nnz : 1), NULL);
