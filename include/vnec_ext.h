@@ -38,6 +38,19 @@ SpB_Info SpB_Matrix_nvals_ext(SpB_Index *nvals, const SpB_Matrix A);
 SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_off,
                                const int64_t *col_off, const int64_t *ptr, const int32_t *col,
                                const double *val);
+/* Graph files as the reference harness reads them (utilities/reader.hpp,
+ * builder.hpp): .mtx (1-based coordinates; symmetric files add reversed
+ * edges; sorted, duplicate edges and self-loops dropped) or .sg (serialized
+ * CSR, taken as stored).  ptr (nrows + 1) and col are allocated here; free
+ * them with vnec_host_free_ext. */
+SpB_Info vnec_read_graph_ext(const char *path, int64_t *nrows, int64_t **ptr, int32_t **col);
+void vnec_host_free_ext(void *p);
+/* A square CSR FP64 matrix from such a file, with the values the reference
+ * harness gives the pattern (test_spmv_vnec.cpp:160-170: a_ij = i, its
+ * known-answer test) or all ones.  Selects SpB_CSR as the matrix format. */
+enum { VNEC_VALUES_ROW = 0, VNEC_VALUES_ONE = 1 };
+SpB_Info SpB_Matrix_read_ext(SpB_Matrix *A, const char *path, int values);
+
 /* Bulk copy of a dense FP64 vector's n values in / out. */
 SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values);
 SpB_Info SpB_Vector_export_ext(const SpB_Vector v, double *values);

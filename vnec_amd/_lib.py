@@ -117,6 +117,9 @@ SIGNATURES = {
     "vnec_s_order_ext": (Info, [vp, i64, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]),
     "vnec_ird_ext": (Info, [vp, i64, C.c_int, C.c_double, C.POINTER(C.c_float),
                             C.POINTER(C.c_int)]),
+    "vnec_read_graph_ext": (Info, [C.c_char_p, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp)]),
+    "vnec_host_free_ext": (None, [vp]),
+    "SpB_Matrix_read_ext": (Info, [C.POINTER(vp), C.c_char_p, C.c_int]),
     "vnec_candidates_ext": (C.c_int, [vp, C.c_int]),
     "vnec_candidate_mask_ext": (Info, [vp, C.POINTER(C.c_uint32)]),
     "vnec_autotune_ext": (Info, [vp, C.c_int, vp, C.POINTER(C.c_int)]),

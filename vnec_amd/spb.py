@@ -348,6 +348,39 @@ def d_layout(A: Matrix, P: int, L_: int, with_values: bool = True):
     return vrp, cs[:g], (va[: g * L_] if with_values else None)
 
 
+# ------------------------------------------------ graph files
+def read_graph(path: str) -> tuple[int, np.ndarray, np.ndarray]:
+    """(n, ptr int64, col int32) of a .mtx / .sg file, read as the reference
+    harness's Builder does (vnec_read_graph_ext)."""
+    lib = L.load()
+    n = C.c_int64()
+    pp, pc = C.c_void_p(), C.c_void_p()
+    L.check(lib.vnec_read_graph_ext(path.encode(), C.byref(n), C.byref(pp), C.byref(pc)),
+            "vnec_read_graph_ext")
+    try:
+        ptr = np.ctypeslib.as_array(C.cast(pp, C.POINTER(C.c_int64)), (n.value + 1,)).copy()
+        nz = int(ptr[-1])
+        col = (np.ctypeslib.as_array(C.cast(pc, C.POINTER(C.c_int32)), (nz,)).copy() if nz
+               else np.zeros(0, np.int32))
+    finally:
+        lib.vnec_host_free_ext(pp)
+        lib.vnec_host_free_ext(pc)
+    return n.value, ptr, col
+
+
+def matrix_from_file(path: str, values: str = "row") -> Matrix:
+    """SpB_Matrix_read_ext: a_ij = i (\"row\", the harness's values) or 1 (\"one\")."""
+    init()
+    m = Matrix.__new__(Matrix)
+    m.h = C.c_void_p()
+    L.check(L.load().SpB_Matrix_read_ext(C.byref(m.h), path.encode(),
+                                         {"row": 0, "one": 1}[values]), "SpB_Matrix_read_ext")
+    n, ptr, _ = read_graph(path)
+    m.nrows = m.ncols = n
+    m.nnz = int(ptr[-1])
+    return m
+
+
 # ------------------------------------------------ autotuner and selector
 def candidates() -> list[TuningInfo]:
     """The candidate configurations (vnec_candidates_ext)."""
