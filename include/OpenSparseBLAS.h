@@ -1,11 +1,12 @@
 /*
  * OpenSparseBLAS.h — the drop-in subset of the OpenSparseBLAS C API that the
- * VNEC FP64 SpMV path uses, implemented by libvnec_mi355x.so on one MI355X.
+ * VNEC SpMV path uses (FP64, and its FP32 twin), implemented by
+ * libvnec_mi355x.so on one MI355X.
  *
  * Every declaration keeps the reference's name, signature, enum values and
  * public struct layout; each cites the line of the reference header
  * (include/OpenSparseBLAS.h in the VNEC repository) that it replaces.  Only
- * the path's subset is declared: the object life cycle, FP64 build,
+ * the path's subset is declared: the object life cycle, FP64 / FP32 build,
  * set/extract, SpB_mxv and the VNEC plan API.
  *
  * Behavioural notes (where this library differs on purpose):
@@ -73,9 +74,11 @@ SpB_Info SpB_Vector_new(SpB_Vector *v, SpB_Type type, SpB_Index n);
 SpB_Info SpB_Vector_free(SpB_Vector *v);
 SpB_Info SpB_Vector_nvals(SpB_Index *nvals, SpB_Vector v);
 
-/* include/OpenSparseBLAS.h:549-554 */
+/* include/OpenSparseBLAS.h:538-543, 549-554 */
+extern SpB_Info SpB_Vector_setElement_FP32(SpB_Vector w, float x, SpB_Index i);
 extern SpB_Info SpB_Vector_setElement_FP64(SpB_Vector w, double x, SpB_Index i);
-/* include/OpenSparseBLAS.h:741-746 */
+/* include/OpenSparseBLAS.h:730-735, 741-746 */
+extern SpB_Info SpB_Vector_extractElement_FP32(float *x, const SpB_Vector v, SpB_Index i);
 extern SpB_Info SpB_Vector_extractElement_FP64(double *x, const SpB_Vector v, SpB_Index i);
 
 /* include/OpenSparseBLAS.h:971, 981, 992, 1000 */
@@ -84,6 +87,9 @@ SpB_Info SpB_Matrix_ncols(SpB_Index *n, SpB_Matrix m);
 SpB_Info SpB_Matrix_new(SpB_Matrix *m, SpB_Type type, SpB_Index nrows, SpB_Index ncols);
 SpB_Info SpB_Matrix_free(SpB_Matrix *A);
 
+/* include/OpenSparseBLAS.h:1152-1160 */
+extern SpB_Info SpB_Matrix_build_FP32(SpB_Matrix C, const SpB_Index *I, const SpB_Index *J,
+                                      const float *X, SpB_Index nvals, const SpB_BinaryOp dup);
 /* include/OpenSparseBLAS.h:1166-1174 */
 extern SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index *J,
                                       const double *X, SpB_Index nvals, const SpB_BinaryOp dup);
@@ -109,6 +115,39 @@ typedef struct coord {
 /* include/OpenSparseBLAS.h:1983 */
 SpB_VNEC_type IRD_VNEC(const SpB_Matrix A, SpB_Type type);
 
+/* include/OpenSparseBLAS.h:1966-1982 (FP32 values, x, y and sums) */
+typedef struct VNEC_D_FP32 {
+    coord *thread_coord_start;
+    coord *thread_coord_end;
+    int *ecr_indices;
+    int *NEC_NUM;
+    int **use_x_indices;
+    float **ecr_xx_val;
+    int *v_row_ptr;
+    int *col_start;
+    float *val_align;
+} VNEC_D_FP32;
+void SpMV_VNEC_D_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_D_FP32 *mat_thd);
+struct VNEC_D_FP32 *Build_VNEC_D_FP32(const SpB_Matrix A);
+
+/* include/OpenSparseBLAS.h:1985-2012 */
+struct csr_f32 {
+    float *nnz;
+    int *col, *row_begin, *row_end;
+    int *task_start;
+    int *task_end;
+};
+typedef struct VNEC_S_FP32 {
+    int *NEC_NUM;
+    int **use_x_indices;
+    float **ecr_xx_val;
+    int *spvv16_len;
+    int **tasks;
+    struct csr_f32 reorder_mat;
+} VNEC_S_FP32;
+void SpMV_VNEC_S_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_S_FP32 *mat_thd);
+struct VNEC_S_FP32 *Build_VNEC_S_FP32(const SpB_Matrix A);
+
 /* include/OpenSparseBLAS.h:2014-2023 */
 typedef struct VNEC_L {
     int num_merge_items;
@@ -120,6 +159,8 @@ typedef struct VNEC_L {
     int *nz_indices;
 } VNEC_L;
 struct VNEC_L *Build_VNEC_L(const SpB_Matrix A);
+/* include/OpenSparseBLAS.h:2024 */
+void SpMV_VNEC_L_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_L *mat_thd);
 
 /* include/OpenSparseBLAS.h:2027-2043 */
 typedef struct VNEC_D_FP64 {

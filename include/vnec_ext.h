@@ -29,6 +29,9 @@ int vnec_device_count_ext(void);
  * SpB_Matrix_build_FP64 on the same entries, without the sort. */
 SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
                                    const double *val);
+/* The same for a matrix made with SpB_FP32 (SpB_Matrix_build_FP32 order). */
+SpB_Info SpB_Matrix_import_csr_fp32_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
+                                        const float *val);
 SpB_Info SpB_Matrix_nvals_ext(SpB_Index *nvals, const SpB_Matrix A);
 /* Batch of nmat independent matrices as one block-diagonal CSR matrix
  * (A made by SpB_Matrix_new with row_off[nmat] rows, col_off[nmat]
@@ -45,23 +48,28 @@ SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_of
  * them with vnec_host_free_ext. */
 SpB_Info vnec_read_graph_ext(const char *path, int64_t *nrows, int64_t **ptr, int32_t **col);
 void vnec_host_free_ext(void *p);
-/* A square CSR FP64 matrix from such a file, with the values the reference
- * harness gives the pattern (test_spmv_vnec.cpp:160-170: a_ij = i, its
- * known-answer test) or all ones.  Selects SpB_CSR as the matrix format. */
+/* A square CSR matrix (type SpB_FP64 or SpB_FP32) from such a file, with
+ * the values the reference harness gives the pattern
+ * (test_spmv_vnec.cpp:160-170: a_ij = i, its known-answer test) or all
+ * ones.  Selects SpB_CSR as the matrix format. */
 enum { VNEC_VALUES_ROW = 0, VNEC_VALUES_ONE = 1 };
-SpB_Info SpB_Matrix_read_ext(SpB_Matrix *A, const char *path, int values);
+SpB_Info SpB_Matrix_read_ext(SpB_Matrix *A, const char *path, SpB_Type type, int values);
 
 /* Bulk copy of a dense FP64 vector's n values in / out. */
 SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values);
 SpB_Info SpB_Vector_export_ext(const SpB_Vector v, double *values);
+/* The same for an FP32 vector. */
+SpB_Info SpB_Vector_import_fp32_ext(SpB_Vector v, const float *values);
+SpB_Info SpB_Vector_export_fp32_ext(const SpB_Vector v, float *values);
 /* Device residency: when on, SpMV reads x from / writes y to device memory
  * only; host values are refreshed by SpB_Vector_sync_host_ext or lazily by
  * extractElement / export, and setElement / import mark the device copy
  * stale.  Off (the default) keeps the reference's host semantics. */
 SpB_Info SpB_Vector_device_resident_ext(SpB_Vector v, bool on);
 SpB_Info SpB_Vector_sync_host_ext(SpB_Vector v);
-/* Device address of a vector's values (allocated on first use). */
-double *SpB_Vector_device_ptr_ext(SpB_Vector v);
+/* Device address of a vector's values (allocated on first use; FP64 or
+ * FP32 elements, as the vector was made). */
+void *SpB_Vector_device_ptr_ext(SpB_Vector v);
 
 /* --------------------------------------------------------- GPU plans */
 typedef enum {
@@ -78,6 +86,9 @@ typedef enum {
     VNEC_ENC_FP64 = 0, /* FP64 values                                   */
     VNEC_ENC_HILO = 1, /* FP32 hi + FP32 lo, a = hi + lo, FP64 accumulate */
     VNEC_ENC_HI = 2,   /* FP32 hi only, FP64 accumulate                 */
+    VNEC_ENC_FP32 = 3, /* the FP32 API: FP32 values, x, y and sums (FP32
+                          matrices only; variants L (one workgroup per
+                          tile), S and D, one right-hand side)          */
     VNEC_ENC_N
 } vnec_encoding;
 
@@ -115,19 +126,20 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
 /* y <- A x (x: ncols * nrhs, y: nrows * nrhs, row-major [i][rhs]).
  * Synchronous: y is valid when the call returns. */
 SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x);
-/* Same on caller device buffers, asynchronous on the library stream. */
-SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev);
+/* Same on caller device buffers, asynchronous on the library stream.
+ * Device buffers hold doubles, or floats for VNEC_ENC_FP32 plans. */
+SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev);
 /* Times `iters` back-to-back SpMVs on the library stream with HIP events
  * (optionally replayed from one captured hipGraph); *ms = mean ms per SpMV.
  * If flush_bytes > 0 a buffer of that size is overwritten before each SpMV
  * (outside the timed kernels: *ms then excludes it, using per-SpMV event
  * pairs) so that the matrix cannot stay resident in the on-die caches. */
-SpB_Info vnec_time_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+SpB_Info vnec_time_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                            int use_graph, size_t flush_bytes, double *ms);
 
 /* Mean duration (ms) of the SpMV kernel and of the carry fix-up kernel,
  * each timed between its own HIP events on the library stream. */
-SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                                double *ms_main, double *ms_fixup);
 
 /* Measured streaming-copy bandwidth: bytes read + bytes written per
@@ -175,7 +187,8 @@ SpB_Info vnec_s_order_ext(const SpB_Matrix A, int64_t P, int L, int cut, int32_t
                           int32_t *s_col, int32_t *task_start, int32_t *task_end);
 
 /* IRD ratio (float)nnz / (float)(G * L) and the D / S decision at (P, L,
- * threshold); IRD_VNEC uses P = 36, L = 8, 0.477 (ird_calc.c:2-123). */
+ * threshold); IRD_VNEC uses P = 36 and L = 8, 0.477 for SpB_FP64 or
+ * L = 16, 0.437 for SpB_FP32 (ird_calc.c:2-123, SpB_mxv.h:26-33). */
 SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *ird, int *decision);
 
 /* --------------------------------------------- autotuner and selector
@@ -266,6 +279,8 @@ SpB_Info vnec_select_ext(const SpB_Matrix A, int mode, int *candidate);
  * public host fields and the device plan behind them. */
 SpB_Info VNEC_D_free_fp64_ext(VNEC_D_FP64 *plan);
 SpB_Info VNEC_S_free_fp64_ext(VNEC_S_FP64 *plan);
+SpB_Info VNEC_D_free_fp32_ext(VNEC_D_FP32 *plan);
+SpB_Info VNEC_S_free_fp32_ext(VNEC_S_FP32 *plan);
 SpB_Info VNEC_L_free_ext(VNEC_L *plan);
 /* Device plan behind a Build_VNEC_* result (owned by it). */
 vnec_plan VNEC_plan_of_ext(const void *reference_plan);

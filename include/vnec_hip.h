@@ -92,6 +92,10 @@ typedef struct {
     int nrhs;
     const double *xhot;       /* nhot packed hot x entries (variant H) */
     int nhot;
+    /* encoding FP32 (FP32 values in hi, FP32 x / y / sums) */
+    const float *x32;         /* ncols                             */
+    float *y32;               /* nrows                             */
+    float *carry32;           /* ntiles scratch                    */
 } vnec_merge_args;
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
@@ -99,6 +103,9 @@ int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
  * >= nrows, the carries of empty trailing tiles, are skipped). */
 int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, int64_t ntiles, double *y,
                          int64_t nrows, int nrhs, void *stream);
+/* The same for FP32 carries and y (one right-hand side). */
+int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carry, int64_t ntiles, float *y,
+                             int64_t nrows, void *stream);
 /* Persistent pipelined form (one right-hand side): nwg workgroups (a
  * multiple of 8) each own a contiguous run of tiles; each leaves the
  * partial row at its end in wg_row / wg_carry (nwg entries) for
@@ -158,6 +165,8 @@ typedef struct {
     double *y;
     int64_t nrows;
     int encoding;
+    const float *x32;           /* encoding FP32: x, y in FP32, values shi  */
+    float *y32;
 } vnec_slice_args;
 
 /* slice_len[s] = longest row of slice s, slice_sz[s] = 64 * slice_len[s],
@@ -191,11 +200,16 @@ typedef struct {
     int64_t nrows;
     int ipt;
     int lanes;                /* L: 4, 8 or 16                                 */
+    int fp32;                 /* FP32 arithmetic: the *32 fields below are used */
+    const float *vals32, *val32, *x32;
+    float *y32, *carry32;
 } vnec_d_args;
 
 /* Wave64 layout of a D value array: in[g * L + j] -> out[((g / 64) * L + j)
  * * 64 + g % 64]; out holds ceil(G / 64) * 64 * L doubles. */
 int vnec_hip_d_interleave(const double *in, int64_t G, int L, double *out, void *stream);
+/* Same layout narrowed to FP32 (exact for values that came from FP32). */
+int vnec_hip_d_interleave_f32(const double *in, int64_t G, int L, float *out, void *stream);
 int vnec_hip_spmv_d(const vnec_d_args *a, void *stream);
 int vnec_hip_d_supported(int ipt, int lanes);
 
@@ -217,6 +231,9 @@ int vnec_hip_s_reorder(const int32_t *ptr, const double *val, const int32_t *ecr
 
 /* y[0 .. n) = 0 (a kernel, safe inside captured graphs). */
 int vnec_hip_zero_f64(double *y, int64_t n, void *stream);
+int vnec_hip_zero_f32(float *y, int64_t n, void *stream);
+/* out[i] = (double)in[i], i < n (exact). */
+int vnec_hip_widen_f32(const float *in, int64_t n, double *out, void *stream);
 
 /* Overwrites n bytes (cache flush helper for timing). */
 int vnec_hip_scribble(void *p, size_t n, void *stream);

@@ -48,6 +48,7 @@ def lib():
             "orc_s_build": (None, [vp, vp, vp, i64, i64, i64, i64, vp, vp, vp, vp, vp, vp,
                                    vp, vp, vp, vp]),
             "orc_spmv_serial": (None, [vp, vp, vp, vp, i64, vp]),
+            "orc_spmv_serial_f32": (None, [vp, vp, vp, vp, i64, vp]),
             "orc_spmv_abs": (None, [vp, vp, vp, vp, i64, vp, vp]),
             "orc_spmm_serial": (None, [vp, vp, vp, vp, i64, i64, vp]),
             "orc_spmv_L": (None, [vp, vp, vp, vp, i64, i64, vp, vp, vp, C.c_int]),
@@ -215,6 +216,33 @@ def spmv_serial(ptr, col, val, x) -> np.ndarray:
     return y
 
 
+def spmv_serial_f32(ptr, col, val, x) -> np.ndarray:
+    """test_spmv_vnec.hpp:34-54: FP32 values, x and row sums, in order."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    y = np.zeros(len(ptr) - 1, dtype=np.float32)
+    lib().orc_spmv_serial_f32(_ptr(ptr), _ptr(np.ascontiguousarray(col, dtype=np.int32)),
+                              _ptr(np.ascontiguousarray(val, dtype=np.float32)),
+                              _ptr(np.ascontiguousarray(x, dtype=np.float32)), len(ptr) - 1, _ptr(y))
+    return y
+
+
+def check_rows_f32(y, y_ref, ptr, ax, tol=1e-4) -> tuple[bool, float]:
+    """FP32 gate: |y - y_ref| <= max(tol, 2 (len_i + 1) 2^-24) (|A||x|)_i.
+    Two FP32 summations of row i in different orders each lie within
+    (len_i + 1) 2^-24 (|A||x|)_i of the exact value, so the gate is the
+    reference's 1e-4 (test_spmv_vnec.cpp:132) wherever that bound is below
+    it, and the bound itself on longer rows; returns (ok, worst ratio to
+    (|A||x|)_i)."""
+    y = np.asarray(y, dtype=np.float64)
+    y_ref = np.asarray(y_ref, dtype=np.float64)
+    lens = np.diff(np.asarray(ptr, dtype=np.int64)).astype(np.float64)
+    t = np.maximum(tol, 2.0 * (lens + 1.0) * 2.0 ** -24)
+    err = np.abs(y - y_ref)
+    scale = np.where(ax > 0, ax, 1.0)
+    ok = bool(np.all(err <= t * ax) and np.all(np.isfinite(y)))
+    return ok, float((err / scale).max() if len(err) else 0.0)
+
+
 def spmv_abs(ptr, col, val, x) -> tuple[np.ndarray, np.ndarray]:
     """((|A||x|)_i, long-double row sums)."""
     ptr = np.ascontiguousarray(ptr, dtype=np.int64)
@@ -312,6 +340,19 @@ def ref():
             "ref_spmv_l": (None, [vp, vp, vp, vp, C.c_int]),
             "ref_free_l": (None, [vp]),
             "ref_time_spmv": (C.c_double, [C.c_int, vp, vp, vp, vp, C.c_int]),
+            "ref_lanes_f32": (C.c_int, []),
+            "ref_matrix32": (vp, [i64, i64, vp, vp, vp]),
+            "ref_ird32": (C.c_int, [vp]),
+            "ref_build_d32": (vp, [vp]),
+            "ref_d32_groups": (i64, [vp, vp]),
+            "ref_d32_export": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "ref_spmv_d32": (None, [vp, vp, vp, vp, C.c_int]),
+            "ref_free_d32": (None, [vp]),
+            "ref_build_s32": (vp, [vp]),
+            "ref_s32_export": (None, [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+            "ref_spmv_s32": (None, [vp, vp, vp, vp, C.c_int]),
+            "ref_free_s32": (None, [vp]),
+            "ref_spmv_l32": (None, [vp, vp, vp, vp, C.c_int]),
         }
         for k, (res, args) in sig.items():
             f = getattr(r, k)
@@ -322,15 +363,20 @@ def ref():
 
 
 class RefMatrix:
-    """A reference SpB_Matrix built from a CSR (kept alive with its arrays)."""
+    """A reference SpB_Matrix built from a CSR (kept alive with its arrays);
+    fp32=True makes an SpB_FP32 matrix and routes every call to the
+    reference's FP32 entry points."""
 
-    def __init__(self, nrows, ncols, ptr, col, val):
+    def __init__(self, nrows, ncols, ptr, col, val, fp32: bool = False):
         self.nrows, self.ncols = nrows, ncols
+        self.fp32 = fp32
+        self.real = np.float32 if fp32 else np.float64
         self.ptr = np.ascontiguousarray(ptr, dtype=np.int64)
         self.col = np.ascontiguousarray(col, dtype=np.int32)
-        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        self.val = np.ascontiguousarray(val, dtype=self.real)
         self.nnz = int(self.ptr[-1])
-        self.h = ref().ref_matrix(nrows, ncols, _ptr(self.ptr), _ptr(self.col), _ptr(self.val))
+        mk = ref().ref_matrix32 if fp32 else ref().ref_matrix
+        self.h = mk(nrows, ncols, _ptr(self.ptr), _ptr(self.col), _ptr(self.val))
 
     def close(self):
         if self.h:
@@ -344,20 +390,20 @@ class RefMatrix:
             pass
 
     def ird(self) -> int:
-        return int(ref().ref_ird(self.h))
+        return int((ref().ref_ird32 if self.fp32 else ref().ref_ird)(self.h))
 
     def build_d(self) -> dict:
         r = ref()
         P = r.ref_num_threads()
-        L = r.ref_lanes_f64()
-        p = r.ref_build_d(self.h)
-        G = r.ref_d_groups(p, self.h)
+        L = r.ref_lanes_f32() if self.fp32 else r.ref_lanes_f64()
+        p = (r.ref_build_d32 if self.fp32 else r.ref_build_d)(self.h)
+        G = (r.ref_d32_groups if self.fp32 else r.ref_d_groups)(p, self.h)
         out = dict(coords=np.zeros((P, 4), np.int32), ecr=np.zeros(max(self.nnz, 1), np.int32),
                    nec=np.zeros(P, np.int32), win=np.zeros(max(self.nnz, 1), np.int32),
                    v_row_ptr=np.zeros(self.nrows + 1, np.int32),
                    col_start=np.zeros(max(G, 1), np.int32),
-                   val_align=np.zeros(max(G * L, 1), np.float64))
-        r.ref_d_export(p, self.h, *[_ptr(out[k]) for k in
+                   val_align=np.zeros(max(G * L, 1), self.real))
+        (r.ref_d32_export if self.fp32 else r.ref_d_export)(p, self.h, *[_ptr(out[k]) for k in
                                     ("coords", "ecr", "nec", "win", "v_row_ptr", "col_start",
                                      "val_align")])
         out["ecr"] = out["ecr"][: self.nnz]
@@ -370,14 +416,14 @@ class RefMatrix:
     def build_s(self) -> dict:
         r = ref()
         P = r.ref_num_threads()
-        p = r.ref_build_s(self.h)
+        p = (r.ref_build_s32 if self.fp32 else r.ref_build_s)(self.h)
         n, z = self.nrows, self.nnz
         out = dict(nec=np.zeros(P, np.int32), win=np.zeros(max(z, 1), np.int32),
                    spvv_len=np.zeros(P, np.int32), tasks=np.zeros(max(n, 1), np.int32),
                    row_begin=np.zeros(max(n, 1), np.int32), row_end=np.zeros(max(n, 1), np.int32),
-                   s_col=np.zeros(max(z, 1), np.int32), s_val=np.zeros(max(z, 1), np.float64),
+                   s_col=np.zeros(max(z, 1), np.int32), s_val=np.zeros(max(z, 1), self.real),
                    task_se=np.zeros((P, 2), np.int32))
-        r.ref_s_export(p, self.h, *[_ptr(out[k]) for k in
+        (r.ref_s32_export if self.fp32 else r.ref_s_export)(p, self.h, *[_ptr(out[k]) for k in
                                     ("nec", "win", "spvv_len", "tasks", "row_begin", "row_end",
                                      "s_col", "s_val", "task_se")])
         out["win"] = out["win"][: int(out["nec"].sum())]
@@ -397,9 +443,13 @@ class RefMatrix:
         return dict(coords=coords, _plan=p)
 
     def spmv(self, variant: str, plan: dict, x, reps: int = 1) -> np.ndarray:
-        y = np.zeros(self.nrows, np.float64)
-        x = np.ascontiguousarray(x, dtype=np.float64)
-        f = {"D": ref().ref_spmv_d, "S": ref().ref_spmv_s, "L": ref().ref_spmv_l}[variant]
+        y = np.zeros(self.nrows, self.real)
+        x = np.ascontiguousarray(x, dtype=self.real)
+        r = ref()
+        if self.fp32:
+            f = {"D": r.ref_spmv_d32, "S": r.ref_spmv_s32, "L": r.ref_spmv_l32}[variant]
+        else:
+            f = {"D": r.ref_spmv_d, "S": r.ref_spmv_s, "L": r.ref_spmv_l}[variant]
         f(plan["_plan"], self.h, _ptr(x), _ptr(y), reps)
         return y
 
@@ -411,5 +461,9 @@ class RefMatrix:
         return float(t), y
 
     def free(self, variant: str, plan: dict):
-        f = {"D": ref().ref_free_d, "S": ref().ref_free_s, "L": ref().ref_free_l}[variant]
+        r = ref()
+        if self.fp32:
+            f = {"D": r.ref_free_d32, "S": r.ref_free_s32, "L": r.ref_free_l}[variant]
+        else:
+            f = {"D": r.ref_free_d, "S": r.ref_free_s, "L": r.ref_free_l}[variant]
         f(plan["_plan"])

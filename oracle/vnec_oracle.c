@@ -316,6 +316,23 @@ void orc_spmv_serial(const int64_t *ptr, const int32_t *col, const double *val, 
     }
 }
 
+/* The reference harness's FP32 serial SpMV (test_spmv_vnec.hpp:34-54):
+ * per row, t1 += t2 * t3 in float, the product rounded to float before the
+ * add (no contraction into a fused multiply-add). */
+__attribute__((optimize("fp-contract=off")))
+void orc_spmv_serial_f32(const int64_t *ptr, const int32_t *col, const float *val, const float *x,
+                         int64_t nrows, float *y)
+{
+    for (int64_t i = 0; i < nrows; i++) {
+        float t1 = 0.0f;
+        for (int64_t j = ptr[i]; j < ptr[i + 1]; j++) {
+            float t2 = val[j], t3 = x[col[j]];
+            t1 += t2 * t3;
+        }
+        y[i] = t1;
+    }
+}
+
 /* Tolerance scale (|A||x|)_i and a long-double row sum for the per-row
  * gate |y - y_ref| <= tol * (|A||x|)_i. */
 void orc_spmv_abs(const int64_t *ptr, const int32_t *col, const double *val, const double *x,

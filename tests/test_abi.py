@@ -89,8 +89,10 @@ def test_vector_layout_and_elements():
 def test_error_paths_return_failure_not_exit():
     lib = spb.init()
     h = C.c_void_p()
-    assert lib.SpB_Matrix_new(C.byref(h), 10, 3, 3) == 1  # FP32 is not on the path
-    assert "SpB_FP64" in L.last_error()
+    assert lib.SpB_Matrix_new(C.byref(h), 4, 3, 3) == 1  # SpB_INT32 is not on the path
+    assert "SpB_FP64 or SpB_FP32" in L.last_error()
+    assert lib.SpB_Vector_new(C.byref(h), 4, 3) == 1
+    assert "SpB_FP64 or SpB_FP32" in L.last_error()
     A = spb.Matrix(4, 4)
     bad_r = np.array([0, 9], np.uint64)
     cc = np.array([0, 1], np.uint64)
@@ -139,3 +141,40 @@ def test_invalid_structure_is_refused_before_any_kernel(which, msg):
         s.ptr[2] = 1
     with pytest.raises(RuntimeError, match=msg):
         spb.partition(A, 2)
+
+
+def test_fp32_objects_and_type_checks_on_the_host():
+    """FP32 matrices and vectors keep the reference layouts (element_type
+    SpB_FP32 = 10, element_size 4, float values); mixing value types is
+    refused before any device work."""
+    import numpy as np_
+    rng = np_.random.default_rng(1)
+    n, m, z = 30, 20, 150
+    r, c = rng.integers(0, n, z), rng.integers(0, m, z)
+    v = rng.uniform(-1, 1, z).astype(np_.float32)
+    A = spb.Matrix.from_coo(n, m, r, c, v, dtype=np_.float32)
+    s = _view(A.h, RefMatrixLayout)
+    assert (s.element_type, s.format_type, s.element_size) == (10, 1, 4)
+    order = np_.lexsort((np_.arange(z), c, r))
+    vals = np_.ctypeslib.as_array(C.cast(s.val, C.POINTER(C.c_float)), shape=(z,))
+    assert np_.array_equal(vals, v[order])
+    x = spb.Vector(m, np_.float32)
+    sv = _view(x.h, RefVectorLayout)
+    assert (sv.element_type, sv.element_size) == (10, 4)
+    x.set(3, 1.5)
+    assert x.get(3) == 1.5
+    assert np_.ctypeslib.as_array(C.cast(sv.values, C.POINTER(C.c_float)), shape=(m,))[3] == 1.5
+    lib = L.load()
+    assert lib.SpB_Vector_setElement_FP64(x.h, 1.0, 0) == 1 and "FP32 values" in L.last_error()
+    out = C.c_double()
+    assert lib.SpB_Vector_extractElement_FP64(C.byref(out), x.h, 0) == 1
+    A64 = spb.Matrix.from_coo(n, m, r, c, v.astype(np_.float64))
+    assert lib.SpB_Matrix_build_FP32(A64.h, None, None, None, 0, 0) == 1
+    assert "another value type" in L.last_error()
+    assert not lib.Build_VNEC_D_FP32(A64.h) and "FP64 values" in L.last_error()
+    assert not lib.Build_VNEC_S_FP32(A64.h) and "FP64 values" in L.last_error()
+    assert not lib.Build_VNEC_D_FP64(A.h) and "FP32 values" in L.last_error()
+    assert not lib.Build_VNEC_S_FP64(A.h) and "FP32 values" in L.last_error()
+    y64 = spb.Vector(n)
+    assert lib.SpB_mxv(y64.h, A.h, x.h, None) == 1 and "FP32" in L.last_error()
+    assert lib.IRD_VNEC(A.h, 4) == 1 and "SpB_FP64 or SpB_FP32" in L.last_error()

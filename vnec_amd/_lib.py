@@ -57,6 +57,28 @@ class VNEC_S_FP64(C.Structure):
                 ("reorder_mat", CsrF64)]
 
 
+class VNEC_D_FP32(C.Structure):
+    _fields_ = [("thread_coord_start", C.POINTER(Coord)), ("thread_coord_end", C.POINTER(Coord)),
+                ("ecr_indices", C.POINTER(C.c_int)), ("NEC_NUM", C.POINTER(C.c_int)),
+                ("use_x_indices", C.POINTER(C.POINTER(C.c_int))),
+                ("ecr_xx_val", C.POINTER(C.POINTER(C.c_float))),
+                ("v_row_ptr", C.POINTER(C.c_int)), ("col_start", C.POINTER(C.c_int)),
+                ("val_align", C.POINTER(C.c_float))]
+
+
+class CsrF32(C.Structure):
+    _fields_ = [("nnz", C.POINTER(C.c_float)), ("col", C.POINTER(C.c_int)),
+                ("row_begin", C.POINTER(C.c_int)), ("row_end", C.POINTER(C.c_int)),
+                ("task_start", C.POINTER(C.c_int)), ("task_end", C.POINTER(C.c_int))]
+
+
+class VNEC_S_FP32(C.Structure):
+    _fields_ = [("NEC_NUM", C.POINTER(C.c_int)), ("use_x_indices", C.POINTER(C.POINTER(C.c_int))),
+                ("ecr_xx_val", C.POINTER(C.POINTER(C.c_float))),
+                ("spvv16_len", C.POINTER(C.c_int)), ("tasks", C.POINTER(C.POINTER(C.c_int))),
+                ("reorder_mat", CsrF32)]
+
+
 class SpB_Object_Format_ext(C.Union):
     _fields_ = [("vector_format", C.c_int), ("matrix_format", C.c_int)]
 
@@ -71,11 +93,14 @@ SIGNATURES = {
     "SpB_Vector_nvals": (Info, [C.POINTER(u64), vp]),
     "SpB_Vector_setElement_FP64": (Info, [vp, C.c_double, u64]),
     "SpB_Vector_extractElement_FP64": (Info, [C.POINTER(C.c_double), vp, u64]),
+    "SpB_Vector_setElement_FP32": (Info, [vp, C.c_float, u64]),
+    "SpB_Vector_extractElement_FP32": (Info, [C.POINTER(C.c_float), vp, u64]),
     "SpB_Matrix_nrows": (Info, [C.POINTER(u64), vp]),
     "SpB_Matrix_ncols": (Info, [C.POINTER(u64), vp]),
     "SpB_Matrix_new": (Info, [C.POINTER(vp), C.c_int, u64, u64]),
     "SpB_Matrix_free": (Info, [C.POINTER(vp)]),
     "SpB_Matrix_build_FP64": (Info, [vp, vp, vp, vp, u64, C.c_int]),
+    "SpB_Matrix_build_FP32": (Info, [vp, vp, vp, vp, u64, C.c_int]),
     "SpB_mxv": (Info, [vp, vp, vp, vp]),
     "IRD_VNEC": (C.c_int, [vp, C.c_int]),
     "Build_VNEC_L": (C.POINTER(VNEC_L), [vp]),
@@ -84,15 +109,23 @@ SIGNATURES = {
     "SpMV_VNEC_L_FP64": (None, [vp, vp, vp, vp]),
     "SpMV_VNEC_D_FP64": (None, [vp, vp, vp, vp]),
     "SpMV_VNEC_S_FP64": (None, [vp, vp, vp, vp]),
+    "Build_VNEC_D_FP32": (C.POINTER(VNEC_D_FP32), [vp]),
+    "Build_VNEC_S_FP32": (C.POINTER(VNEC_S_FP32), [vp]),
+    "SpMV_VNEC_L_FP32": (None, [vp, vp, vp, vp]),
+    "SpMV_VNEC_D_FP32": (None, [vp, vp, vp, vp]),
+    "SpMV_VNEC_S_FP32": (None, [vp, vp, vp, vp]),
     "SpB_Global_Option_set_ext": (Info, [C.c_int, SpB_Object_Format_ext]),
     # vnec_ext.h
     "vnec_last_error": (C.c_char_p, []),
     "vnec_device_count_ext": (C.c_int, []),
     "SpB_Matrix_import_csr_ext": (Info, [vp, vp, vp, vp]),
+    "SpB_Matrix_import_csr_fp32_ext": (Info, [vp, vp, vp, vp]),
     "SpB_Matrix_nvals_ext": (Info, [C.POINTER(u64), vp]),
     "vnec_batch_import_ext": (Info, [vp, i64, vp, vp, vp, vp, vp]),
     "SpB_Vector_import_ext": (Info, [vp, vp]),
     "SpB_Vector_export_ext": (Info, [vp, vp]),
+    "SpB_Vector_import_fp32_ext": (Info, [vp, vp]),
+    "SpB_Vector_export_fp32_ext": (Info, [vp, vp]),
     "SpB_Vector_device_resident_ext": (Info, [vp, C.c_bool]),
     "SpB_Vector_sync_host_ext": (Info, [vp]),
     "SpB_Vector_device_ptr_ext": (vp, [vp]),
@@ -119,7 +152,7 @@ SIGNATURES = {
                             C.POINTER(C.c_int)]),
     "vnec_read_graph_ext": (Info, [C.c_char_p, C.POINTER(i64), C.POINTER(vp), C.POINTER(vp)]),
     "vnec_host_free_ext": (None, [vp]),
-    "SpB_Matrix_read_ext": (Info, [C.POINTER(vp), C.c_char_p, C.c_int]),
+    "SpB_Matrix_read_ext": (Info, [C.POINTER(vp), C.c_char_p, C.c_int, C.c_int]),
     "vnec_candidates_ext": (C.c_int, [vp, C.c_int]),
     "vnec_candidate_mask_ext": (Info, [vp, C.POINTER(C.c_uint32)]),
     "vnec_autotune_ext": (Info, [vp, C.c_int, vp, C.POINTER(C.c_int)]),
@@ -136,6 +169,8 @@ SIGNATURES = {
     "vnec_select_ext": (Info, [vp, C.c_int, C.POINTER(C.c_int)]),
     "VNEC_D_free_fp64_ext": (Info, [vp]),
     "VNEC_S_free_fp64_ext": (Info, [vp]),
+    "VNEC_D_free_fp32_ext": (Info, [vp]),
+    "VNEC_S_free_fp32_ext": (Info, [vp]),
     "VNEC_L_free_ext": (Info, [vp]),
     "VNEC_plan_of_ext": (vp, [vp]),
     # vnec_hip.h (thin C-ABI to the kernels)
@@ -150,6 +185,7 @@ SIGNATURES = {
     "vnec_hip_split_hilo": (C.c_int, [vp, i64, vp, vp, vp]),
     "vnec_hip_spmv_merge": (C.c_int, [vp, vp]),
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),
+    "vnec_hip_carry_fixup_f32": (C.c_int, [vp, vp, i64, vp, i64, vp]),
     "vnec_hip_merge_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "vnec_hip_stream_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "vnec_hip_spmv_stream": (C.c_int, [vp, C.c_int, vp, vp, vp]),
@@ -161,6 +197,8 @@ SIGNATURES = {
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
     "vnec_hip_zero_f64": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_zero_f32": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_widen_f32": (C.c_int, [vp, i64, vp, vp]),
     "vnec_hip_hot_supported": (C.c_int, [C.c_int, C.c_int]),
     "vnec_hip_hot_grid": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
     "vnec_hip_spmv_hot": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp]),
@@ -173,6 +211,7 @@ SIGNATURES = {
     "vnec_hip_s_reorder": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, i64, C.c_int, vp, vp, vp,
                                      vp, vp]),
     "vnec_hip_d_interleave": (C.c_int, [vp, i64, C.c_int, vp, vp]),
+    "vnec_hip_d_interleave_f32": (C.c_int, [vp, i64, C.c_int, vp, vp]),
     "vnec_hip_spmv_d": (C.c_int, [vp, vp]),
     "vnec_hip_d_supported": (C.c_int, [C.c_int, C.c_int]),
     "vnec_hip_feature_rows": (C.c_int, [vp, i64, vp, vp, vp]),

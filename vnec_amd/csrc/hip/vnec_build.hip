@@ -432,6 +432,40 @@ extern "C" int vnec_hip_zero_f64(double *y, int64_t n, void *stream)
     return (int)hipGetLastError();
 }
 
+__global__ void k_zero_f32(float *__restrict__ y, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = 0.0f;
+}
+
+extern "C" int vnec_hip_zero_f32(float *y, int64_t n, void *stream)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_f32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, y, n);
+    return (int)hipGetLastError();
+}
+
+// Device mirror of an FP32 matrix: its values widened (exactly) to FP64
+// for the layout builders, which are shared with FP64 matrices.
+__global__ void k_widen_f32(const float *__restrict__ in, int64_t n, double *__restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (double)in[i];
+}
+
+extern "C" int vnec_hip_widen_f32(const float *in, int64_t n, double *out, void *stream)
+{
+    if (n <= 0) return 0;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_widen_f32, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, in, n, out);
+    return (int)hipGetLastError();
+}
+
 extern "C" int vnec_hip_scribble(void *p, size_t n, void *stream)
 {
     static uint32_t salt = 0x9E3779B9u;

@@ -37,6 +37,18 @@ __device__ __forceinline__ double dpp_f64(double src)
     return r.d;
 }
 
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_f32(float src)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(src), CTRL, ROW_MASK, 0xF, true));
+}
+
+// One DPP move of a value of either real type.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_real(double v) { return dpp_f64<CTRL, ROW_MASK>(v); }
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_real(float v) { return dpp_f32<CTRL, ROW_MASK>(v); }
+
 // Inclusive wavefront sum scan of an int.
 __device__ __forceinline__ int wave_incl_scan_i32(int v)
 {
@@ -52,16 +64,18 @@ __device__ __forceinline__ int wave_incl_scan_i32(int v)
 // Inclusive segmented wavefront scan: the pair (f, v) combines as
 // (f1, v1) . (f2, v2) = (f1 | f2, f2 ? v2 : v1 + v2), i.e. a set flag
 // restarts the running sum at that lane's own value.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ void seg_step(int &f, double &v)
+template <int CTRL, int ROW_MASK, typename F>
+__device__ __forceinline__ void seg_step(int &f, F &v)
 {
     int fo = dpp_i32<CTRL, ROW_MASK>(f);
-    double vo = dpp_f64<CTRL, ROW_MASK>(v);
+    F vo = dpp_real<CTRL, ROW_MASK>(v);
     if (!f) v += vo;
     f |= fo;
 }
 
-__device__ __forceinline__ void wave_seg_scan_f64(int &f, double &v)
+// F = double or float.
+template <typename F>
+__device__ __forceinline__ void wave_seg_scan(int &f, F &v)
 {
     seg_step<DPP_ROW_SHR(1), 0xF>(f, v);
     seg_step<DPP_ROW_SHR(2), 0xF>(f, v);
@@ -70,6 +84,8 @@ __device__ __forceinline__ void wave_seg_scan_f64(int &f, double &v)
     seg_step<DPP_ROW_BCAST15, 0xA>(f, v);
     seg_step<DPP_ROW_BCAST31, 0xC>(f, v);
 }
+
+__device__ __forceinline__ void wave_seg_scan_f64(int &f, double &v) { wave_seg_scan(f, v); }
 
 // Value of lane (l - 1) (lane 0 gets `first`).
 __device__ __forceinline__ double wave_shift_up1_f64(double v, double first)
@@ -80,6 +96,15 @@ __device__ __forceinline__ double wave_shift_up1_f64(double v, double first)
     r.i[1] = __shfl_up(u.i[1], 1, VNEC_WAVE);
     return lane_id() == 0 ? first : r.d;
 }
+
+__device__ __forceinline__ float wave_shift_up1_f32(float v, float first)
+{
+    float r = __shfl_up(v, 1, VNEC_WAVE);
+    return lane_id() == 0 ? first : r;
+}
+
+__device__ __forceinline__ double wave_shift_up1(double v, double first) { return wave_shift_up1_f64(v, first); }
+__device__ __forceinline__ float wave_shift_up1(float v, float first) { return wave_shift_up1_f32(v, first); }
 
 __device__ __forceinline__ int wave_shift_up1_i32(int v, int first)
 {
@@ -134,10 +159,11 @@ static inline int64_t xcd_grid(int64_t ntiles)
 // lane per window position, finding its run by binary search over the run
 // descriptors staged in s_runs (2 * nr ints of LDS; with s_runs == nullptr
 // the long-run form is always used).
-__device__ __forceinline__ void fill_window(const double *__restrict__ x,
+template <typename F>
+__device__ __forceinline__ void fill_window(const F *__restrict__ x,
                                             const int32_t *__restrict__ run_start,
                                             const int32_t *__restrict__ run_wpos, int r0, int r1,
-                                            int nec, double *s_win, int *s_runs)
+                                            int nec, F *s_win, int *s_runs)
 {
     const int nr = r1 - r0;
     if (nr <= 0) return;
@@ -166,6 +192,45 @@ __device__ __forceinline__ void fill_window(const double *__restrict__ x,
         }
     }
 }
+
+// Joins per-thread (flag, tail) pairs of a segmented sum across the
+// workgroup.  Returns the exclusive carry into this thread; *incl (if not
+// NULL) gets the inclusive value (the last thread's closes the tile).
+// s_f / s_v: VNEC_NWAVES + 1 entries.
+template <typename F>
+__device__ __forceinline__ F block_seg_carry(int flag, F tail, int *s_f, F *s_v, F *incl)
+{
+    int f = flag;
+    F v = tail;
+    wave_seg_scan(f, v);
+    const int w = wave_id();
+    if (lane_id() == VNEC_WAVE - 1) {
+        s_f[w] = f;
+        s_v[w] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        F c = 0;
+        for (int k = 0; k < VNEC_NWAVES; k++) {
+            const F tw = s_v[k];
+            const int fw = s_f[k];
+            s_v[k] = c;  // carry into wave k
+            c = fw ? tw : c + tw;
+        }
+    }
+    __syncthreads();
+    const F cw = s_v[w];
+    const F blk = f ? v : v + cw;
+    if (incl) *incl = blk;
+    return wave_shift_up1(blk, cw);
+}
+
+// Value type a kernel computes in for a value encoding: encodings 0-2
+// (FP64 values, FP32 hi + lo, FP32 hi) compute in FP64; encoding 3 is the
+// FP32 arithmetic of the FP32 API (FP32 values, x, y and sums).
+#define VNEC_ENC_F32_K 3
+template <int ENC> struct enc_real { typedef double type; };
+template <> struct enc_real<VNEC_ENC_F32_K> { typedef float type; };
 
 // Merge-path search (spmv_vnec_common.c:3-33 semantics): merge list A =
 // row ends re[0..na), list B = b0 + 0..nb; returns the number of A items

@@ -10,6 +10,9 @@
 // access, and no cross-lane reduction is needed.  Padding slots are never
 // read (each lane stops at its own row length).
 //
+// Values are FP64, FP32 hi + lo or FP32 hi (summed in FP64), or, for the
+// FP32 API, FP32 with FP32 x and sums.
+//
 // Tiles are runs of whole slices, balanced by the merge-path partitioner
 // over the slice layout (row snap on).  Each tile's x window (its distinct
 // columns, ECR) is gathered into LDS and columns are stored as 16-bit
@@ -133,26 +136,34 @@ extern "C" int vnec_hip_slice_fill(const int32_t *ptr, const int32_t *col, const
 
 // --------------------------------------------------------------- SpMV
 template <int ENC>
-__device__ __forceinline__ double slice_value(const vnec_slice_args &a, int64_t slot)
+__device__ __forceinline__ typename enc_real<ENC>::type slice_value(const vnec_slice_args &a, int64_t slot)
 {
     if (ENC == 0) return __builtin_nontemporal_load(a.sv + slot);
     if (ENC == 1)
         return (double)__builtin_nontemporal_load(a.shi + slot) +
                (double)__builtin_nontemporal_load(a.slo + slot);
-    return (double)__builtin_nontemporal_load(a.shi + slot);
+    return __builtin_nontemporal_load(a.shi + slot);  // FP32 hi (FP64 sums), or FP32 values
 }
+
+__device__ __forceinline__ const double *sx_of(const vnec_slice_args &a, double) { return a.x; }
+__device__ __forceinline__ const float *sx_of(const vnec_slice_args &a, float) { return a.x32; }
+__device__ __forceinline__ double *sy_of(const vnec_slice_args &a, double) { return a.y; }
+__device__ __forceinline__ float *sy_of(const vnec_slice_args &a, float) { return a.y32; }
 
 // One workgroup per tile; wavefront w takes slices w, w + 4, ... of it.
 template <int ENC, bool WIN>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_slice(vnec_slice_args a, int64_t grid)
 {
-    extern __shared__ double s_win[];
+    typedef typename enc_real<ENC>::type F;
+    extern __shared__ double s_dyn[];
+    F *s_win = (F *)s_dyn;
     const int64_t t = xcd_tile(blockIdx.x, grid);
     if (t >= a.ntiles) return;
+    const F *__restrict__ x = sx_of(a, F());
     const int2 *co = (const int2 *)a.tile_slices;
     const int sb = co[t].x, se = co[t + 1].x;
     if (WIN) {
-        fill_window(a.x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], a.nec[t], s_win,
+        fill_window(x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], a.nec[t], s_win,
                     (int *)nullptr);
         __syncthreads();
     }
@@ -164,10 +175,10 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_slice(vnec_slice_args a, in
         if (row < a.nrows) rl = a.ptr[row + 1] - a.ptr[row];
         const int len = a.slice_len[s];
         const int64_t base = (int64_t)a.slice_ptr[s] + lane;
-        double acc = 0.0;
+        F acc = 0;
         int l = 0;
         for (; l + 4 <= len; l += 4) {
-            double v[4];
+            F v[4];
             int c[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -180,18 +191,18 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_slice(vnec_slice_args a, in
             }
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                if (l + u < rl) acc += v[u] * (WIN ? s_win[c[u]] : a.x[c[u]]);
+                if (l + u < rl) acc += v[u] * (WIN ? s_win[c[u]] : x[c[u]]);
         }
         for (; l < len; l++) {
             if (l < rl) {
                 const int64_t slot = base + (int64_t)l * SLICE;
-                const double v = slice_value<ENC>(a, slot);
+                const F v = slice_value<ENC>(a, slot);
                 const int c = WIN ? (int)__builtin_nontemporal_load(a.slcol + slot)
                                   : __builtin_nontemporal_load(a.scol + slot);
-                acc += v * (WIN ? s_win[c] : a.x[c]);
+                acc += v * (WIN ? s_win[c] : x[c]);
             }
         }
-        if (row < a.nrows) a.y[row] = acc;
+        if (row < a.nrows) sy_of(a, F())[row] = acc;
     }
 }
 
@@ -203,10 +214,12 @@ static slice_kernel_t pick_slice(int enc, bool win)
         if (enc == 0) return k_spmv_slice<0, true>;
         if (enc == 1) return k_spmv_slice<1, true>;
         if (enc == 2) return k_spmv_slice<2, true>;
+        if (enc == VNEC_ENC_F32_K) return k_spmv_slice<VNEC_ENC_F32_K, true>;
     } else {
         if (enc == 0) return k_spmv_slice<0, false>;
         if (enc == 1) return k_spmv_slice<1, false>;
         if (enc == 2) return k_spmv_slice<2, false>;
+        if (enc == VNEC_ENC_F32_K) return k_spmv_slice<VNEC_ENC_F32_K, false>;
     }
     return nullptr;
 }
@@ -217,7 +230,8 @@ extern "C" int vnec_hip_spmv_slice(const vnec_slice_args *a, void *stream)
     if (!k) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
     int64_t grid = xcd_grid(a->ntiles);
-    size_t lds = a->win ? sizeof(double) * (size_t)(a->max_nec > 0 ? a->max_nec : 1) : 0;
+    size_t esz = a->encoding == VNEC_ENC_F32_K ? sizeof(float) : sizeof(double);
+    size_t lds = a->win ? esz * (size_t)(a->max_nec > 0 ? a->max_nec : 1) : 0;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), lds, (hipStream_t)stream, *a, grid);
     return (int)hipGetLastError();
 }

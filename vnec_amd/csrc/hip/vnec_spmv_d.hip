@@ -1,4 +1,5 @@
-// vnec_spmv_d.hip — VNEC-D FP64 SpMV for gfx950 (variant D).
+// vnec_spmv_d.hip — VNEC-D SpMV for gfx950 (variant D), in FP64 or, for
+// the FP32 API, in FP32 (values, x, sums).
 //
 // The D format (spmv_vnec_D_for_mxv.c:356-392) cuts each row segment of a
 // partition into groups of L consecutive positions of the partition's
@@ -29,57 +30,43 @@
 #include "vnec_common.h"
 #include "vnec_hip.h"
 
-__global__ void k_d_interleave(const double *__restrict__ in, int64_t G, int L,
-                               double *__restrict__ out)
+template <typename T>
+__global__ void k_d_interleave(const double *__restrict__ in, int64_t G, int L, T *__restrict__ out)
 {
     const int64_t n = G * L;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t g = i / L, j = i - g * L;
-        out[((g >> 6) * L + j) * 64 + (g & 63)] = in[i];
+        out[((g >> 6) * L + j) * 64 + (g & 63)] = (T)in[i];
     }
 }
 
-// Joins per-thread (flag, tail) pairs across the workgroup (as in the
-// merge kernels); returns the exclusive carry into this thread.
-__device__ __forceinline__ double d_block_seg_carry(int flag, double tail, int *s_f, double *s_v)
-{
-    int f = flag;
-    double v = tail;
-    wave_seg_scan_f64(f, v);
-    const int w = wave_id();
-    if (lane_id() == VNEC_WAVE - 1) {
-        s_f[w] = f;
-        s_v[w] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double c = 0.0;
-        for (int k = 0; k < VNEC_NWAVES; k++) {
-            const double tw = s_v[k];
-            const int fw = s_f[k];
-            s_v[k] = c;
-            c = fw ? tw : c + tw;
-        }
-    }
-    __syncthreads();
-    const double cw = s_v[w];
-    const double blk = f ? v : v + cw;
-    return wave_shift_up1_f64(blk, cw);
-}
+// Operands in the kernel's value type (FP64, or FP32 for the FP32 API).
+__device__ __forceinline__ const double *dvals_of(const vnec_d_args &a, double) { return a.vals; }
+__device__ __forceinline__ const float *dvals_of(const vnec_d_args &a, float) { return a.vals32; }
+__device__ __forceinline__ const double *dval_of(const vnec_d_args &a, double) { return a.val; }
+__device__ __forceinline__ const float *dval_of(const vnec_d_args &a, float) { return a.val32; }
+__device__ __forceinline__ const double *dx_of(const vnec_d_args &a, double) { return a.x; }
+__device__ __forceinline__ const float *dx_of(const vnec_d_args &a, float) { return a.x32; }
+__device__ __forceinline__ double *dy_of(const vnec_d_args &a, double) { return a.y; }
+__device__ __forceinline__ float *dy_of(const vnec_d_args &a, float) { return a.y32; }
+__device__ __forceinline__ double *dcarry_of(const vnec_d_args &a, double) { return a.carry; }
+__device__ __forceinline__ float *dcarry_of(const vnec_d_args &a, float) { return a.carry32; }
 
-template <int IPT, int L>
+template <int IPT, int L, typename F>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t grid)
 {
     constexpr int T = VNEC_BLOCK * IPT;
-    __shared__ double s_win[T + L];
-    __shared__ double s_prod[T];
+    __shared__ F s_win[T + L];
+    __shared__ F s_prod[T];
     __shared__ int s_rowend[T];
     __shared__ int s_f[VNEC_NWAVES + 1];
-    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ F s_v[VNEC_NWAVES + 1];
 
     const int64_t t = xcd_tile(blockIdx.x, grid);
     if (t >= a.ntiles) return;
+    const F *__restrict__ x = dx_of(a, F());
+    F *__restrict__ y = dy_of(a, F());
     const int2 *co = (const int2 *)a.coords;
     const int2 c0 = co[t], c1 = co[t + 1];
     const int x0 = c0.x, y0 = c0.y, x1 = c1.x, y1 = c1.y;
@@ -88,9 +75,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t gr
     const int nec = a.nec[t];
 
     // 1. window + L zeros
-    fill_window(a.x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], nec, s_win,
+    fill_window(x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], nec, s_win,
                 (int *)s_prod);
-    if (tid < L) s_win[nec + tid] = 0.0;
+    if (tid < L) s_win[nec + tid] = 0;
     __syncthreads();
 
     // 2. group products and row ends (group space)
@@ -102,12 +89,12 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t gr
         const int k = u * VNEC_BLOCK + tid;
         if (k < ng) {
             const int64_t g = (int64_t)G0 + k;
-            const double *v = a.vals + ((g >> 6) * L) * 64 + (g & 63);
+            const F *v = dvals_of(a, F()) + ((g >> 6) * L) * 64 + (g & 63);
             const int cs = __builtin_nontemporal_load(a.col_start + g);
-            double vv[L];
+            F vv[L];
 #pragma unroll
             for (int j = 0; j < L; j++) vv[j] = __builtin_nontemporal_load(v + j * 64);
-            double s = 0.0;
+            F s = 0;
 #pragma unroll
             for (int j = 0; j < L; j++) s = fma(vv[j], s_win[cs + j], s);
             s_prod[k] = s;
@@ -121,7 +108,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t gr
     const int d1 = d0 + IPT < items ? d0 + IPT : items;
     int rx = (int)merge_search(s_rowend, nrow, G0, ng, d0);
     int ny = d0 - rx;
-    double acc = 0.0, head_val = 0.0;
+    F acc = 0, head_val = 0;
     int flag = 0, head_row = 0;
 #pragma unroll
     for (int u = 0; u < IPT; u++) {
@@ -132,9 +119,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t gr
                     head_val = acc;
                     flag = 1;
                 } else {
-                    a.y[x0 + rx] = acc;
+                    y[x0 + rx] = acc;
                 }
-                acc = 0.0;
+                acc = 0;
                 rx++;
             } else {
                 acc += s_prod[ny];
@@ -142,32 +129,33 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_d(vnec_d_args a, int64_t gr
             }
         }
     }
-    const double cin = d_block_seg_carry(flag, acc, s_f, s_v);
-    if (flag) a.y[head_row] = head_val + cin;
+    const F cin = block_seg_carry(flag, acc, s_f, s_v, (F *)nullptr);
+    if (flag) y[head_row] = head_val + cin;
 
     // 4. head of row x1 inside the tile, from the CSR values
-    double c = 0.0;
+    F c = 0;
     if (x1 < a.nrows) {
+        const F *__restrict__ val = dval_of(a, F());
         const int b = a.ptr[x1] > y0 ? a.ptr[x1] : y0;
-        for (int j = b + tid; j < y1; j += VNEC_BLOCK) c = fma(a.val[j], s_win[a.lcol[j]], c);
+        for (int j = b + tid; j < y1; j += VNEC_BLOCK) c = fma(val[j], s_win[a.lcol[j]], c);
     }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, VNEC_WAVE);
     __syncthreads();  // s_v is free again
     if (lane_id() == 0) s_v[wave_id()] = c;
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
+        F s = 0;
         for (int w = 0; w < VNEC_NWAVES; w++) s += s_v[w];
-        a.carry[t] = s;
+        dcarry_of(a, F())[t] = s;
     }
 }
 
 typedef void (*d_kernel_t)(vnec_d_args, int64_t);
 
-static d_kernel_t pick_d(int ipt, int lanes)
+static d_kernel_t pick_d(int ipt, int lanes, bool fp32)
 {
 #define CASE_D(I, LL) \
-    if (ipt == I && lanes == LL) return k_spmv_d<I, LL>;
+    if (ipt == I && lanes == LL) return fp32 ? k_spmv_d<I, LL, float> : k_spmv_d<I, LL, double>;
     CASE_D(5, 4) CASE_D(5, 8) CASE_D(5, 16)
     CASE_D(7, 4) CASE_D(7, 8) CASE_D(7, 16)
     CASE_D(9, 4) CASE_D(9, 8) CASE_D(9, 16)
@@ -175,21 +163,31 @@ static d_kernel_t pick_d(int ipt, int lanes)
     return nullptr;
 }
 
-extern "C" int vnec_hip_d_supported(int ipt, int lanes) { return pick_d(ipt, lanes) != nullptr; }
+extern "C" int vnec_hip_d_supported(int ipt, int lanes) { return pick_d(ipt, lanes, false) != nullptr; }
 
 extern "C" int vnec_hip_d_interleave(const double *in, int64_t G, int L, double *out, void *stream)
 {
     if (G <= 0) return 0;
     int64_t n = G * L, blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(k_d_interleave, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_d_interleave<double>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       in, G, L, out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_d_interleave_f32(const double *in, int64_t G, int L, float *out, void *stream)
+{
+    if (G <= 0) return 0;
+    int64_t n = G * L, blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_d_interleave<float>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        in, G, L, out);
     return (int)hipGetLastError();
 }
 
 extern "C" int vnec_hip_spmv_d(const vnec_d_args *a, void *stream)
 {
-    d_kernel_t k = pick_d(a->ipt, a->lanes);
+    d_kernel_t k = pick_d(a->ipt, a->lanes, a->fp32 != 0);
     if (!k) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
     int64_t grid = xcd_grid(a->ntiles);

@@ -1,4 +1,4 @@
-// vnec_spmv_merge.hip — merge-path FP64 SpMV for gfx950 (variants L and E).
+// vnec_spmv_merge.hip — merge-path SpMV for gfx950 (variants L and E).
 //
 // Restates the merge-based CSR of spmv_vnec_L_for_mxv.c:124-165 for wave64
 // workgroups: a tile is a fixed share of the merge list (row ends +
@@ -8,7 +8,8 @@
 //      described as runs of consecutive columns — is gathered into LDS
 //      (the GPU form of the compact-x gather, spmv_vnec_D_for_mxv.c:416-436);
 //   2. row ends and the products a_j * x(col_j) are staged in LDS with
-//      coalesced loads (values FP64, or FP32 hi + lo / hi only);
+//      coalesced loads (values FP64, or FP32 hi + lo / hi only, all summed
+//      in FP64; or, for the FP32 API, FP32 values, x and sums);
 //   3. each thread walks IPT merge items sequentially, writing rows it
 //      completes and keeping the partial sums at its two ends;
 //   4. a segmented scan over the threads (DPP within the wavefront, LDS
@@ -21,58 +22,38 @@
 #include "vnec_hip.h"
 
 template <int ENC>
-__device__ __forceinline__ double load_value(const vnec_merge_args &a, int64_t j)
+__device__ __forceinline__ typename enc_real<ENC>::type load_value(const vnec_merge_args &a, int64_t j)
 {
     if (ENC == 0) return __builtin_nontemporal_load(a.val + j);
     if (ENC == 1)
         return (double)__builtin_nontemporal_load(a.hi + j) +
                (double)__builtin_nontemporal_load(a.lo + j);
-    return (double)__builtin_nontemporal_load(a.hi + j);
+    return __builtin_nontemporal_load(a.hi + j);  // FP32 hi (FP64 sums), or FP32 values
 }
 
-// Joins per-thread (flag, tail) pairs across the workgroup.  Returns the
-// exclusive carry into this thread; *incl gets the inclusive value (the
-// last thread's is the tile carry).  s_f / s_v: VNEC_NWAVES + 1 entries.
-__device__ __forceinline__ double block_seg_carry(int flag, double tail, int *s_f, double *s_v,
-                                                  double *incl)
-{
-    int f = flag;
-    double v = tail;
-    wave_seg_scan_f64(f, v);
-    int w = wave_id();
-    if (lane_id() == VNEC_WAVE - 1) {
-        s_f[w] = f;
-        s_v[w] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double c = 0.0;
-        for (int k = 0; k < VNEC_NWAVES; k++) {
-            double tw = s_v[k];
-            int fw = s_f[k];
-            s_v[k] = c;  // carry into wave k
-            c = fw ? tw : c + tw;
-        }
-    }
-    __syncthreads();
-    double cw = s_v[w];
-    double blk = f ? v : v + cw;
-    *incl = blk;
-    return wave_shift_up1_f64(blk, cw);
-}
+// x, y and the tile carries in the kernel's value type.
+__device__ __forceinline__ const double *x_of(const vnec_merge_args &a, double) { return a.x; }
+__device__ __forceinline__ const float *x_of(const vnec_merge_args &a, float) { return a.x32; }
+__device__ __forceinline__ double *y_of(const vnec_merge_args &a, double) { return a.y; }
+__device__ __forceinline__ float *y_of(const vnec_merge_args &a, float) { return a.y32; }
+__device__ __forceinline__ double *carry_of(const vnec_merge_args &a, double) { return a.carry; }
+__device__ __forceinline__ float *carry_of(const vnec_merge_args &a, float) { return a.carry32; }
 
 template <int IPT, bool WIN, int ENC>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, int64_t grid)
 {
+    typedef typename enc_real<ENC>::type F;
     constexpr int T = VNEC_BLOCK * IPT;
-    __shared__ double s_prod[T];
+    __shared__ F s_prod[T];
     __shared__ int s_rowend[T];
-    __shared__ double s_win[WIN ? T : 1];
+    __shared__ F s_win[WIN ? T : 1];
     __shared__ int s_f[VNEC_NWAVES + 1];
-    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ F s_v[VNEC_NWAVES + 1];
 
     const int64_t t = xcd_tile(blockIdx.x, grid);
     if (t >= a.ntiles) return;
+    const F *__restrict__ x = x_of(a, F());
+    F *__restrict__ y = y_of(a, F());
     const int2 *co = (const int2 *)a.coords;
     const int2 c0 = co[t], c1 = co[t + 1];
     const int x0 = c0.x, y0 = c0.y;
@@ -81,7 +62,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
 
     if (WIN) {
         // 1. window fill (run descriptors staged in s_prod when needed)
-        fill_window(a.x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], a.nec[t], s_win,
+        fill_window(x, a.run_start, a.run_wpos, a.run_off[t], a.run_off[t + 1], a.nec[t], s_win,
                     (int *)s_prod);
         __syncthreads();
     }
@@ -89,7 +70,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     // 2. row ends and products
     for (int i = tid; i < nrow; i += VNEC_BLOCK) s_rowend[i] = a.ptr[x0 + 1 + i];
     {
-        double av[IPT];
+        F av[IPT];
         int cv[IPT];
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
@@ -104,7 +85,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             int k = u * VNEC_BLOCK + tid;
-            if (k < nz) s_prod[k] = av[u] * (WIN ? s_win[cv[u]] : a.x[cv[u]]);
+            if (k < nz) s_prod[k] = av[u] * (WIN ? s_win[cv[u]] : x[cv[u]]);
         }
     }
     __syncthreads();
@@ -115,7 +96,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     const int d1 = d0 + IPT < items ? d0 + IPT : items;
     int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
     int ny = d0 - rx;
-    double acc = 0.0, head_val = 0.0;
+    F acc = 0, head_val = 0;
     int flag = 0, head_row = 0;
 #pragma unroll
     for (int u = 0; u < IPT; u++) {
@@ -126,9 +107,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
                     head_val = acc;
                     flag = 1;
                 } else {
-                    a.y[x0 + rx] = acc;
+                    y[x0 + rx] = acc;
                 }
-                acc = 0.0;
+                acc = 0;
                 rx++;
             } else {
                 acc += s_prod[ny];
@@ -138,11 +119,11 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     }
 
     // 4. join rows that span threads
-    double incl;
-    double cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
-    if (flag) a.y[head_row] = head_val + cin;
+    F incl;
+    F cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
+    if (flag) y[head_row] = head_val + cin;
     // 5. tile carry (partial row c1.x)
-    if (tid == VNEC_BLOCK - 1) a.carry[t] = incl;
+    if (tid == VNEC_BLOCK - 1) carry_of(a, F())[t] = incl;
 }
 
 // Multi-RHS (variant L): x is [ncols][K], y is [nrows][K].  Values and
@@ -227,8 +208,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeK(vnec_merge_args a, i
 // two partitions carry into one row, spmv_vnec_L_for_mxv.c:160-164).
 // Trailing tiles can be empty (ipt = ceil(M / P) may cover M early); their
 // carries belong to row nrows, which does not exist, and are zero.
-__global__ void k_carry_fixup(const int2 *__restrict__ co, const double *__restrict__ carry,
-                              int64_t ntiles, double *__restrict__ y, int64_t nrows, int K)
+template <typename F>
+__global__ void k_carry_fixup(const int2 *__restrict__ co, const F *__restrict__ carry,
+                              int64_t ntiles, F *__restrict__ y, int64_t nrows, int K)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles - 1) return;
@@ -236,7 +218,7 @@ __global__ void k_carry_fixup(const int2 *__restrict__ co, const double *__restr
     if (row >= nrows) return;
     if (t > 0 && co[t].x == row) return;  // not the first of its run
     for (int c = 0; c < K; c++) {
-        double s = 0.0;
+        F s = 0;
         for (int64_t u = t; u < ntiles - 1 && co[u + 1].x == row; u++) s += carry[u * K + c];
         y[(int64_t)row * K + c] += s;
     }
@@ -253,7 +235,7 @@ static merge_kernel_t pick1(int ipt, bool win, int enc)
 #define CASE1(I, W, E) \
     if (ipt == I && win == W && enc == E) return k_spmv_merge1<I, W, E>;
 #define ENC3(I, W) CASE1(I, W, 0) CASE1(I, W, 1) CASE1(I, W, 2)
-#define WIN2(I, _) ENC3(I, false) ENC3(I, true)
+#define WIN2(I, _) ENC3(I, false) ENC3(I, true) CASE1(I, false, VNEC_ENC_F32_K)
     IPT_LIST(WIN2, 0)
     return nullptr;
 #undef WIN2
@@ -304,7 +286,17 @@ extern "C" int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, 
 {
     if (ntiles <= 1) return 0;
     int64_t grid = (ntiles - 1 + 255) / 256;
-    hipLaunchKernelGGL(k_carry_fixup, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_carry_fixup<double>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, nrhs);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carry, int64_t ntiles,
+                                        float *y, int64_t nrows, void *stream)
+{
+    if (ntiles <= 1) return 0;
+    int64_t grid = (ntiles - 1 + 255) / 256;
+    hipLaunchKernelGGL(k_carry_fixup<float>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+                       (const int2 *)coords, carry, ntiles, y, nrows, 1);
     return (int)hipGetLastError();
 }

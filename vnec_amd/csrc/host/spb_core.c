@@ -1,6 +1,7 @@
 /*
  * spb_core.c — library runtime and the OpenSparseBLAS object subset of the
- * VNEC path (matrix CSR FP64, dense FP64 vector), with device mirrors.
+ * VNEC path (CSR matrix and dense vector, FP64 or FP32), with device
+ * mirrors.
  *
  * Behaviour follows the reference objects (src/core/src/SpB_Matrix.c,
  * SpB_Matrix_CSC_or_CSR.c, SpB_Vector.c, SpB_Vector_Dense.c): zero-filled
@@ -142,7 +143,8 @@ static void *aligned64(size_t bytes)
 SpB_Info SpB_Matrix_new(SpB_Matrix *m, SpB_Type type, SpB_Index nrows, SpB_Index ncols)
 {
     if (!m) return vnec_fail("SpB_Matrix_new: NULL handle");
-    if (type != SpB_FP64) return vnec_fail("SpB_Matrix_new: only SpB_FP64 is on the VNEC path");
+    if (type != SpB_FP64 && type != SpB_FP32)
+        return vnec_fail("SpB_Matrix_new: the VNEC path takes SpB_FP64 or SpB_FP32 values");
     if (nrows >= (SpB_Index)INT32_MAX || ncols >= (SpB_Index)INT32_MAX)
         return vnec_fail("SpB_Matrix_new: dimensions must be below 2^31 - 1");
     spb_csr *a = (spb_csr *)calloc(1, sizeof(spb_csr));
@@ -151,7 +153,7 @@ SpB_Info SpB_Matrix_new(SpB_Matrix *m, SpB_Type type, SpB_Index nrows, SpB_Index
     a->base.format_type = g_csc_default ? SpB_CSC : SpB_CSR;
     a->base.row = nrows;
     a->base.col = ncols;
-    a->base.element_size = sizeof(double);
+    a->base.element_size = type == SpB_FP32 ? sizeof(float) : sizeof(double);
     a->ptr_len = nrows + 1;
     a->ptr = (SpB_Index *)calloc((size_t)nrows + 1, sizeof(SpB_Index));
     a->magic = SPB_MAGIC_MATRIX;
@@ -224,7 +226,7 @@ static SpB_Info matrix_reserve(spb_csr *a, SpB_Index n)
     free(a->indices);
     free(a->val);
     a->indices = (SpB_Index *)aligned64(sizeof(SpB_Index) * (size_t)n);
-    a->val = aligned64(sizeof(double) * (size_t)n);
+    a->val = aligned64(a->base.element_size * (size_t)n);
     if (!a->indices || !a->val) return vnec_fail("matrix build: out of memory (%lu entries)", n);
     a->capacity = n;
     a->nnz = n;
@@ -242,29 +244,30 @@ static int cmp_entry(const void *p, const void *q)
     return a[1] < b[1] ? -1 : (a[1] > b[1] ? 1 : 0); /* a[1] = input position */
 }
 
-SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index *J, const double *X,
-                               SpB_Index nvals, const SpB_BinaryOp dup)
+static SpB_Info matrix_build(const char *fn, SpB_Type type, SpB_Matrix C, const SpB_Index *I,
+                             const SpB_Index *J, const void *X, SpB_Index nvals, const SpB_BinaryOp dup)
 {
     spb_csr *a = (spb_csr *)C;
-    if (!a || a->magic != SPB_MAGIC_MATRIX) return vnec_fail("SpB_Matrix_build_FP64: bad matrix");
+    if (!a || a->magic != SPB_MAGIC_MATRIX) return vnec_fail("%s: bad matrix", fn);
+    if (a->base.element_type != type)
+        return vnec_fail("%s: the matrix was made with another value type", fn);
     if (dup != SpB_BINARYOP_NULL)
-        return vnec_fail("SpB_Matrix_build_FP64: only SpB_BINARYOP_NULL is supported");
+        return vnec_fail("%s: only SpB_BINARYOP_NULL is supported", fn);
     if (a->base.format_type != SpB_CSR)
-        return vnec_fail("SpB_Matrix_build_FP64: the VNEC path builds CSR matrices");
-    if (nvals >= (SpB_Index)INT32_MAX) return vnec_fail("SpB_Matrix_build_FP64: nnz >= 2^31");
-    if (nvals && (!I || !J || !X)) return vnec_fail("SpB_Matrix_build_FP64: NULL arrays");
+        return vnec_fail("%s: the VNEC path builds CSR matrices", fn);
+    if (nvals >= (SpB_Index)INT32_MAX) return vnec_fail("%s: nnz >= 2^31", fn);
+    if (nvals && (!I || !J || !X)) return vnec_fail("%s: NULL arrays", fn);
     SpB_Index nr = a->base.row, nc = a->base.col;
     for (SpB_Index k = 0; k < nvals; k++)
         if (I[k] >= nr || J[k] >= nc)
-            return vnec_fail("SpB_Matrix_build_FP64: entry %lu (%lu, %lu) out of range", k, I[k],
-                             J[k]);
+            return vnec_fail("%s: entry %lu (%lu, %lu) out of range", fn, k, I[k], J[k]);
     if (matrix_reserve(a, nvals) != SpB_SUCCESS) return SpB_FAILURE;
     SpB_Index *cnt = (SpB_Index *)calloc((size_t)nr + 1, sizeof(SpB_Index));
     SpB_Index *order = (SpB_Index *)malloc(sizeof(SpB_Index) * (size_t)(nvals ? nvals : 1));
     SpB_Index *pairs = (SpB_Index *)malloc(sizeof(SpB_Index) * 2 * 64);
     if (!cnt || !order || !pairs) {
         free(cnt); free(order); free(pairs);
-        return vnec_fail("SpB_Matrix_build_FP64: out of memory");
+        return vnec_fail("%s: out of memory", fn);
     }
     for (SpB_Index k = 0; k < nvals; k++) cnt[I[k] + 1]++;
     for (SpB_Index r = 0; r < nr; r++) cnt[r + 1] += cnt[r];
@@ -278,7 +281,7 @@ SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index
             SpB_Index *np = (SpB_Index *)realloc(pairs, sizeof(SpB_Index) * 2 * pcap);
             if (!np) {
                 free(cnt); free(order); free(pairs);
-                return vnec_fail("SpB_Matrix_build_FP64: out of memory");
+                return vnec_fail("%s: out of memory", fn);
             }
             pairs = np;
         }
@@ -289,7 +292,10 @@ SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index
         qsort(pairs, (size_t)len, 2 * sizeof(SpB_Index), cmp_entry);
         for (SpB_Index k = 0; k < len; k++) {
             a->indices[b + k] = pairs[2 * k];
-            ((double *)a->val)[b + k] = X[pairs[2 * k + 1]];
+            if (type == SpB_FP32)
+                ((float *)a->val)[b + k] = ((const float *)X)[pairs[2 * k + 1]];
+            else
+                ((double *)a->val)[b + k] = ((const double *)X)[pairs[2 * k + 1]];
         }
     }
     free(cnt);
@@ -298,11 +304,25 @@ SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index
     return SpB_SUCCESS;
 }
 
-SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
-                                   const double *val)
+SpB_Info SpB_Matrix_build_FP64(SpB_Matrix C, const SpB_Index *I, const SpB_Index *J, const double *X,
+                               SpB_Index nvals, const SpB_BinaryOp dup)
+{
+    return matrix_build("SpB_Matrix_build_FP64", SpB_FP64, C, I, J, X, nvals, dup);
+}
+
+SpB_Info SpB_Matrix_build_FP32(SpB_Matrix C, const SpB_Index *I, const SpB_Index *J, const float *X,
+                               SpB_Index nvals, const SpB_BinaryOp dup)
+{
+    return matrix_build("SpB_Matrix_build_FP32", SpB_FP32, C, I, J, X, nvals, dup);
+}
+
+static SpB_Info import_csr(SpB_Matrix A, SpB_Type type, const int64_t *ptr, const int32_t *col,
+                           const void *val)
 {
     spb_csr *a = (spb_csr *)A;
     if (!a || a->magic != SPB_MAGIC_MATRIX) return vnec_fail("SpB_Matrix_import_csr_ext: bad matrix");
+    if (a->base.element_type != type)
+        return vnec_fail("SpB_Matrix_import_csr_ext: the matrix was made with another value type");
     if (a->base.format_type != SpB_CSR)
         return vnec_fail("SpB_Matrix_import_csr_ext: the VNEC path needs CSR");
     SpB_Index nr = a->base.row, nc = a->base.col;
@@ -319,9 +339,21 @@ SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32
         }
         a->indices[j] = (SpB_Index)col[j];
     }
-    memcpy(a->val, val, sizeof(double) * (size_t)nnz);
+    if (nnz) memcpy(a->val, val, a->base.element_size * (size_t)nnz);
     for (SpB_Index r = 0; r <= nr; r++) a->ptr[r] = (SpB_Index)ptr[r];
     return SpB_SUCCESS;
+}
+
+SpB_Info SpB_Matrix_import_csr_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
+                                   const double *val)
+{
+    return import_csr(A, SpB_FP64, ptr, col, val);
+}
+
+SpB_Info SpB_Matrix_import_csr_fp32_ext(SpB_Matrix A, const int64_t *ptr, const int32_t *col,
+                                        const float *val)
+{
+    return import_csr(A, SpB_FP32, ptr, col, val);
 }
 
 /* Block-diagonal batch of nmat matrices as one CSR matrix: member m owns
@@ -357,7 +389,10 @@ SpB_Info vnec_batch_import_ext(SpB_Matrix A, int64_t nmat, const int64_t *row_of
     return SpB_SUCCESS;
 }
 
-/* Device mirror: int32 row offsets and columns, FP64 values. */
+/* Device mirror: int32 row offsets and columns, FP64 values.  An FP32
+ * matrix's values are widened exactly (FP32 -> FP64 loses nothing): the
+ * layout builders are shared by both value types, and FP32 plans narrow
+ * their own value arrays back to FP32, again exactly. */
 SpB_Info spb_matrix_upload(spb_csr *a)
 {
     if (a->dev_ready) return SpB_SUCCESS;
@@ -412,9 +447,19 @@ SpB_Info spb_matrix_upload(spb_csr *a)
     if (rc == SpB_SUCCESS && nnz)
         rc = vnec_hip_ok(hipMemcpyAsync(a->d_col, hc, sizeof(int32_t) * (size_t)nnz,
                                         hipMemcpyHostToDevice, s), "upload col");
-    if (rc == SpB_SUCCESS && nnz)
+    if (rc == SpB_SUCCESS && nnz && a->base.element_type == SpB_FP64)
         rc = vnec_hip_ok(hipMemcpyAsync(a->d_val, a->val, sizeof(double) * (size_t)nnz,
                                         hipMemcpyHostToDevice, s), "upload val");
+    if (rc == SpB_SUCCESS && nnz && a->base.element_type == SpB_FP32) {
+        float *tmp = (float *)vnec_malloc_dev(sizeof(float) * (size_t)nnz, NULL);
+        rc = tmp ? SpB_SUCCESS : SpB_FAILURE;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipMemcpyAsync(tmp, a->val, sizeof(float) * (size_t)nnz,
+                                            hipMemcpyHostToDevice, s), "upload val");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_widen_f32(tmp, nnz, a->d_val, s), "widen");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "upload sync");
+        vnec_free_dev(tmp);
+    }
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "upload sync");
     free(hp);
     free(hc);
@@ -440,19 +485,20 @@ spb_dense *spb_as_dense(const SpB_Vector v)
 SpB_Info SpB_Vector_new(SpB_Vector *v, SpB_Type type, SpB_Index n)
 {
     if (!v) return vnec_fail("SpB_Vector_new: NULL handle");
-    if (type != SpB_FP64) return vnec_fail("SpB_Vector_new: only SpB_FP64 is on the VNEC path");
+    if (type != SpB_FP64 && type != SpB_FP32)
+        return vnec_fail("SpB_Vector_new: the VNEC path takes SpB_FP64 or SpB_FP32 values");
     spb_dense *d = (spb_dense *)calloc(1, sizeof(spb_dense));
     if (!d) return vnec_fail("SpB_Vector_new: out of memory");
     d->base.element_type = type;
     d->base.format_type = SpB_DENSE;
     d->base.n = n;
-    d->base.element_size = sizeof(double);
-    d->values = aligned64(sizeof(double) * (size_t)n);
+    d->base.element_size = type == SpB_FP32 ? sizeof(float) : sizeof(double);
+    d->values = aligned64(d->base.element_size * (size_t)n);
     if (!d->values) {
         free(d);
         return vnec_fail("SpB_Vector_new: out of memory");
     }
-    memset(d->values, 0, sizeof(double) * (size_t)n);
+    memset(d->values, 0, d->base.element_size * (size_t)n);
     d->magic = SPB_MAGIC_VECTOR;
     d->dev_stale = 1;
     *v = (SpB_Vector)d;
@@ -484,7 +530,7 @@ SpB_Info spb_vector_device_alloc(spb_dense *v)
 {
     if (v->d_values) return SpB_SUCCESS;
     if (!vnec_runtime_ready()) return SpB_FAILURE;
-    v->d_values = (double *)vnec_malloc_dev(sizeof(double) * (size_t)v->base.n, NULL);
+    v->d_values = vnec_malloc_dev(v->base.element_size * (size_t)v->base.n, NULL);
     if (!v->d_values) return SpB_FAILURE;
     v->dev_stale = 1;
     return SpB_SUCCESS;
@@ -496,7 +542,7 @@ SpB_Info spb_vector_to_device(spb_dense *v)
     if (!v->resident || v->dev_stale) {
         if (v->host_stale) return SpB_SUCCESS; /* device already newer */
         SpB_Info rc = vnec_hip_ok(hipMemcpyAsync(v->d_values, v->values,
-                                                 sizeof(double) * (size_t)v->base.n,
+                                                 v->base.element_size * (size_t)v->base.n,
                                                  hipMemcpyHostToDevice, vnec_stream()),
                                   "vector upload");
         if (rc != SpB_SUCCESS) return rc;
@@ -509,7 +555,7 @@ SpB_Info spb_vector_from_device(spb_dense *v)
 {
     if (!v->host_stale) return SpB_SUCCESS;
     SpB_Info rc = vnec_hip_ok(hipMemcpyAsync(v->values, v->d_values,
-                                             sizeof(double) * (size_t)v->base.n,
+                                             v->base.element_size * (size_t)v->base.n,
                                              hipMemcpyDeviceToHost, vnec_stream()),
                               "vector download");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "download sync");
@@ -517,9 +563,20 @@ SpB_Info spb_vector_from_device(spb_dense *v)
     return rc;
 }
 
+static spb_dense *typed_vector(const SpB_Vector v, SpB_Type type, const char *fn)
+{
+    spb_dense *d = spb_as_dense(v);
+    if (!d) return NULL;
+    if (d->base.element_type != type) {
+        vnec_fail("%s: the vector holds %s values", fn, d->base.element_type == SpB_FP32 ? "FP32" : "FP64");
+        return NULL;
+    }
+    return d;
+}
+
 SpB_Info SpB_Vector_setElement_FP64(SpB_Vector w, double x, SpB_Index i)
 {
-    spb_dense *d = spb_as_dense(w);
+    spb_dense *d = typed_vector(w, SpB_FP64, "SpB_Vector_setElement_FP64");
     if (!d) return SpB_FAILURE;
     if (i >= d->base.n) return vnec_fail("SpB_Vector_setElement_FP64: index %lu >= %lu", i, d->base.n);
     if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
@@ -528,21 +585,40 @@ SpB_Info SpB_Vector_setElement_FP64(SpB_Vector w, double x, SpB_Index i)
     return SpB_SUCCESS;
 }
 
+SpB_Info SpB_Vector_setElement_FP32(SpB_Vector w, float x, SpB_Index i)
+{
+    spb_dense *d = typed_vector(w, SpB_FP32, "SpB_Vector_setElement_FP32");
+    if (!d) return SpB_FAILURE;
+    if (i >= d->base.n) return vnec_fail("SpB_Vector_setElement_FP32: index %lu >= %lu", i, d->base.n);
+    if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
+    ((float *)d->values)[i] = x;
+    d->dev_stale = 1;
+    return SpB_SUCCESS;
+}
+
 SpB_Info SpB_Vector_extractElement_FP64(double *x, const SpB_Vector v, SpB_Index i)
 {
-    spb_dense *d = spb_as_dense(v);
+    spb_dense *d = typed_vector(v, SpB_FP64, "SpB_Vector_extractElement_FP64");
     if (!d || !x) return SpB_FAILURE;
-    if (d->base.element_type != SpB_FP64)
-        return vnec_fail("SpB_Vector_extractElement_FP64: vector is not FP64");
     if (i >= d->base.n) return vnec_fail("SpB_Vector_extractElement_FP64: index out of range");
     if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
     *x = ((double *)d->values)[i];
     return SpB_SUCCESS;
 }
 
+SpB_Info SpB_Vector_extractElement_FP32(float *x, const SpB_Vector v, SpB_Index i)
+{
+    spb_dense *d = typed_vector(v, SpB_FP32, "SpB_Vector_extractElement_FP32");
+    if (!d || !x) return SpB_FAILURE;
+    if (i >= d->base.n) return vnec_fail("SpB_Vector_extractElement_FP32: index out of range");
+    if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
+    *x = ((float *)d->values)[i];
+    return SpB_SUCCESS;
+}
+
 SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values)
 {
-    spb_dense *d = spb_as_dense(v);
+    spb_dense *d = typed_vector(v, SpB_FP64, "SpB_Vector_import_ext");
     if (!d || !values) return SpB_FAILURE;
     memcpy(d->values, values, sizeof(double) * (size_t)d->base.n);
     d->host_stale = 0;
@@ -552,10 +628,29 @@ SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values)
 
 SpB_Info SpB_Vector_export_ext(const SpB_Vector v, double *values)
 {
-    spb_dense *d = spb_as_dense(v);
+    spb_dense *d = typed_vector(v, SpB_FP64, "SpB_Vector_export_ext");
     if (!d || !values) return SpB_FAILURE;
     if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
     memcpy(values, d->values, sizeof(double) * (size_t)d->base.n);
+    return SpB_SUCCESS;
+}
+
+SpB_Info SpB_Vector_import_fp32_ext(SpB_Vector v, const float *values)
+{
+    spb_dense *d = typed_vector(v, SpB_FP32, "SpB_Vector_import_fp32_ext");
+    if (!d || !values) return SpB_FAILURE;
+    memcpy(d->values, values, sizeof(float) * (size_t)d->base.n);
+    d->host_stale = 0;
+    d->dev_stale = 1;
+    return SpB_SUCCESS;
+}
+
+SpB_Info SpB_Vector_export_fp32_ext(const SpB_Vector v, float *values)
+{
+    spb_dense *d = typed_vector(v, SpB_FP32, "SpB_Vector_export_fp32_ext");
+    if (!d || !values) return SpB_FAILURE;
+    if (d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;
+    memcpy(values, d->values, sizeof(float) * (size_t)d->base.n);
     return SpB_SUCCESS;
 }
 
@@ -576,7 +671,7 @@ SpB_Info SpB_Vector_sync_host_ext(SpB_Vector v)
     return spb_vector_from_device(d);
 }
 
-double *SpB_Vector_device_ptr_ext(SpB_Vector v)
+void *SpB_Vector_device_ptr_ext(SpB_Vector v)
 {
     spb_dense *d = spb_as_dense(v);
     if (!d) return NULL;

@@ -39,7 +39,7 @@ struct SpB_Vector_opaque {
 
 typedef struct spb_csr {
     struct SpB_Matrix_opaque base;
-    void *val;          /* nnz FP64 values (host)            */
+    void *val;          /* nnz values (host), FP64 or FP32   */
     SpB_Index *indices; /* nnz column indices (host, 64-bit) */
     SpB_Index *ptr;     /* ptr_len row offsets (host)        */
     SpB_Index ptr_len;
@@ -50,17 +50,17 @@ typedef struct spb_csr {
     int dev_ready;      /* device mirror matches the host arrays */
     int32_t *d_ptr;     /* nrows + 1 */
     int32_t *d_col;     /* nnz       */
-    double *d_val;      /* nnz       */
+    double *d_val;      /* nnz, FP64 (FP32 values widened)   */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
     int tuned_cand;     /* autotuner's best candidate, -1 until measured */
 } spb_csr;
 
 typedef struct spb_dense {
     struct SpB_Vector_opaque base;
-    void *values; /* n FP64 (host) */
+    void *values; /* n values (host), FP64 or FP32 */
     /* ---- private ---- */
     uint32_t magic;
-    double *d_values;
+    void *d_values; /* n values (device), same type */
     int resident;   /* device copy is authoritative between calls      */
     int host_stale; /* device holds newer values than host             */
     int dev_stale;  /* host holds newer values than device             */
@@ -94,6 +94,7 @@ struct vnec_plan_s {
     int32_t *vrp;       /* nrows + 1 */
     int32_t *col_start; /* groups    */
     double *dvals;      /* groups * lanes, wave64 layout */
+    float *dvals32;     /* the same in FP32 (encoding FP32) */
     int64_t groups;
     /* variant H: hot x entries */
     int32_t *hcol;      /* nnz: column, or -1 - slot for a hot column */
@@ -107,16 +108,17 @@ struct vnec_plan_s {
     int64_t dev_bytes;
     /* cached graph for timing */
     hipGraphExec_t gexec;
-    const double *g_x;
-    double *g_y;
+    const void *g_x;
+    void *g_y;
     int g_reps;
 };
 
 /* ---- library runtime (spb_core.c) ---- */
 hipStream_t vnec_stream(void);
 SpB_Info vnec_fail(const char *fmt, ...);
-/* Entry points that return void (the reference's SpMV_VNEC_*) clear the
- * error slot first, so vnec_last_error() afterwards describes that call. */
+/* Entry points without a status return (the reference's SpMV_VNEC_*,
+ * IRD_VNEC) clear the error slot first, so vnec_last_error() afterwards
+ * describes that call. */
 void vnec_clear_error(void);
 SpB_Info vnec_hip_ok(int rc, const char *what);
 int vnec_runtime_ready(void);
@@ -130,7 +132,8 @@ SpB_Info spb_vector_from_device(spb_dense *v);
 SpB_Info spb_vector_device_alloc(spb_dense *v);
 
 /* ---- plans (spb_plan.c) ---- */
-SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev);
+/* x_dev / y_dev: doubles, or floats for an encoding-FP32 plan. */
+SpB_Info vnec_plan_run(struct vnec_plan_s *p, const void *x_dev, void *y_dev);
 double vnec_slice_padding(const spb_csr *a);
 
 /* ---- autotuner and selector (spb_tune.c) ---- */

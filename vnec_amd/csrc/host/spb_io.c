@@ -175,25 +175,34 @@ SpB_Info vnec_read_graph_ext(const char *path, int64_t *nrows, int64_t **ptr, in
 
 void vnec_host_free_ext(void *p) { free(p); }
 
-SpB_Info SpB_Matrix_read_ext(SpB_Matrix *A, const char *path, int values)
+SpB_Info SpB_Matrix_read_ext(SpB_Matrix *A, const char *path, SpB_Type type, int values)
 {
     if (!A) return vnec_fail("SpB_Matrix_read_ext: NULL handle");
+    *A = NULL;
     if (values != VNEC_VALUES_ROW && values != VNEC_VALUES_ONE)
         return vnec_fail("SpB_Matrix_read_ext: values must be VNEC_VALUES_ROW or VNEC_VALUES_ONE");
+    if (type != SpB_FP64 && type != SpB_FP32)
+        return vnec_fail("SpB_Matrix_read_ext: type must be SpB_FP64 or SpB_FP32");
     int64_t n = 0, *ptr = NULL;
     int32_t *col = NULL;
     if (vnec_read_graph_ext(path, &n, &ptr, &col) != SpB_SUCCESS) return SpB_FAILURE;
     int64_t nnz = ptr[n];
-    double *val = (double *)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    size_t esz = type == SpB_FP32 ? sizeof(float) : sizeof(double);
+    void *val = malloc(esz * (size_t)(nnz ? nnz : 1));
     SpB_Info rc = val ? SpB_SUCCESS : vnec_fail("SpB_Matrix_read_ext: out of memory");
     for (int64_t r = 0; rc == SpB_SUCCESS && r < n; r++)
-        for (int64_t j = ptr[r]; j < ptr[r + 1]; j++) val[j] = values == VNEC_VALUES_ROW ? (double)r : 1.0;
+        for (int64_t j = ptr[r]; j < ptr[r + 1]; j++) {
+            double v = values == VNEC_VALUES_ROW ? (double)r : 1.0;
+            if (type == SpB_FP32) ((float *)val)[j] = (float)v;
+            else ((double *)val)[j] = v;
+        }
     SpB_Object_Format_ext fmt;
     fmt.matrix_format = SpB_CSR;
     if (rc == SpB_SUCCESS) rc = SpB_Global_Option_set_ext(SpB_MATRIX_FORMAT, fmt);
-    *A = NULL;
-    if (rc == SpB_SUCCESS) rc = SpB_Matrix_new(A, SpB_FP64, (SpB_Index)n, (SpB_Index)n);
-    if (rc == SpB_SUCCESS) rc = SpB_Matrix_import_csr_ext(*A, ptr, col, val);
+    if (rc == SpB_SUCCESS) rc = SpB_Matrix_new(A, type, (SpB_Index)n, (SpB_Index)n);
+    if (rc == SpB_SUCCESS)
+        rc = type == SpB_FP32 ? SpB_Matrix_import_csr_fp32_ext(*A, ptr, col, (const float *)val)
+                              : SpB_Matrix_import_csr_ext(*A, ptr, col, (const double *)val);
     if (rc != SpB_SUCCESS && *A) SpB_Matrix_free(A);
     free(ptr);
     free(col);

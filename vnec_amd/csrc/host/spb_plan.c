@@ -38,6 +38,7 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->vrp);
     vnec_free_dev(p->col_start);
     vnec_free_dev(p->dvals);
+    vnec_free_dev(p->dvals32);
     vnec_free_dev(p->hcol);
     vnec_free_dev(p->hot_order);
     vnec_free_dev(p->xhot);
@@ -245,21 +246,27 @@ static SpB_Info build_d(struct vnec_plan_s *p, const spb_csr *a)
     if (rc == SpB_SUCCESS && G * L >= INT32_MAX) rc = vnec_fail("variant D: %lld groups too many", (long long)G);
     double *va = NULL;
     int64_t nblk = (G + 63) / 64;
+    const int fp32 = p->tune.encoding == VNEC_ENC_FP32;
+    const size_t esz = fp32 ? sizeof(float) : sizeof(double), nslot = (size_t)(nblk ? nblk * 64 * L : 1);
     if (rc == SpB_SUCCESS) {
         p->groups = G;
         p->col_start = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(G ? G : 1), &p->dev_bytes);
-        p->dvals = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nblk ? nblk * 64 * L : 1), &p->dev_bytes);
+        if (fp32) p->dvals32 = (float *)vnec_malloc_dev(sizeof(float) * nslot, &p->dev_bytes);
+        else p->dvals = (double *)vnec_malloc_dev(sizeof(double) * nslot, &p->dev_bytes);
         va = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(G ? G * L : 1), NULL);
-        if (!p->col_start || !p->dvals || !va) rc = SpB_FAILURE;
+        if (!p->col_start || !(fp32 ? (void *)p->dvals32 : (void *)p->dvals) || !va) rc = SpB_FAILURE;
     }
     if (rc == SpB_SUCCESS && G)
         rc = vnec_hip_ok(hipMemsetAsync(va, 0, sizeof(double) * (size_t)(G * L), s), "memset");
     if (rc == SpB_SUCCESS && nblk)
-        rc = vnec_hip_ok(hipMemsetAsync(p->dvals, 0, sizeof(double) * (size_t)(nblk * 64 * L), s), "memset");
+        rc = vnec_hip_ok(hipMemsetAsync(fp32 ? (void *)p->dvals32 : (void *)p->dvals, 0, esz * nslot, s),
+                         "memset");
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(vnec_hip_d_fill(a->d_ptr, ecr, a->d_val, nr, p->coords, P, L, p->vrp,
                                          p->col_start, va, 1, s), "d fill");
-    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_d_interleave(va, G, L, p->dvals, s), "d interleave");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(fp32 ? vnec_hip_d_interleave_f32(va, G, L, p->dvals32, s)
+                              : vnec_hip_d_interleave(va, G, L, p->dvals, s), "d interleave");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
     vnec_free_dev(va);
     vnec_free_dev(ecr);
@@ -318,7 +325,18 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (!a) return SpB_FAILURE;
     if (!vnec_runtime_ready()) return SpB_FAILURE;
     vnec_tuning tune;
-    if (t && t->variant >= 0) {
+    const int fp32 = a->base.element_type == SpB_FP32;
+    if (fp32 && (!t || t->variant < 0)) {
+        /* FP32 matrices: the 64-row slice layout when its padding is small,
+         * else merge-path tiles of 7 items per thread (the selector's
+         * measurements are of the FP64 kernels) */
+        vnec_tuning f = {VNEC_VAR_L, 7, 64, VNEC_ENC_FP32, 1, -1};
+        if (vnec_slice_padding(a) <= 1.15) {
+            f.variant = VNEC_VAR_S;
+            f.ipt = 3;
+        }
+        tune = f;
+    } else if (t && t->variant >= 0) {
         tune = *t;
     } else {
         int nrhs = t && t->nrhs > 0 ? t->nrhs : 1;
@@ -335,6 +353,15 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (tune.variant < 0 || tune.variant >= VNEC_VAR_N)
         return vnec_fail("variant %d is not available", tune.variant);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
+    if (fp32 != (tune.encoding == VNEC_ENC_FP32))
+        return vnec_fail(fp32 ? "an FP32 matrix takes encoding VNEC_ENC_FP32"
+                              : "encoding VNEC_ENC_FP32 needs a matrix made with SpB_FP32");
+    if (fp32) {
+        if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_S && tune.variant != VNEC_VAR_D)
+            return vnec_fail("encoding VNEC_ENC_FP32: variants L, S and D");
+        if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
+        if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
+    }
     if (tune.variant == VNEC_VAR_H) {
         if (!vnec_hip_hot_supported(tune.ipt, tune.lanes))
             return vnec_fail("variant H: ipt 5 or 7 and lanes (hot entries) 4096 or 8192");
@@ -344,7 +371,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     } else if (tune.variant == VNEC_VAR_D) {
         if (!vnec_hip_d_supported(tune.ipt, tune.lanes))
             return vnec_fail("variant D: ipt in {5, 7, 9} and lanes in {4, 8, 16}");
-        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant D stores FP64 values only");
+        if (tune.encoding != VNEC_ENC_FP64 && tune.encoding != VNEC_ENC_FP32)
+            return vnec_fail("variant D stores FP64 values (or FP32 for FP32 matrices)");
         if (tune.nrhs != 1) return vnec_fail("variant D takes one right-hand side");
         tune.grid = -1;
     } else if (tune.variant == VNEC_VAR_S) {
@@ -450,7 +478,7 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
     return SpB_SUCCESS;
 }
 
-static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *y_dev,
+static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_dev,
                       vnec_merge_args *out)
 {
     vnec_merge_args m;
@@ -467,9 +495,15 @@ static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *
     m.run_start = p->run_start;
     m.run_wpos = p->run_wpos;
     m.nec = p->nec;
-    m.x = x_dev;
-    m.y = y_dev;
-    m.carry = p->carry;
+    if (p->tune.encoding == VNEC_ENC_FP32) {
+        m.x32 = (const float *)x_dev;
+        m.y32 = (float *)y_dev;
+        m.carry32 = (float *)p->carry; /* ntiles doubles hold ntiles floats */
+    } else {
+        m.x = (const double *)x_dev;
+        m.y = (double *)y_dev;
+        m.carry = p->carry;
+    }
     m.nrows = p->nrows;
     m.ipt = p->tune.ipt;
     m.variant = p->tune.variant;
@@ -483,7 +517,7 @@ static void fill_args(const struct vnec_plan_s *p, const double *x_dev, double *
     *out = m;
 }
 
-static SpB_Info launch_slice(struct vnec_plan_s *p, const double *x_dev, double *y_dev, hipStream_t s)
+static SpB_Info launch_slice(struct vnec_plan_s *p, const void *x_dev, void *y_dev, hipStream_t s)
 {
     vnec_slice_args a;
     memset(&a, 0, sizeof a);
@@ -503,15 +537,20 @@ static SpB_Info launch_slice(struct vnec_plan_s *p, const double *x_dev, double 
     a.nec = p->nec;
     a.max_nec = p->max_nec;
     a.win = p->win;
-    a.x = x_dev;
-    a.y = y_dev;
+    if (p->tune.encoding == VNEC_ENC_FP32) {
+        a.x32 = (const float *)x_dev;
+        a.y32 = (float *)y_dev;
+    } else {
+        a.x = (const double *)x_dev;
+        a.y = (double *)y_dev;
+    }
     a.nrows = p->nrows;
     a.encoding = p->tune.encoding;
     return vnec_hip_ok(vnec_hip_spmv_slice(&a, s), "slice kernel");
 }
 
 /* Launch the SpMV kernel only (its carries stay in the plan's scratch). */
-static SpB_Info launch_d(struct vnec_plan_s *p, const double *x_dev, double *y_dev, hipStream_t s)
+static SpB_Info launch_d(struct vnec_plan_s *p, const void *x_dev, void *y_dev, hipStream_t s)
 {
     vnec_d_args a;
     memset(&a, 0, sizeof a);
@@ -527,19 +566,29 @@ static SpB_Info launch_d(struct vnec_plan_s *p, const double *x_dev, double *y_d
     a.run_start = p->run_start;
     a.run_wpos = p->run_wpos;
     a.nec = p->nec;
-    a.x = x_dev;
-    a.y = y_dev;
-    a.carry = p->carry;
+    if (p->tune.encoding == VNEC_ENC_FP32) {
+        a.fp32 = 1;
+        a.vals32 = p->dvals32;
+        a.val32 = p->hi;
+        a.x32 = (const float *)x_dev;
+        a.y32 = (float *)y_dev;
+        a.carry32 = (float *)p->carry;
+    } else {
+        a.x = (const double *)x_dev;
+        a.y = (double *)y_dev;
+        a.carry = p->carry;
+    }
     a.nrows = p->nrows;
     a.ipt = p->tune.ipt;
     a.lanes = p->tune.lanes;
     return vnec_hip_ok(vnec_hip_spmv_d(&a, s), "d kernel");
 }
 
-static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hipStream_t s)
+static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, const void *x_dev,
+                            void *y_dev, hipStream_t s)
 {
-    if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, m->x, m->y, s);
-    if (p->tune.variant == VNEC_VAR_D) return launch_d(p, m->x, m->y, s);
+    if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, x_dev, y_dev, s);
+    if (p->tune.variant == VNEC_VAR_D) return launch_d(p, x_dev, y_dev, s);
     if (p->tune.variant == VNEC_VAR_H) {
         if (vnec_hip_ok(vnec_hip_gather_hot(m->x, p->hot_order, p->nhot, p->xhot, s), "hot gather"))
             return SpB_FAILURE;
@@ -551,9 +600,12 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, hip
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
 }
 
-static SpB_Info launch_fixup(struct vnec_plan_s *p, double *y_dev, hipStream_t s)
+static SpB_Info launch_fixup(struct vnec_plan_s *p, void *y_dev, hipStream_t s)
 {
     if (p->tune.variant == VNEC_VAR_S) return SpB_SUCCESS; /* row-aligned tiles: no carries */
+    if (p->tune.encoding == VNEC_ENC_FP32)
+        return vnec_hip_ok(vnec_hip_carry_fixup_f32(p->coords, (const float *)p->carry, p->ntiles,
+                                                    (float *)y_dev, p->nrows, s), "carry fix-up");
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_wg_fixup(p->wg_row, p->wg_carry, p->nwg, y_dev, p->nrows, s),
                            "carry fix-up");
@@ -562,19 +614,21 @@ static SpB_Info launch_fixup(struct vnec_plan_s *p, double *y_dev, hipStream_t s
 }
 
 /* Enqueue one SpMV (kernel + fix-up) on the library stream. */
-SpB_Info vnec_plan_run(struct vnec_plan_s *p, const double *x_dev, double *y_dev)
+SpB_Info vnec_plan_run(struct vnec_plan_s *p, const void *x_dev, void *y_dev)
 {
     hipStream_t s = vnec_stream();
     if (p->ntiles == 0) {
-        return vnec_hip_ok(vnec_hip_zero_f64(y_dev, p->nrows * p->tune.nrhs, s), "zero y");
+        if (p->tune.encoding == VNEC_ENC_FP32)
+            return vnec_hip_ok(vnec_hip_zero_f32((float *)y_dev, p->nrows, s), "zero y");
+        return vnec_hip_ok(vnec_hip_zero_f64((double *)y_dev, p->nrows * p->tune.nrhs, s), "zero y");
     }
     vnec_merge_args m;
     fill_args(p, x_dev, y_dev, &m);
-    if (launch_main(p, &m, s) != SpB_SUCCESS) return SpB_FAILURE;
+    if (launch_main(p, &m, x_dev, y_dev, s) != SpB_SUCCESS) return SpB_FAILURE;
     return launch_fixup(p, y_dev, s);
 }
 
-SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev)
+SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev)
 {
     if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmv_dev_ext: not a plan");
     return vnec_plan_run(plan, x_dev, y_dev);
@@ -586,6 +640,10 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)
     spb_dense *xv = spb_as_dense(x), *yv = spb_as_dense(y);
     if (!xv || !yv) return SpB_FAILURE;
     int k = plan->tune.nrhs;
+    SpB_Type vt = plan->tune.encoding == VNEC_ENC_FP32 ? SpB_FP32 : SpB_FP64;
+    if (xv->base.element_type != vt || yv->base.element_type != vt)
+        return vnec_fail("vnec_spmv_ext: the plan computes in %s; x and y must be %s vectors",
+                         vt == SpB_FP32 ? "FP32" : "FP64", vt == SpB_FP32 ? "SpB_FP32" : "SpB_FP64");
     if ((int64_t)xv->base.n != plan->ncols * k || (int64_t)yv->base.n != plan->nrows * k)
         return vnec_fail("vnec_spmv_ext: x has %lu entries (need %ld), y %lu (need %ld)",
                          xv->base.n, (long)(plan->ncols * k), yv->base.n, (long)(plan->nrows * k));
@@ -602,7 +660,7 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)
 static void *g_flush = NULL;
 static size_t g_flush_bytes = 0;
 
-SpB_Info vnec_time_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+SpB_Info vnec_time_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                            int use_graph, size_t flush_bytes, double *ms)
 {
     if (!plan || plan->magic != SPB_MAGIC_PLAN || iters < 1 || !ms)
@@ -682,7 +740,7 @@ SpB_Info vnec_time_dev_ext(vnec_plan plan, const double *x_dev, double *y_dev, i
 
 /* Mean duration of the SpMV kernel and of the fix-up kernel, each between
  * its own pair of events on the library stream (iters SpMVs). */
-SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_dev, int iters,
+SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                                double *ms_main, double *ms_fixup)
 {
     if (!plan || plan->magic != SPB_MAGIC_PLAN || iters < 1)
@@ -703,7 +761,7 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const double *x_dev, double *y_de
     for (int i = 0; rc == SpB_SUCCESS && i < iters; i++) {
         float a = 0.f, b = 0.f;
         rc = vnec_hip_ok(hipEventRecord(ev[0], s), "record");
-        if (rc == SpB_SUCCESS) rc = launch_main(plan, &m, s);
+        if (rc == SpB_SUCCESS) rc = launch_main(plan, &m, x_dev, y_dev, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[1], s), "record");
         if (rc == SpB_SUCCESS) rc = launch_fixup(plan, y_dev, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[2], s), "record");

@@ -201,6 +201,8 @@ SpB_Info vnec_autotune_ext(const SpB_Matrix A, int reps, double times_ms[VNEC_NC
 {
     spb_csr *a = spb_as_csr(A);
     if (!a || !times_ms || reps < 1) return vnec_fail("vnec_autotune_ext: bad arguments");
+    if (a->base.element_type != SpB_FP64)
+        return vnec_fail("vnec_autotune_ext: the candidates are FP64 kernels; A must hold FP64 values");
     uint64_t ints[VNEC_NFEAT_INTS];
     if (vnec_feature_ints_ext(A, ints) != SpB_SUCCESS) return SpB_FAILURE;
     uint32_t mask = vnec_candidate_mask_from_ints_ext(ints);

@@ -1,14 +1,17 @@
 /*
  * spb_vnec.c — the reference's VNEC entry points (IRD_VNEC, Build_VNEC_*,
- * SpMV_VNEC_*_FP64) and SpB_mxv, on the GPU.
+ * SpMV_VNEC_*_FP64 and their FP32 twins) and SpB_mxv, on the GPU.
  *
  * Each Build_VNEC_* call returns the reference's public plan struct with
  * its host fields filled at the reference's compiled parameters (P = 36
- * partitions, L = 8 lanes, src/operation/include/SpB_mxv.h:26-33), computed
- * by the GPU builders, and keeps behind it a GPU execution plan laid out
- * for wave64 tiles.  SpMV_VNEC_* runs that GPU plan; y is valid on return.
- * The D plan's val_align stays NULL: the device plan holds the values, and
- * the reference's val_align would take 64 bytes per nonzero of host memory.
+ * partitions, L = 8 lanes for FP64 and 16 for FP32,
+ * src/operation/include/SpB_mxv.h:26-33), computed by the GPU builders,
+ * and keeps behind it a GPU execution plan laid out for wave64 tiles.
+ * SpMV_VNEC_* runs that GPU plan; y is valid on return.  The FP32 entry
+ * points compute as the reference's FP32 kernels do: FP32 values, x, y and
+ * sums.  The D plans' val_align stays NULL: the device plan holds the
+ * values, and the reference's val_align would take L values per nonzero
+ * group of host memory.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -17,10 +20,12 @@
 
 #define REF_P 36     /* NUM_THREADS, SpB_mxv.h:28 */
 #define REF_L 8      /* nLanes_f64,  SpB_mxv.h:30 */
-#define REF_IRD_THR 0.477 /* IRD_thr_fp64, SpB_mxv.h:32 */
+#define REF_L32 16   /* nLanes_f32,  SpB_mxv.h:29 */
+#define REF_IRD_THR 0.477   /* IRD_thr_fp64, SpB_mxv.h:32 */
+#define REF_IRD_THR32 0.437 /* IRD_thr_fp32, SpB_mxv.h:31 */
 #define REF_CROSS_MAX 32  /* longest row sent to the cross-row kernel, spmv_vnec_S_for_mxv.c:489 */
 
-typedef enum { KIND_L = 1, KIND_D = 2, KIND_S = 3 } plan_kind;
+typedef enum { KIND_L = 1, KIND_D = 2, KIND_S = 3, KIND_D32 = 4, KIND_S32 = 5 } plan_kind;
 
 /* Registry of public plans handed out, so SpB_mxv can tell them apart. */
 typedef struct wrap {
@@ -375,31 +380,37 @@ SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *i
 SpB_VNEC_type IRD_VNEC(const SpB_Matrix A, SpB_Type type)
 {
     int dec = SpB_VNEC_S;
-    if (type != SpB_FP64) {
-        vnec_fail("IRD_VNEC: only SpB_FP64 is on this path");
+    vnec_clear_error(); /* no status return: the error slot tells */
+    if (type != SpB_FP64 && type != SpB_FP32) {
+        vnec_fail("IRD_VNEC: type must be SpB_FP64 or SpB_FP32");
         return SpB_VNEC_S;
     }
-    if (vnec_ird_ext(A, REF_P, REF_L, REF_IRD_THR, NULL, &dec) != SpB_SUCCESS) return SpB_VNEC_S;
+    int L = type == SpB_FP32 ? REF_L32 : REF_L;
+    double thr = type == SpB_FP32 ? REF_IRD_THR32 : REF_IRD_THR;
+    if (vnec_ird_ext(A, REF_P, L, thr, NULL, &dec) != SpB_SUCCESS) return SpB_VNEC_S;
     return (SpB_VNEC_type)dec;
 }
 
 /* ------------------------------------------------------ public plans */
 /* The device plan behind a Build_VNEC_* result runs the matching GPU
  * kernel: VNEC-L the merge-path CSR kernel (7 items per thread,
- * persistent), VNEC-D the wave64 D-block kernel (groups of 8 window
- * positions, the reference's nLanes), VNEC-S the 64-row slice kernel when
- * the slice layout applies (padding <= 3) and the learned selector's
- * choice otherwise. */
+ * persistent for FP64, one workgroup per tile for FP32), VNEC-D the wave64
+ * D-block kernel (groups of the reference's nLanes window positions: 8 for
+ * FP64, 16 for FP32), VNEC-S the 64-row slice kernel when the slice layout
+ * applies (padding <= 3) and otherwise the learned selector's choice
+ * (FP64) or the merge-path kernel (FP32). */
 static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
 {
     vnec_plan p = NULL;
-    vnec_tuning t = {VNEC_SELECT_LEARNED, 7, 64, VNEC_ENC_FP64, 1, -1};
+    const int fp32 = a->base.element_type == SpB_FP32;
+    vnec_tuning t = {VNEC_SELECT_LEARNED, 7, 64, fp32 ? VNEC_ENC_FP32 : VNEC_ENC_FP64, 1, -1};
+    if (fp32) t.variant = VNEC_VAR_L; /* grid -1: one workgroup per tile */
     if (kind == KIND_L) {
         t.variant = VNEC_VAR_L;
-        t.grid = 0;
-    } else if (kind == KIND_D) {
+        t.grid = fp32 ? -1 : 0;
+    } else if (kind == KIND_D || kind == KIND_D32) {
         t.variant = VNEC_VAR_D;
-        t.lanes = REF_L;
+        t.lanes = fp32 ? REF_L32 : REF_L;
     } else {
         uint32_t mask = 0;
         if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return NULL;
@@ -416,7 +427,7 @@ static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
     return p;
 }
 
-/* spmv_vnec_L_for_mxv.c:2-35 */
+/* spmv_vnec_L_for_mxv.c:2-35 (one plan type for both value types) */
 struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
 {
     spb_csr *a = spb_as_csr(A);
@@ -443,6 +454,13 @@ struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
     l->thread_coord_start = (coord *)malloc(sizeof(coord) * REF_P);
     l->thread_coord_end = (coord *)malloc(sizeof(coord) * REF_P);
     l->nz_indices = (int *)malloc(sizeof(int) * (size_t)(nnz ? nnz : 1));
+    if (!l->diagonal_start || !l->diagonal_end || !l->thread_coord_start || !l->thread_coord_end ||
+        !l->nz_indices) {
+        free(co);
+        VNEC_L_free_ext(l);
+        vnec_fail("Build_VNEC_L: out of memory");
+        return NULL;
+    }
     for (int t = 0; t < REF_P; t++) {
         int64_t ds = ipt * t < M ? ipt * t : M;
         l->diagonal_start[t] = (int)ds;
@@ -463,130 +481,240 @@ struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
     return l;
 }
 
-/* spmv_vnec_D_for_mxv.c:242-398, host fields at P = 36, L = 8 */
-struct VNEC_D_FP64 *Build_VNEC_D_FP64(const SpB_Matrix A)
+/* Host fields the D plans of both value types share (all but the compact
+ * x buffers ecr_xx_val, whose element type differs). */
+typedef struct {
+    coord *tcs, *tce;
+    int *ecr_indices, *NEC_NUM, **use_x_indices, *v_row_ptr, *col_start;
+} d_fields;
+
+static void d_fields_free(d_fields *f)
 {
-    spb_csr *a = spb_as_csr(A);
-    if (!a) return NULL;
+    for (int t = 0; f->use_x_indices && t < REF_P; t++) free(f->use_x_indices[t]);
+    free(f->tcs);
+    free(f->tce);
+    free(f->ecr_indices);
+    free(f->NEC_NUM);
+    free(f->use_x_indices);
+    free(f->v_row_ptr);
+    free(f->col_start);
+    memset(f, 0, sizeof *f);
+}
+
+/* spmv_vnec_D_for_mxv.c:242-398 (FP64) and :2-241 (FP32): partitions,
+ * ECR and the group layout at P = 36 and L lanes, computed on the GPU. */
+static SpB_Info d_public(spb_csr *a, int L, const char *fn, d_fields *f)
+{
+    memset(f, 0, sizeof *f);
     int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
     dlay d;
-    if (d_layout_device(a, REF_P, REF_L, 0, &d) != SpB_SUCCESS) return NULL;
-    VNEC_D_FP64 *p = (VNEC_D_FP64 *)calloc(1, sizeof(VNEC_D_FP64));
+    if (d_layout_device(a, REF_P, L, 0, &d) != SpB_SUCCESS) return SpB_FAILURE;
     int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
     int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (REF_P + 1));
     int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
     hipStream_t s = vnec_stream();
-    SpB_Info rc = (p && co && off && win) ? SpB_SUCCESS : vnec_fail("Build_VNEC_D_FP64: out of memory");
-    if (rc == SpB_SUCCESS) {
-        p->thread_coord_start = (coord *)malloc(sizeof(coord) * REF_P);
-        p->thread_coord_end = (coord *)malloc(sizeof(coord) * REF_P);
-        p->ecr_indices = (int *)calloc((size_t)nnz + 10, sizeof(int));
-        p->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
-        p->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
-        p->ecr_xx_val = (double **)calloc(REF_P + 1, sizeof(double *));
-        p->v_row_ptr = (int *)malloc(sizeof(int) * (size_t)(nr + 1));
-        p->col_start = (int *)malloc(sizeof(int) * (size_t)(d.G ? d.G : 1));
-        if (!p->thread_coord_start || !p->thread_coord_end || !p->ecr_indices || !p->NEC_NUM ||
-            !p->use_x_indices || !p->ecr_xx_val || !p->v_row_ptr || !p->col_start)
-            rc = vnec_fail("Build_VNEC_D_FP64: out of memory");
-    }
+    f->tcs = (coord *)malloc(sizeof(coord) * REF_P);
+    f->tce = (coord *)malloc(sizeof(coord) * REF_P);
+    f->ecr_indices = (int *)calloc((size_t)nnz + 10, sizeof(int));
+    f->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
+    f->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
+    f->v_row_ptr = (int *)malloc(sizeof(int) * (size_t)(nr + 1));
+    f->col_start = (int *)malloc(sizeof(int) * (size_t)(d.G ? d.G : 1));
+    SpB_Info rc = (co && off && win && f->tcs && f->tce && f->ecr_indices && f->NEC_NUM &&
+                   f->use_x_indices && f->v_row_ptr && f->col_start)
+                      ? SpB_SUCCESS
+                      : vnec_fail("%s: out of memory", fn);
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(hipMemcpyAsync(co, d.coords, sizeof(int32_t) * 2 * (REF_P + 1), hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(hipMemcpyAsync(off, d.win_off, sizeof(int64_t) * (REF_P + 1), hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS && nnz)
-        rc = vnec_hip_ok(hipMemcpyAsync(p->ecr_indices, d.ecr, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
+        rc = vnec_hip_ok(hipMemcpyAsync(f->ecr_indices, d.ecr, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS)
-        rc = vnec_hip_ok(hipMemcpyAsync(p->v_row_ptr, d.vrp, sizeof(int32_t) * (size_t)(nr + 1), hipMemcpyDeviceToHost, s), "d2h");
+        rc = vnec_hip_ok(hipMemcpyAsync(f->v_row_ptr, d.vrp, sizeof(int32_t) * (size_t)(nr + 1), hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS && d.G)
-        rc = vnec_hip_ok(hipMemcpyAsync(p->col_start, d.col_start, sizeof(int32_t) * (size_t)d.G, hipMemcpyDeviceToHost, s), "d2h");
+        rc = vnec_hip_ok(hipMemcpyAsync(f->col_start, d.col_start, sizeof(int32_t) * (size_t)d.G, hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS && nnz)
         rc = vnec_hip_ok(hipMemcpyAsync(win, d.win, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
-    if (rc == SpB_SUCCESS) {
-        for (int t = 0; t < REF_P; t++) {
-            p->thread_coord_start[t].x = co[2 * t];
-            p->thread_coord_start[t].y = co[2 * t + 1];
-            p->thread_coord_end[t].x = co[2 * t + 2];
-            p->thread_coord_end[t].y = co[2 * t + 3];
-            int n = (int)(off[t + 1] - off[t]);
-            p->NEC_NUM[t] = n;
-            p->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
-            p->ecr_xx_val[t] = (double *)malloc(sizeof(double) * (size_t)(n + REF_L));
-            if (p->use_x_indices[t]) memcpy(p->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
-        }
+    for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
+        f->tcs[t].x = co[2 * t];
+        f->tcs[t].y = co[2 * t + 1];
+        f->tce[t].x = co[2 * t + 2];
+        f->tce[t].y = co[2 * t + 3];
+        int n = (int)(off[t + 1] - off[t]);
+        f->NEC_NUM[t] = n;
+        f->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
+        if (!f->use_x_indices[t]) rc = vnec_fail("%s: out of memory", fn);
+        else memcpy(f->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
     }
     free(co);
     free(off);
     free(win);
     dlay_free(&d);
-    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a, KIND_D) : NULL;
+    if (rc != SpB_SUCCESS) d_fields_free(f);
+    return rc;
+}
+
+struct VNEC_D_FP64 *Build_VNEC_D_FP64(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    if (a->base.element_type != SpB_FP64) {
+        vnec_fail("Build_VNEC_D_FP64: the matrix holds FP32 values (use Build_VNEC_D_FP32)");
+        return NULL;
+    }
+    d_fields f;
+    if (d_public(a, REF_L, "Build_VNEC_D_FP64", &f) != SpB_SUCCESS) return NULL;
+    VNEC_D_FP64 *p = (VNEC_D_FP64 *)calloc(1, sizeof(VNEC_D_FP64));
+    double **xx = (double **)calloc(REF_P + 1, sizeof(double *));
+    SpB_Info rc = (p && xx) ? SpB_SUCCESS : vnec_fail("Build_VNEC_D_FP64: out of memory");
+    for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
+        xx[t] = (double *)malloc(sizeof(double) * (size_t)(f.NEC_NUM[t] + REF_L));
+        if (!xx[t]) rc = vnec_fail("Build_VNEC_D_FP64: out of memory");
+    }
+    if (rc != SpB_SUCCESS) {
+        for (int t = 0; xx && t < REF_P; t++) free(xx[t]);
+        free(xx);
+        free(p);
+        d_fields_free(&f);
+        return NULL;
+    }
+    p->thread_coord_start = f.tcs;
+    p->thread_coord_end = f.tce;
+    p->ecr_indices = f.ecr_indices;
+    p->NEC_NUM = f.NEC_NUM;
+    p->use_x_indices = f.use_x_indices;
+    p->ecr_xx_val = xx;
+    p->v_row_ptr = f.v_row_ptr;
+    p->col_start = f.col_start;
+    struct vnec_plan_s *g = gpu_plan_for(a, KIND_D);
     if (!g) {
-        if (p) VNEC_D_free_fp64_ext(p);
+        VNEC_D_free_fp64_ext(p);
         return NULL;
     }
     add_wrap(p, KIND_D, g, a);
     return p;
 }
 
-/* spmv_vnec_S_for_mxv.c:336-588; ECR fields at the snapped partitions */
-struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
+struct VNEC_D_FP32 *Build_VNEC_D_FP32(const SpB_Matrix A)
 {
     spb_csr *a = spb_as_csr(A);
     if (!a) return NULL;
-    int64_t nnz = (int64_t)a->nnz;
-    VNEC_S_FP64 *p = (VNEC_S_FP64 *)calloc(1, sizeof(VNEC_S_FP64));
+    if (a->base.element_type != SpB_FP32) {
+        vnec_fail("Build_VNEC_D_FP32: the matrix holds FP64 values (use Build_VNEC_D_FP64)");
+        return NULL;
+    }
+    d_fields f;
+    if (d_public(a, REF_L32, "Build_VNEC_D_FP32", &f) != SpB_SUCCESS) return NULL;
+    VNEC_D_FP32 *p = (VNEC_D_FP32 *)calloc(1, sizeof(VNEC_D_FP32));
+    float **xx = (float **)calloc(REF_P + 1, sizeof(float *));
+    SpB_Info rc = (p && xx) ? SpB_SUCCESS : vnec_fail("Build_VNEC_D_FP32: out of memory");
+    for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
+        xx[t] = (float *)malloc(sizeof(float) * (size_t)(f.NEC_NUM[t] + REF_L32));
+        if (!xx[t]) rc = vnec_fail("Build_VNEC_D_FP32: out of memory");
+    }
+    if (rc != SpB_SUCCESS) {
+        for (int t = 0; xx && t < REF_P; t++) free(xx[t]);
+        free(xx);
+        free(p);
+        d_fields_free(&f);
+        return NULL;
+    }
+    p->thread_coord_start = f.tcs;
+    p->thread_coord_end = f.tce;
+    p->ecr_indices = f.ecr_indices;
+    p->NEC_NUM = f.NEC_NUM;
+    p->use_x_indices = f.use_x_indices;
+    p->ecr_xx_val = xx;
+    p->v_row_ptr = f.v_row_ptr;
+    p->col_start = f.col_start;
+    struct vnec_plan_s *g = gpu_plan_for(a, KIND_D32);
+    if (!g) {
+        VNEC_D_free_fp32_ext(p);
+        return NULL;
+    }
+    add_wrap(p, KIND_D32, g, a);
+    return p;
+}
+
+/* Host fields the S plans of both value types share; val holds the
+ * reordered values in FP64 (exact for FP32 matrices, whose device mirror
+ * is their exact widening) for the typed builder to keep or narrow. */
+typedef struct {
+    int *NEC_NUM, **use_x_indices, *spvv_len, **tasks;
+    int *col, *row_begin, *row_end, *task_start, *task_end;
+    double *val;
+} s_fields;
+
+static void s_fields_free(s_fields *f)
+{
+    for (int t = 0; t < REF_P; t++) {
+        if (f->use_x_indices) free(f->use_x_indices[t]);
+        if (f->tasks) free(f->tasks[t]);
+    }
+    free(f->NEC_NUM);
+    free(f->use_x_indices);
+    free(f->spvv_len);
+    free(f->tasks);
+    free(f->col);
+    free(f->row_begin);
+    free(f->row_end);
+    free(f->task_start);
+    free(f->task_end);
+    free(f->val);
+    memset(f, 0, sizeof *f);
+}
+
+/* spmv_vnec_S_for_mxv.c:336-588 (FP64) and :3-264 (FP32): snapped
+ * partitions, their ECR, the task order (rows of length <= 32 in groups of
+ * L first) and the reordered matrix, at P = 36 and L lanes. */
+static SpB_Info s_public(spb_csr *a, int L, const char *fn, s_fields *f)
+{
+    memset(f, 0, sizeof *f);
+    int64_t nnz = (int64_t)a->nnz, nr = (int64_t)a->base.row;
     int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
     int32_t *ecr = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
     int32_t *nec = (int32_t *)malloc(sizeof(int32_t) * REF_P);
     int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (REF_P + 1));
     int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
-    SpB_Info rc = (p && co && ecr && nec && off && win) ? SpB_SUCCESS : vnec_fail("Build_VNEC_S_FP64: out of memory");
-    if (rc == SpB_SUCCESS) rc = vnec_partition_ext(A, REF_P, 1, co);
-    if (rc == SpB_SUCCESS) rc = vnec_ecr_ext(A, REF_P, co, 0, ecr, nec, off, win);
+    int32_t *tasks = NULL;
+    size_t rb = sizeof(int) * (size_t)(nr ? nr : 1), zb = (size_t)(nnz ? nnz : 1);
+    SpB_Info rc = (co && ecr && nec && off && win) ? SpB_SUCCESS : vnec_fail("%s: out of memory", fn);
+    if (rc == SpB_SUCCESS) rc = vnec_partition_ext((SpB_Matrix)a, REF_P, 1, co);
+    if (rc == SpB_SUCCESS) rc = vnec_ecr_ext((SpB_Matrix)a, REF_P, co, 0, ecr, nec, off, win);
     if (rc == SpB_SUCCESS) {
-        p->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
-        p->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
-        p->ecr_xx_val = (double **)calloc(REF_P + 1, sizeof(double *));
-        if (!p->NEC_NUM || !p->use_x_indices || !p->ecr_xx_val) rc = vnec_fail("out of memory");
+        f->NEC_NUM = (int *)calloc(REF_P + 1, sizeof(int));
+        f->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
+        f->spvv_len = (int *)calloc(REF_P + 1, sizeof(int));
+        f->tasks = (int **)calloc(REF_P + 1, sizeof(int *));
+        f->val = (double *)malloc(sizeof(double) * zb);
+        f->col = (int *)malloc(sizeof(int) * zb);
+        f->row_begin = (int *)malloc(rb);
+        f->row_end = (int *)malloc(rb);
+        f->task_start = (int *)calloc(REF_P + 1, sizeof(int));
+        f->task_end = (int *)calloc(REF_P + 1, sizeof(int));
+        tasks = (int32_t *)malloc(rb);
+        if (!f->NEC_NUM || !f->use_x_indices || !f->spvv_len || !f->tasks || !f->val || !f->col ||
+            !f->row_begin || !f->row_end || !f->task_start || !f->task_end || !tasks)
+            rc = vnec_fail("%s: out of memory", fn);
     }
-    if (rc == SpB_SUCCESS) {
-        for (int t = 0; t < REF_P; t++) {
-            int n = nec[t];
-            p->NEC_NUM[t] = n;
-            p->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
-            p->ecr_xx_val[t] = (double *)malloc(sizeof(double) * (size_t)(n ? n : 1));
-            if (p->use_x_indices[t]) memcpy(p->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
-        }
+    for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
+        int n = nec[t];
+        f->NEC_NUM[t] = n;
+        f->use_x_indices[t] = (int *)malloc(sizeof(int) * (size_t)(n ? n : 1));
+        if (!f->use_x_indices[t]) rc = vnec_fail("%s: out of memory", fn);
+        else memcpy(f->use_x_indices[t], win + off[t], sizeof(int) * (size_t)n);
     }
     /* task order and reordered matrix (spmv_vnec_S_for_mxv.c:442-585) */
-    int64_t nr = (int64_t)a->base.row;
-    int32_t *tasks = NULL;
-    if (rc == SpB_SUCCESS) {
-        size_t rb = sizeof(int) * (size_t)(nr ? nr : 1), zb = (size_t)(nnz ? nnz : 1);
-        tasks = (int32_t *)malloc(rb);
-        p->spvv8_len = (int *)calloc(REF_P + 1, sizeof(int));
-        p->tasks = (int **)calloc(REF_P + 1, sizeof(int *));
-        p->reorder_mat.nnz = (double *)malloc(sizeof(double) * zb);
-        p->reorder_mat.col = (int *)malloc(sizeof(int) * zb);
-        p->reorder_mat.row_begin = (int *)malloc(rb);
-        p->reorder_mat.row_end = (int *)malloc(rb);
-        p->reorder_mat.task_start = (int *)calloc(REF_P + 1, sizeof(int));
-        p->reorder_mat.task_end = (int *)calloc(REF_P + 1, sizeof(int));
-        if (!tasks || !p->spvv8_len || !p->tasks || !p->reorder_mat.nnz || !p->reorder_mat.col ||
-            !p->reorder_mat.row_begin || !p->reorder_mat.row_end || !p->reorder_mat.task_start ||
-            !p->reorder_mat.task_end)
-            rc = vnec_fail("Build_VNEC_S_FP64: out of memory");
-    }
     if (rc == SpB_SUCCESS)
-        rc = vnec_s_order_ext(A, REF_P, REF_L, REF_CROSS_MAX, tasks, p->spvv8_len, p->reorder_mat.row_begin,
-                              p->reorder_mat.row_end, p->reorder_mat.nnz, p->reorder_mat.col,
-                              p->reorder_mat.task_start, p->reorder_mat.task_end);
+        rc = vnec_s_order_ext((SpB_Matrix)a, REF_P, L, REF_CROSS_MAX, tasks, f->spvv_len, f->row_begin,
+                              f->row_end, f->val, f->col, f->task_start, f->task_end);
     for (int t = 0; rc == SpB_SUCCESS && t < REF_P; t++) {
-        int n = p->reorder_mat.task_end[t] - p->reorder_mat.task_start[t];
-        p->tasks[t] = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-        if (!p->tasks[t]) rc = vnec_fail("Build_VNEC_S_FP64: out of memory");
-        else if (n > 0) memcpy(p->tasks[t], tasks + p->reorder_mat.task_start[t], sizeof(int) * (size_t)n);
+        int n = f->task_end[t] - f->task_start[t];
+        f->tasks[t] = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        if (!f->tasks[t]) rc = vnec_fail("%s: out of memory", fn);
+        else if (n > 0) memcpy(f->tasks[t], tasks + f->task_start[t], sizeof(int) * (size_t)n);
     }
     free(tasks);
     free(co);
@@ -594,39 +722,173 @@ struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
     free(nec);
     free(off);
     free(win);
-    struct vnec_plan_s *g = rc == SpB_SUCCESS ? gpu_plan_for(a, KIND_S) : NULL;
+    if (rc != SpB_SUCCESS) s_fields_free(f);
+    return rc;
+}
+
+static double **alloc_xx_f64(const int *nec)
+{
+    double **xx = (double **)calloc(REF_P + 1, sizeof(double *));
+    for (int t = 0; xx && t < REF_P; t++)
+        if (!(xx[t] = (double *)malloc(sizeof(double) * (size_t)(nec[t] ? nec[t] : 1)))) {
+            for (int u = 0; u < t; u++) free(xx[u]);
+            free(xx);
+            return NULL;
+        }
+    return xx;
+}
+
+static float **alloc_xx_f32(const int *nec)
+{
+    float **xx = (float **)calloc(REF_P + 1, sizeof(float *));
+    for (int t = 0; xx && t < REF_P; t++)
+        if (!(xx[t] = (float *)malloc(sizeof(float) * (size_t)(nec[t] ? nec[t] : 1)))) {
+            for (int u = 0; u < t; u++) free(xx[u]);
+            free(xx);
+            return NULL;
+        }
+    return xx;
+}
+
+struct VNEC_S_FP64 *Build_VNEC_S_FP64(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    if (a->base.element_type != SpB_FP64) {
+        vnec_fail("Build_VNEC_S_FP64: the matrix holds FP32 values (use Build_VNEC_S_FP32)");
+        return NULL;
+    }
+    s_fields f;
+    if (s_public(a, REF_L, "Build_VNEC_S_FP64", &f) != SpB_SUCCESS) return NULL;
+    VNEC_S_FP64 *p = (VNEC_S_FP64 *)calloc(1, sizeof(VNEC_S_FP64));
+    double **xx = alloc_xx_f64(f.NEC_NUM);
+    if (!p || !xx) {
+        free(p);
+        if (xx)
+            for (int t = 0; t < REF_P; t++) free(xx[t]);
+        free(xx);
+        s_fields_free(&f);
+        vnec_fail("Build_VNEC_S_FP64: out of memory");
+        return NULL;
+    }
+    p->NEC_NUM = f.NEC_NUM;
+    p->use_x_indices = f.use_x_indices;
+    p->ecr_xx_val = xx;
+    p->spvv8_len = f.spvv_len;
+    p->tasks = f.tasks;
+    p->reorder_mat.nnz = f.val;
+    p->reorder_mat.col = f.col;
+    p->reorder_mat.row_begin = f.row_begin;
+    p->reorder_mat.row_end = f.row_end;
+    p->reorder_mat.task_start = f.task_start;
+    p->reorder_mat.task_end = f.task_end;
+    struct vnec_plan_s *g = gpu_plan_for(a, KIND_S);
     if (!g) {
-        if (p) VNEC_S_free_fp64_ext(p);
+        VNEC_S_free_fp64_ext(p);
         return NULL;
     }
     add_wrap(p, KIND_S, g, a);
     return p;
 }
 
+struct VNEC_S_FP32 *Build_VNEC_S_FP32(const SpB_Matrix A)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a) return NULL;
+    if (a->base.element_type != SpB_FP32) {
+        vnec_fail("Build_VNEC_S_FP32: the matrix holds FP64 values (use Build_VNEC_S_FP64)");
+        return NULL;
+    }
+    s_fields f;
+    if (s_public(a, REF_L32, "Build_VNEC_S_FP32", &f) != SpB_SUCCESS) return NULL;
+    int64_t nnz = (int64_t)a->nnz;
+    VNEC_S_FP32 *p = (VNEC_S_FP32 *)calloc(1, sizeof(VNEC_S_FP32));
+    float **xx = alloc_xx_f32(f.NEC_NUM);
+    float *v32 = (float *)malloc(sizeof(float) * (size_t)(nnz ? nnz : 1));
+    if (!p || !xx || !v32) {
+        free(p);
+        if (xx)
+            for (int t = 0; t < REF_P; t++) free(xx[t]);
+        free(xx);
+        free(v32);
+        s_fields_free(&f);
+        vnec_fail("Build_VNEC_S_FP32: out of memory");
+        return NULL;
+    }
+    for (int64_t j = 0; j < nnz; j++) v32[j] = (float)f.val[j]; /* exact: widened FP32 values */
+    free(f.val);
+    f.val = NULL;
+    p->NEC_NUM = f.NEC_NUM;
+    p->use_x_indices = f.use_x_indices;
+    p->ecr_xx_val = xx;
+    p->spvv16_len = f.spvv_len;
+    p->tasks = f.tasks;
+    p->reorder_mat.nnz = v32;
+    p->reorder_mat.col = f.col;
+    p->reorder_mat.row_begin = f.row_begin;
+    p->reorder_mat.row_end = f.row_end;
+    p->reorder_mat.task_start = f.task_start;
+    p->reorder_mat.task_end = f.task_end;
+    struct vnec_plan_s *g = gpu_plan_for(a, KIND_S32);
+    if (!g) {
+        VNEC_S_free_fp32_ext(p);
+        return NULL;
+    }
+    add_wrap(p, KIND_S32, g, a);
+    return p;
+}
+
 /* ------------------------------------------------------------- SpMV */
-static SpB_Info run_wrapped(const void *pub, plan_kind kind, SpB_Vector y, const SpB_Matrix A,
-                            const SpB_Vector x)
+static SpB_Info check_types(const spb_csr *a, SpB_Vector y, SpB_Vector x, SpB_Type want, const char *fn)
+{
+    spb_dense *xv = spb_as_dense(x), *yv = spb_as_dense(y);
+    if (!xv || !yv) return SpB_FAILURE;
+    const char *tn = want == SpB_FP32 ? "FP32" : "FP64";
+    if (a->base.element_type != want) return vnec_fail("%s: the matrix does not hold %s values", fn, tn);
+    if (xv->base.element_type != want || yv->base.element_type != want)
+        return vnec_fail("%s: x and y must be %s vectors", fn, tn);
+    return SpB_SUCCESS;
+}
+
+static SpB_Info run_wrapped(const void *pub, plan_kind kind, SpB_Type type, SpB_Vector y,
+                            const SpB_Matrix A, const SpB_Vector x, const char *fn)
 {
     vnec_clear_error();
     wrap *w = find_wrap(pub);
-    if (!w || w->kind != kind) return vnec_fail("plan was not made by this library's Build_VNEC_*");
-    if ((const spb_csr *)A != w->A) return vnec_fail("plan was built for another matrix");
+    if (!w || w->kind != kind) return vnec_fail("%s: plan was not made by this library's Build_VNEC_*", fn);
+    if ((const spb_csr *)A != w->A) return vnec_fail("%s: plan was built for another matrix", fn);
+    if (check_types(w->A, y, (SpB_Vector)x, type, fn) != SpB_SUCCESS) return SpB_FAILURE;
     return vnec_spmv_ext(w->gpu, y, x);
 }
 
 void SpMV_VNEC_L_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_L *mat_thd)
 {
-    run_wrapped(mat_thd, KIND_L, y, A, x);
+    run_wrapped(mat_thd, KIND_L, SpB_FP64, y, A, x, "SpMV_VNEC_L_FP64");
+}
+
+void SpMV_VNEC_L_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_L *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_L, SpB_FP32, y, A, x, "SpMV_VNEC_L_FP32");
 }
 
 void SpMV_VNEC_D_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_D_FP64 *mat_thd)
 {
-    run_wrapped(mat_thd, KIND_D, y, A, x);
+    run_wrapped(mat_thd, KIND_D, SpB_FP64, y, A, x, "SpMV_VNEC_D_FP64");
+}
+
+void SpMV_VNEC_D_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_D_FP32 *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_D32, SpB_FP32, y, A, x, "SpMV_VNEC_D_FP32");
 }
 
 void SpMV_VNEC_S_FP64(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_S_FP64 *mat_thd)
 {
-    run_wrapped(mat_thd, KIND_S, y, A, x);
+    run_wrapped(mat_thd, KIND_S, SpB_FP64, y, A, x, "SpMV_VNEC_S_FP64");
+}
+
+void SpMV_VNEC_S_FP32(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, VNEC_S_FP32 *mat_thd)
+{
+    run_wrapped(mat_thd, KIND_S32, SpB_FP32, y, A, x, "SpMV_VNEC_S_FP32");
 }
 
 SpB_Info SpB_mxv(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, void *VNEC)
@@ -638,6 +900,7 @@ SpB_Info SpB_mxv(SpB_Vector y, const SpB_Matrix A, const SpB_Vector x, void *VNE
     if (xv->base.n != a->base.col || yv->base.n != a->base.row)
         return vnec_fail("SpB_mxv: dimension mismatch (A %lux%lu, x %lu, y %lu)", a->base.row,
                          a->base.col, xv->base.n, yv->base.n);
+    if (check_types(a, y, x, a->base.element_type, "SpB_mxv") != SpB_SUCCESS) return SpB_FAILURE;
     if (VNEC) {
         wrap *w = find_wrap(VNEC);
         if (w) {
@@ -673,18 +936,26 @@ SpB_Info VNEC_D_free_fp64_ext(VNEC_D_FP64 *p)
     if (!p) return vnec_fail("VNEC_D_free_fp64_ext: NULL plan");
     struct vnec_plan_s *g = drop_wrap(p);
     if (g) vnec_plan_free_ext(&g);
-    for (int t = 0; t < REF_P; t++) {
-        if (p->use_x_indices) free(p->use_x_indices[t]);
-        if (p->ecr_xx_val) free(p->ecr_xx_val[t]);
-    }
-    free(p->thread_coord_start);
-    free(p->thread_coord_end);
-    free(p->ecr_indices);
-    free(p->NEC_NUM);
-    free(p->use_x_indices);
+    for (int t = 0; p->ecr_xx_val && t < REF_P; t++) free(p->ecr_xx_val[t]);
     free(p->ecr_xx_val);
-    free(p->v_row_ptr);
-    free(p->col_start);
+    d_fields f = {p->thread_coord_start, p->thread_coord_end, p->ecr_indices, p->NEC_NUM,
+                  p->use_x_indices, p->v_row_ptr, p->col_start};
+    d_fields_free(&f);
+    free(p->val_align);
+    free(p);
+    return SpB_SUCCESS;
+}
+
+SpB_Info VNEC_D_free_fp32_ext(VNEC_D_FP32 *p)
+{
+    if (!p) return vnec_fail("VNEC_D_free_fp32_ext: NULL plan");
+    struct vnec_plan_s *g = drop_wrap(p);
+    if (g) vnec_plan_free_ext(&g);
+    for (int t = 0; p->ecr_xx_val && t < REF_P; t++) free(p->ecr_xx_val[t]);
+    free(p->ecr_xx_val);
+    d_fields f = {p->thread_coord_start, p->thread_coord_end, p->ecr_indices, p->NEC_NUM,
+                  p->use_x_indices, p->v_row_ptr, p->col_start};
+    d_fields_free(&f);
     free(p->val_align);
     free(p);
     return SpB_SUCCESS;
@@ -695,22 +966,28 @@ SpB_Info VNEC_S_free_fp64_ext(VNEC_S_FP64 *p)
     if (!p) return vnec_fail("VNEC_S_free_fp64_ext: NULL plan");
     struct vnec_plan_s *g = drop_wrap(p);
     if (g) vnec_plan_free_ext(&g);
-    for (int t = 0; t < REF_P; t++) {
-        if (p->use_x_indices) free(p->use_x_indices[t]);
-        if (p->ecr_xx_val) free(p->ecr_xx_val[t]);
-        if (p->tasks) free(p->tasks[t]);
-    }
-    free(p->NEC_NUM);
-    free(p->use_x_indices);
+    for (int t = 0; p->ecr_xx_val && t < REF_P; t++) free(p->ecr_xx_val[t]);
     free(p->ecr_xx_val);
-    free(p->spvv8_len);
-    free(p->tasks);
+    s_fields f = {p->NEC_NUM, p->use_x_indices, p->spvv8_len, p->tasks, p->reorder_mat.col,
+                  p->reorder_mat.row_begin, p->reorder_mat.row_end, p->reorder_mat.task_start,
+                  p->reorder_mat.task_end, p->reorder_mat.nnz};
+    s_fields_free(&f);
+    free(p);
+    return SpB_SUCCESS;
+}
+
+SpB_Info VNEC_S_free_fp32_ext(VNEC_S_FP32 *p)
+{
+    if (!p) return vnec_fail("VNEC_S_free_fp32_ext: NULL plan");
+    struct vnec_plan_s *g = drop_wrap(p);
+    if (g) vnec_plan_free_ext(&g);
+    for (int t = 0; p->ecr_xx_val && t < REF_P; t++) free(p->ecr_xx_val[t]);
+    free(p->ecr_xx_val);
     free(p->reorder_mat.nnz);
-    free(p->reorder_mat.col);
-    free(p->reorder_mat.row_begin);
-    free(p->reorder_mat.row_end);
-    free(p->reorder_mat.task_start);
-    free(p->reorder_mat.task_end);
+    s_fields f = {p->NEC_NUM, p->use_x_indices, p->spvv16_len, p->tasks, p->reorder_mat.col,
+                  p->reorder_mat.row_begin, p->reorder_mat.row_end, p->reorder_mat.task_start,
+                  p->reorder_mat.task_end, NULL};
+    s_fields_free(&f);
     free(p);
     return SpB_SUCCESS;
 }

@@ -2,8 +2,10 @@
 C-ABI of libvnec_mi355x.so.
 
 Names follow the reference (SpB_Matrix / SpB_Vector objects, IRD_VNEC,
-Build_VNEC_* / SpMV_VNEC_*_FP64, SpB_mxv); errors raise RuntimeError with
-the library's reason.  Everything runs on the GPU through the library.
+Build_VNEC_* / SpMV_VNEC_*_FP64 and _FP32, SpB_mxv); errors raise
+RuntimeError with the library's reason.  Everything runs on the GPU through
+the library.  Objects hold FP64 values unless made with dtype=np.float32
+(the reference's FP32 API: FP32 values, x, y and sums).
 """
 from __future__ import annotations
 
@@ -14,11 +16,12 @@ import numpy as np
 
 from . import _lib as L
 
+SpB_FP32 = 10
 SpB_FP64 = 11
 SpB_MATRIX_FORMAT = 1
 SpB_CSR = 1
 VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4}
-VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2}
+VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2, "fp32": 3}
 # Selection modes (negative vnec_tuning.variant values).
 SELECT = {"learned": L.SELECT_LEARNED, "autotune": L.SELECT_AUTOTUNE, "rule": L.SELECT_RULE,
           "reference": L.SELECT_REFERENCE}
@@ -46,24 +49,39 @@ def _p(a: np.ndarray):
     return a.ctypes.data
 
 
-class Matrix:
-    """SpB_Matrix (CSR, FP64)."""
+def _spb_type(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return SpB_FP64
+    if dt == np.float32:
+        return SpB_FP32
+    raise ValueError(f"the VNEC path takes float64 or float32 values, not {dt}")
 
-    def __init__(self, nrows: int, ncols: int):
+
+class Matrix:
+    """SpB_Matrix (CSR, FP64 or FP32 values)."""
+
+    def __init__(self, nrows: int, ncols: int, dtype=np.float64):
         lib = init()
         self.h = C.c_void_p()
-        L.check(lib.SpB_Matrix_new(C.byref(self.h), SpB_FP64, nrows, ncols), "SpB_Matrix_new")
+        self.dtype = np.dtype(dtype)
+        L.check(lib.SpB_Matrix_new(C.byref(self.h), _spb_type(dtype), nrows, ncols),
+                "SpB_Matrix_new")
         self.nrows, self.ncols = nrows, ncols
         self.nnz = 0
 
+    @property
+    def fp32(self) -> bool:
+        return self.dtype == np.float32
+
     @classmethod
-    def from_csr(cls, nrows, ncols, ptr, col, val) -> "Matrix":
-        m = cls(nrows, ncols)
+    def from_csr(cls, nrows, ncols, ptr, col, val, dtype=np.float64) -> "Matrix":
+        m = cls(nrows, ncols, dtype)
         ptr = np.ascontiguousarray(ptr, dtype=np.int64)
         col = np.ascontiguousarray(col, dtype=np.int32)
-        val = np.ascontiguousarray(val, dtype=np.float64)
-        L.check(L.load().SpB_Matrix_import_csr_ext(m.h, _p(ptr), _p(col), _p(val)),
-                "SpB_Matrix_import_csr_ext")
+        val = np.ascontiguousarray(val, dtype=m.dtype)
+        f = "SpB_Matrix_import_csr_fp32_ext" if m.fp32 else "SpB_Matrix_import_csr_ext"
+        L.check(getattr(L.load(), f)(m.h, _p(ptr), _p(col), _p(val)), f)
         m.nnz = int(ptr[-1])
         return m
 
@@ -82,13 +100,13 @@ class Matrix:
         return m
 
     @classmethod
-    def from_coo(cls, nrows, ncols, rows, cols, vals) -> "Matrix":
-        m = cls(nrows, ncols)
+    def from_coo(cls, nrows, ncols, rows, cols, vals, dtype=np.float64) -> "Matrix":
+        m = cls(nrows, ncols, dtype)
         r = np.ascontiguousarray(rows, dtype=np.uint64)
         c = np.ascontiguousarray(cols, dtype=np.uint64)
-        v = np.ascontiguousarray(vals, dtype=np.float64)
-        L.check(L.load().SpB_Matrix_build_FP64(m.h, _p(r), _p(c), _p(v), len(v), 0),
-                "SpB_Matrix_build_FP64")
+        v = np.ascontiguousarray(vals, dtype=m.dtype)
+        f = "SpB_Matrix_build_FP32" if m.fp32 else "SpB_Matrix_build_FP64"
+        L.check(getattr(L.load(), f)(m.h, _p(r), _p(c), _p(v), len(v), 0), f)
         m.nnz = len(v)
         return m
 
@@ -106,34 +124,41 @@ class Matrix:
 
 
 class Vector:
-    """SpB_Vector (dense, FP64)."""
+    """SpB_Vector (dense, FP64 or FP32 values)."""
 
-    def __init__(self, n: int):
+    def __init__(self, n: int, dtype=np.float64):
         lib = init()
         self.h = C.c_void_p()
-        L.check(lib.SpB_Vector_new(C.byref(self.h), SpB_FP64, n), "SpB_Vector_new")
+        self.dtype = np.dtype(dtype)
+        L.check(lib.SpB_Vector_new(C.byref(self.h), _spb_type(dtype), n), "SpB_Vector_new")
         self.n = n
 
+    @property
+    def fp32(self) -> bool:
+        return self.dtype == np.float32
+
     @classmethod
-    def from_array(cls, a) -> "Vector":
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        v = cls(len(a))
-        L.check(L.load().SpB_Vector_import_ext(v.h, _p(a)), "SpB_Vector_import_ext")
+    def from_array(cls, a, dtype=np.float64) -> "Vector":
+        a = np.ascontiguousarray(a, dtype=dtype)
+        v = cls(len(a), dtype)
+        f = "SpB_Vector_import_fp32_ext" if v.fp32 else "SpB_Vector_import_ext"
+        L.check(getattr(L.load(), f)(v.h, _p(a)), f)
         return v
 
     def to_array(self) -> np.ndarray:
-        out = np.empty(self.n, dtype=np.float64)
-        L.check(L.load().SpB_Vector_export_ext(self.h, _p(out)), "SpB_Vector_export_ext")
+        out = np.empty(self.n, dtype=self.dtype)
+        f = "SpB_Vector_export_fp32_ext" if self.fp32 else "SpB_Vector_export_ext"
+        L.check(getattr(L.load(), f)(self.h, _p(out)), f)
         return out
 
     def set(self, i: int, x: float):
-        L.check(L.load().SpB_Vector_setElement_FP64(self.h, float(x), i),
-                "SpB_Vector_setElement_FP64")
+        f = "SpB_Vector_setElement_FP32" if self.fp32 else "SpB_Vector_setElement_FP64"
+        L.check(getattr(L.load(), f)(self.h, float(x), i), f)
 
     def get(self, i: int) -> float:
-        out = C.c_double()
-        L.check(L.load().SpB_Vector_extractElement_FP64(C.byref(out), self.h, i),
-                "SpB_Vector_extractElement_FP64")
+        out = C.c_float() if self.fp32 else C.c_double()
+        f = "SpB_Vector_extractElement_FP32" if self.fp32 else "SpB_Vector_extractElement_FP64"
+        L.check(getattr(L.load(), f)(C.byref(out), self.h, i), f)
         return out.value
 
     def resident(self, on: bool = True):
@@ -208,8 +233,10 @@ class Plan:
     """GPU execution plan (vnec_plan_build_ext)."""
 
     def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
-                 encoding: str = "fp64", nrhs: int = 1, grid: int = 0):
+                 encoding: str | None = None, nrhs: int = 1, grid: int = 0):
         init()
+        if encoding is None:  # the matrix's own arithmetic
+            encoding = "fp32" if A.fp32 else "fp64"
         code = -1 if variant is None else (VNEC_VAR[variant] if variant in VNEC_VAR
                                             else SELECT[variant])
         t = L.Tuning(code, ipt, lanes, VNEC_ENC[encoding], nrhs, grid)
@@ -262,8 +289,10 @@ class Plan:
 
 
 # ------------------------------------------------ reference entry points
-def ird_vnec(A: Matrix) -> int:
-    return int(L.load().IRD_VNEC(A.h, SpB_FP64))
+def ird_vnec(A: Matrix, fp32: bool | None = None) -> int:
+    """IRD_VNEC(A, SpB_FP32 or SpB_FP64); the type defaults to A's."""
+    fp32 = A.fp32 if fp32 is None else fp32
+    return int(L.load().IRD_VNEC(A.h, SpB_FP32 if fp32 else SpB_FP64))
 
 
 def ird_ext(A: Matrix, P=36, L_=8, thr=0.477) -> tuple[int, float]:
@@ -273,22 +302,28 @@ def ird_ext(A: Matrix, P=36, L_=8, thr=0.477) -> tuple[int, float]:
 
 
 def build_vnec(kind: str, A: Matrix):
-    f = {"L": "Build_VNEC_L", "D": "Build_VNEC_D_FP64", "S": "Build_VNEC_S_FP64"}[kind]
+    """Build_VNEC_{L, D_FP64, S_FP64} (or the FP32 builders for an FP32 A)."""
+    sfx = "FP32" if A.fp32 else "FP64"
+    f = {"L": "Build_VNEC_L", "D": f"Build_VNEC_D_{sfx}", "S": f"Build_VNEC_S_{sfx}"}[kind]
     p = getattr(L.load(), f)(A.h)
     if not p:
         raise RuntimeError(f"{f}: {L.last_error()}")
     return p
 
 
-def spmv_vnec(kind: str, y: Vector, A: Matrix, x: Vector, plan):
-    f = {"L": "SpMV_VNEC_L_FP64", "D": "SpMV_VNEC_D_FP64", "S": "SpMV_VNEC_S_FP64"}[kind]
+def spmv_vnec(kind: str, y: Vector, A: Matrix, x: Vector, plan, fp32: bool | None = None):
+    """SpMV_VNEC_{L,D,S}_FP64 (or _FP32: by default A's type); returns the
+    library's error text ("" on success), as these calls return void."""
+    fp32 = A.fp32 if fp32 is None else fp32
+    f = f"SpMV_VNEC_{kind}_{'FP32' if fp32 else 'FP64'}"
     getattr(L.load(), f)(y.h, A.h, x.h, C.cast(plan, C.c_void_p))
     err = L.last_error()
     return err
 
 
-def free_vnec(kind: str, plan):
-    f = {"L": "VNEC_L_free_ext", "D": "VNEC_D_free_fp64_ext", "S": "VNEC_S_free_fp64_ext"}[kind]
+def free_vnec(kind: str, plan, fp32: bool = False):
+    sfx = "fp32" if fp32 else "fp64"
+    f = {"L": "VNEC_L_free_ext", "D": f"VNEC_D_free_{sfx}_ext", "S": f"VNEC_S_free_{sfx}_ext"}[kind]
     L.check(getattr(L.load(), f)(C.cast(plan, C.c_void_p)), f)
 
 
@@ -368,12 +403,13 @@ def read_graph(path: str) -> tuple[int, np.ndarray, np.ndarray]:
     return n.value, ptr, col
 
 
-def matrix_from_file(path: str, values: str = "row") -> Matrix:
+def matrix_from_file(path: str, values: str = "row", dtype=np.float64) -> Matrix:
     """SpB_Matrix_read_ext: a_ij = i (\"row\", the harness's values) or 1 (\"one\")."""
     init()
     m = Matrix.__new__(Matrix)
     m.h = C.c_void_p()
-    L.check(L.load().SpB_Matrix_read_ext(C.byref(m.h), path.encode(),
+    m.dtype = np.dtype(dtype)
+    L.check(L.load().SpB_Matrix_read_ext(C.byref(m.h), path.encode(), _spb_type(dtype),
                                          {"row": 0, "one": 1}[values]), "SpB_Matrix_read_ext")
     n, ptr, _ = read_graph(path)
     m.nrows = m.ncols = n
