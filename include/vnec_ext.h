@@ -79,6 +79,9 @@ typedef enum {
     VNEC_VAR_D = 3, /* dense column blocks of width `lanes`                 */
     VNEC_VAR_H = 4, /* persistent merge-path CSR with the `lanes` (4096 or
                        8192) most used x entries held in LDS                */
+    VNEC_VAR_G = 5, /* pattern slices: rows whose columns are a subset of
+                       the dominant column-offset pattern store a position
+                       mask and values only; the rest run a sub-plan       */
     VNEC_VAR_N
 } vnec_variant;
 
@@ -185,6 +188,20 @@ SpB_Info vnec_d_layout_ext(const SpB_Matrix A, int64_t P, int L, int32_t *v_row_
 SpB_Info vnec_s_order_ext(const SpB_Matrix A, int64_t P, int L, int cut, int32_t *tasks,
                           int32_t *spvv_len, int32_t *row_begin, int32_t *row_end, double *s_val,
                           int32_t *s_col, int32_t *task_start, int32_t *task_end);
+
+/* Variant G's row pattern: the column offsets P[0 .. L) (position l of
+ * row i is column i + P[l]) of the most frequent row among 32 sample rows
+ * i_s = ((s + 1) 2654435761 mod 2^32) nrows / 2^32 (the first on ties);
+ * *L = 0 when no sample row has 1 .. VNEC_PATTERN_MAX entries.
+ * *nregular receives the rows whose columns are a subset of the pattern
+ * (counted on the GPU); the plan stores those as masks and values. */
+#define VNEC_PATTERN_MAX 32
+SpB_Info vnec_pattern_ext(const SpB_Matrix A, int32_t pattern[VNEC_PATTERN_MAX], int *L,
+                          int64_t *nregular);
+/* masks[i] = positions of the pattern row i of A holds, flags[i] = 1 when
+ * it is not a subset of the pattern (GPU; host arrays of nrows). */
+SpB_Info vnec_pattern_match_ext(const SpB_Matrix A, const int32_t *pattern, int L, uint32_t *masks,
+                                int32_t *flags);
 
 /* IRD ratio (float)nnz / (float)(G * L) and the D / S decision at (P, L,
  * threshold); IRD_VNEC uses P = 36 and L = 8, 0.477 for SpB_FP64 or

@@ -213,6 +213,38 @@ int vnec_hip_d_interleave_f32(const double *in, int64_t G, int L, float *out, vo
 int vnec_hip_spmv_d(const vnec_d_args *a, void *stream);
 int vnec_hip_d_supported(int ipt, int lanes);
 
+/* ---- variant G: pattern slices ---- */
+/* Longest row pattern a plan stores (positions per row; one mask bit each). */
+#define VNEC_PATTERN_MAX 32
+typedef struct {
+    const uint32_t *masks; /* nrows: pattern positions each row holds (0: none,
+                              or a row left to the sub-plan)                 */
+    const double *gv;      /* nslices * L * 64 values, slice s stored [l][lane],
+                              rows 64 s .. 64 s + 63, 0 where a bit is clear  */
+    const int32_t *pat;    /* L column offsets: position l of row i is column
+                              i + pat[l]                                      */
+    int L;
+    int64_t nrows;
+    int64_t nslices;       /* ceil(nrows / 64)                                */
+    const double *x;
+    double *y;
+} vnec_pattern_args;
+/* masks[i] = positions of the pattern row i holds; flags[i] = 1 when row i
+ * is not a subset of the pattern (a column outside it, or repeated). */
+int vnec_hip_pattern_match(const int32_t *ptr, const int32_t *col, int64_t nrows, const int32_t *pat,
+                           int L, uint32_t *masks, int32_t *flags, void *stream);
+/* With pos = exclusive scan of flags: irr[pos[i]] = i for flagged rows. */
+int vnec_hip_pattern_split(const int32_t *flags, const int32_t *pos, int64_t nrows, int32_t *irr,
+                           void *stream);
+/* gv[(s L + l) 64 + lane] = value of row 64 s + lane at position l (0 when
+ * the row does not hold it, or past nrows). */
+int vnec_hip_pattern_fill(const int32_t *ptr, const double *val, const uint32_t *masks, int64_t nrows,
+                          int L, double *gv, void *stream);
+/* y[i] = sum over held positions l of gv * x[i + pat[l]], every row i. */
+int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream);
+/* y[rows[i]] = ysub[i], i < n. */
+int vnec_hip_scatter_rows(const double *ysub, const int32_t *rows, int64_t n, double *y, void *stream);
+
 /* ---- VNEC-S task order and reordered matrix (builder) ---- */
 /* Stable sort of n (key, value) pairs by the low key_bits bits of the keys
  * (LSD radix, 8-bit digits); allocates its own scratch. */

@@ -1,0 +1,166 @@
+"""Variant G (pattern slices) through the C-ABI.
+
+The dominant row pattern and the per-row masks and flags are integer
+structures: bit-exact against a numpy restatement of the rule (32 sample
+rows at multiplicative-hash positions, the most frequent offset list, the
+first on ties; a row fits when each of its columns is row + pattern[l] for
+a distinct l, its mask holding those l).  SpMV results meet the FP64 gate
+|y - y_ref| <= 1e-12 (|A||x|)_i against the serial oracle, for the rows
+that fit and for the rest (run by the sub-plan) alike.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from vnec_amd import matgen as g
+from vnec_amd import spb
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if spb.device_count() < 1:
+        pytest.fail("no GPU visible: the gpu tests need an MI355X")
+    spb.init()
+
+
+def np_pattern(ptr, col):
+    n = len(ptr) - 1
+    samples = []
+    for s in range(32):
+        i = ((((s + 1) * 2654435761) % 2 ** 32) * n) >> 32
+        b, e = ptr[i], ptr[i + 1]
+        samples.append(tuple((col[b:e].astype(np.int64) - i).tolist()) if 1 <= e - b <= 32 else None)
+    best, best_n = None, 0
+    for cand in samples:
+        if cand is None:
+            continue
+        c = sum(1 for t in samples if t == cand)
+        if c > best_n:
+            best, best_n = cand, c
+    return np.array(best if best else [], np.int64)
+
+
+def np_match(ptr, col, pat):
+    """(masks, flags) by a per-row Python walk (small matrices)."""
+    n = len(ptr) - 1
+    pos = {int(p): l for l, p in enumerate(pat)}
+    masks = np.zeros(n, np.uint32)
+    flags = np.zeros(n, np.int32)
+    for i in range(n):
+        m, last = 0, -1
+        for c in col[ptr[i]:ptr[i + 1]]:
+            l = pos.get(int(c) - i, -1)
+            if l <= last:
+                flags[i] = 1
+                m = 0
+                break
+            m |= 1 << l
+            last = l
+        masks[i] = m
+    return masks, flags
+
+
+def planted(n=20000, frac_irregular=0.3, seed=3):
+    """Rows follow offsets (-7, -1, 0, 2, 9) except a random 30 %, which get
+    random columns; rows near the edges lose out-of-range entries."""
+    rng = np.random.default_rng(seed)
+    pat = np.array([-7, -1, 0, 2, 9])
+    cols = []
+    for i in range(n):
+        if rng.random() < frac_irregular:
+            c = np.unique(rng.integers(0, n, rng.integers(0, 12)))
+        else:
+            c = i + pat
+            c = c[(c >= 0) & (c < n)]
+        cols.append(np.sort(c))
+    ptr = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64)
+    col = np.concatenate(cols).astype(np.int32)
+    return g.Csr(n, n, ptr, col, rng.uniform(-1, 1, len(col)), rng.uniform(-1, 1, n), name="planted")
+
+
+def ring(n=50000, seed=4):
+    rng = np.random.default_rng(seed)
+    off = np.array([-3, -1, 0, 1, 3])
+    col = np.sort((np.arange(n)[:, None] + off[None, :]) % n, axis=1).ravel().astype(np.int32)
+    ptr = np.arange(0, 5 * n + 1, 5, dtype=np.int64)
+    return g.Csr(n, n, ptr, col, rng.uniform(-1, 1, len(col)), rng.uniform(-1, 1, n), name="ring")
+
+
+CASES = [("stencil20", g.stencil27(20, seed=5)), ("banded", g.banded(6000, 9, seed=6)),
+         ("planted", planted()), ("ring", ring()), ("rmat12", g.rmat(12, 16, seed=7))]
+
+
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+def test_pattern_and_flags_bit_exact(name, m):
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    pat, nreg = spb.pattern(A)
+    ref = np_pattern(m.ptr, m.col)
+    assert np.array_equal(pat, ref), (pat, ref)
+    if len(pat):
+        masks, flags = spb.pattern_match(A, pat)
+        ref_masks, ref_flags = np_match(m.ptr, m.col, ref)
+        assert np.array_equal(flags, ref_flags)
+        assert np.array_equal(masks, ref_masks)
+        assert nreg == m.nrows - int(ref_flags.sum())
+
+
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+def test_g_plan_matches_oracle(name, m):
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    print(f"case {name} variant G", flush=True)
+    p = spb.Plan(A, "G")
+    x = spb.Vector.from_array(m.x)
+    y = spb.Vector(m.nrows)
+    p.spmv(y, x)
+    yy = y.to_array()
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(yy, y0, ax, TOL)
+    assert ok, worst
+    p.spmv(y, x)  # repeated runs give the same bits
+    assert np.array_equal(y.to_array(), yy)
+    p.free()
+
+
+def test_g_known_answer_is_exact():
+    """a_ij = i, x_j = j on a stencil: every partial sum is an integer below
+    2^53, so any summation order gives the exact y."""
+    m = g.stencil27(24, seed=8)
+    val = np.repeat(np.arange(m.nrows, dtype=np.float64), np.diff(m.ptr))
+    x = np.arange(m.ncols, dtype=np.float64)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
+    p = spb.Plan(A, "G")
+    y = spb.Vector(m.nrows)
+    p.spmv(y, spb.Vector.from_array(x))
+    assert np.array_equal(y.to_array(), O.spmv_serial(m.ptr, m.col, val, x))
+
+
+def test_selection_rule_picks_g_on_large_stencils_only():
+    big = g.stencil27(64, seed=9)  # 262,144 rows, 6.9M nonzeros; every row fits the pattern
+    A = spb.Matrix.from_csr(big.nrows, big.ncols, big.ptr, big.col, big.val)
+    p = spb.Plan(A, "learned")
+    assert p.info().variant == "G"
+    y = spb.Vector(big.nrows)
+    p.spmv(y, spb.Vector.from_array(big.x))
+    y0 = O.spmv_serial(big.ptr, big.col, big.val, big.x)
+    ax, _ = O.spmv_abs(big.ptr, big.col, big.val, big.x)
+    assert O.check_rows(y.to_array(), y0, ax, TOL)[0]
+    p.free()
+    small = g.stencil27(16, seed=10)  # below 2^20 nonzeros: the selector decides
+    B = spb.Matrix.from_csr(small.nrows, small.ncols, small.ptr, small.col, small.val)
+    assert spb.Plan(B, "learned").info().variant != "G"
+    R = spb.Matrix.from_csr(*(lambda r: (r.nrows, r.ncols, r.ptr, r.col, r.val))(g.rmat(17, 16, seed=11)))
+    assert spb.Plan(R, "learned").info().variant != "G"
+
+
+def test_g_refuses_what_it_cannot_store():
+    m = g.stencil27(8, seed=12)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    with pytest.raises(RuntimeError, match="FP64 values only"):
+        spb.Plan(A, "G", encoding="hi")
+    E = spb.Matrix.from_csr(4, 4, np.zeros(5, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    with pytest.raises(RuntimeError, match="no row pattern"):
+        spb.Plan(E, "G")

@@ -1,0 +1,179 @@
+// vnec_pattern.hip — variant G: pattern slices for gfx950.
+//
+// Many structured matrices (stencils, banded systems) have one dominant
+// row pattern: a list of column offsets P[0 .. L) such that row i's entries
+// sit at columns i + P[l] for a subset of l — all of them inside the
+// domain, the ones that would fall outside dropped at its boundary.  For
+// such a row the column indices carry nothing beyond the L-bit mask of
+// which pattern positions it holds, so the plan stores the mask and the
+// values only: 64 consecutive rows form a slice whose values are laid out
+// [l][lane], zero where the mask bit is clear (one coalesced 512-byte load
+// per wavefront per l).  A lane owns one row and reads x[row + P[l]] for
+// the positions it holds: consecutive rows read consecutive x entries, so
+// these gathers coalesce too.  Every row is summed over l, i.e. in its CSR
+// column order.
+//
+// Rows that are not such a subset (a column outside the pattern, or one
+// repeated) get mask 0 here — the kernel writes y = 0 for them — and form
+// a CSR sub-matrix that an ordinary plan runs into a scratch vector,
+// scattered back over those rows by k_scatter_rows afterwards.
+//
+// Per stored slot the kernel moves 8 bytes (the value) plus 4 bytes of
+// mask per row, instead of 10-12 bytes per nonzero for a column-indexed
+// format.
+#include <hip/hip_runtime.h>
+
+#include "vnec_common.h"
+#include "vnec_hip.h"
+
+// masks[i] = bits of the pattern positions row i holds, flags[i] = 1 when
+// row i is not a subset of the pattern (its entries must then go through
+// the sub-plan).  Columns ascend within a row, as do the offsets.
+__global__ void k_pattern_match(const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                int64_t nrows, const int32_t *__restrict__ pat, int L,
+                                uint32_t *__restrict__ masks, int32_t *__restrict__ flags)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int b = ptr[i], e = ptr[i + 1];
+        uint32_t m = 0;
+        int ok = e - b <= L, l = 0;
+        for (int j = b; ok && j < e; j++) {
+            const int64_t off = (int64_t)col[j] - i;
+            while (l < L && (int64_t)pat[l] < off) l++;
+            if (l < L && (int64_t)pat[l] == off) {
+                m |= 1u << l;
+                l++;
+            } else {
+                ok = 0;
+            }
+        }
+        masks[i] = ok ? m : 0u;
+        flags[i] = !ok;
+    }
+}
+
+// pos = exclusive scan of flags: irregular row i goes to irr[pos[i]].
+__global__ void k_pattern_split(const int32_t *__restrict__ flags, const int32_t *__restrict__ pos,
+                                int64_t nrows, int32_t *__restrict__ irr)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) irr[pos[i]] = (int32_t)i;
+}
+
+// gv[(s L + l) 64 + lane] = value of row 64 s + lane at pattern position l
+// (its k-th stored entry, k = set bits of the mask below l), else 0.
+__global__ void k_pattern_fill(const int32_t *__restrict__ ptr, const double *__restrict__ val,
+                               const uint32_t *__restrict__ masks, int64_t nrows, int64_t nslices,
+                               int L, double *__restrict__ gv)
+{
+    const int64_t n = nslices * L * 64;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = t / (64 * (int64_t)L), r = t - s * 64 * L;
+        const int l = (int)(r / 64), lane = (int)(r % 64);
+        const int64_t row = s * 64 + lane;
+        double v = 0.0;
+        if (row < nrows) {
+            const uint32_t m = masks[row];
+            if ((m >> l) & 1u) v = val[ptr[row] + __popc(m & ((1u << l) - 1u))];
+        }
+        gv[t] = v;
+    }
+}
+
+// One wavefront per slice, four slices per workgroup, XCD-aware order.
+template <int U>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
+{
+    const int64_t b = xcd_tile(blockIdx.x, grid);
+    const int64_t s = b * VNEC_NWAVES + wave_id();
+    if (s >= a.nslices) return;
+    const int64_t row = s * 64 + lane_id();
+    const bool live = row < a.nrows;
+    const uint32_t m = live ? __builtin_nontemporal_load(a.masks + row) : 0u;
+    const double *__restrict__ v = a.gv + s * (int64_t)a.L * 64 + lane_id();
+    const double *__restrict__ x = a.x;
+    double acc = 0.0;
+    int l = 0;
+    for (; l + U <= a.L; l += U) {
+        double vv[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) vv[u] = __builtin_nontemporal_load(v + (int64_t)(l + u) * 64);
+#pragma unroll
+        for (int u = 0; u < U; u++) xv[u] = ((m >> (l + u)) & 1u) ? x[row + a.pat[l + u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; u++) acc += vv[u] * xv[u];
+    }
+    for (; l < a.L; l++) {
+        const double vv = __builtin_nontemporal_load(v + (int64_t)l * 64);
+        if ((m >> l) & 1u) acc += vv * x[row + a.pat[l]];
+    }
+    if (live) a.y[row] = acc;
+}
+
+__global__ void k_scatter_rows(const double *__restrict__ ysub, const int32_t *__restrict__ rows,
+                               int64_t n, double *__restrict__ y)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        y[rows[i]] = ysub[i];
+}
+
+static unsigned grid_for(int64_t n)
+{
+    int64_t g = (n + 255) / 256;
+    return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+extern "C" int vnec_hip_pattern_match(const int32_t *ptr, const int32_t *col, int64_t nrows,
+                                      const int32_t *pat, int L, uint32_t *masks, int32_t *flags,
+                                      void *stream)
+{
+    if (nrows <= 0) return 0;
+    if (L < 1 || L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pattern_match, dim3(grid_for(nrows)), dim3(256), 0, (hipStream_t)stream,
+                       ptr, col, nrows, pat, L, masks, flags);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_pattern_split(const int32_t *flags, const int32_t *pos, int64_t nrows,
+                                      int32_t *irr, void *stream)
+{
+    if (nrows <= 0) return 0;
+    hipLaunchKernelGGL(k_pattern_split, dim3(grid_for(nrows)), dim3(256), 0, (hipStream_t)stream,
+                       flags, pos, nrows, irr);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_pattern_fill(const int32_t *ptr, const double *val, const uint32_t *masks,
+                                     int64_t nrows, int L, double *gv, void *stream)
+{
+    const int64_t ns = (nrows + 63) / 64;
+    if (ns <= 0) return 0;
+    if (L < 1 || L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pattern_fill, dim3(grid_for(ns * L * 64)), dim3(256), 0, (hipStream_t)stream,
+                       ptr, val, masks, nrows, ns, L, gv);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
+{
+    if (a->nslices <= 0) return 0;
+    if (a->L < 1 || a->L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
+    const int64_t blocks = (a->nslices + VNEC_NWAVES - 1) / VNEC_NWAVES;
+    const int64_t grid = xcd_grid(blocks);
+    hipLaunchKernelGGL(k_spmv_pattern<8>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0,
+                       (hipStream_t)stream, *a, grid);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_scatter_rows(const double *ysub, const int32_t *rows, int64_t n, double *y,
+                                     void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, ysub, rows,
+                       n, y);
+    return (int)hipGetLastError();
+}

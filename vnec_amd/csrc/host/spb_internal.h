@@ -101,6 +101,17 @@ struct vnec_plan_s {
     int32_t *hot_order; /* ncols: columns by descending use; first nhot are hot */
     double *xhot;       /* nhot: packed before each SpMV */
     int nhot;
+    /* variant G: pattern slices + a sub-plan for the irregular rows */
+    int32_t *g_pat;     /* L offsets                                  */
+    int g_L;
+    uint32_t *g_masks;  /* nrows                                      */
+    int64_t g_nslices;
+    double *g_vals;     /* nslices * L * 64                           */
+    int32_t *g_irr;     /* nirr rows that are not pattern subsets     */
+    int64_t g_nirr;
+    SpB_Matrix g_sub;   /* CSR of the irregular rows (owned)          */
+    struct vnec_plan_s *g_subplan;
+    double *g_ysub;     /* nirr                                       */
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;
@@ -130,6 +141,16 @@ SpB_Info spb_matrix_upload(spb_csr *a);
 SpB_Info spb_vector_to_device(spb_dense *v);
 SpB_Info spb_vector_from_device(spb_dense *v);
 SpB_Info spb_vector_device_alloc(spb_dense *v);
+
+/* ---- variant G (spb_pattern.c) ---- */
+/* Dominant row pattern of a (host arrays): its offsets and length (0: no
+ * row qualifies), picked among evenly spaced sample rows. */
+int vnec_pattern_choose(const spb_csr *a, int32_t pat[VNEC_PATTERN_MAX]);
+/* Builds the pattern slices and the irregular-row sub-plan of p. */
+SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a);
+/* Rows of a whose columns are a subset of its dominant pattern (0 when
+ * there is none). */
+SpB_Info vnec_pattern_coverage(spb_csr *a, int64_t *nregular);
 
 /* ---- plans (spb_plan.c) ---- */
 /* x_dev / y_dev: doubles, or floats for an encoding-FP32 plan. */

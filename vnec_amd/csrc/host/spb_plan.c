@@ -16,6 +16,13 @@ static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
 static void plan_release(struct vnec_plan_s *p)
 {
     if (p->gexec) hipGraphExecDestroy(p->gexec);
+    if (p->g_subplan) vnec_plan_free_ext(&p->g_subplan);
+    if (p->g_sub) SpB_Matrix_free(&p->g_sub);
+    vnec_free_dev(p->g_pat);
+    vnec_free_dev(p->g_masks);
+    vnec_free_dev(p->g_vals);
+    vnec_free_dev(p->g_irr);
+    vnec_free_dev(p->g_ysub);
     vnec_free_dev(p->coords);
     vnec_free_dev(p->carry);
     vnec_free_dev(p->wg_row);
@@ -317,6 +324,25 @@ static SpB_Info build_hot(struct vnec_plan_s *p, const spb_csr *a)
     return rc;
 }
 
+/* Selection rule for variant G: at least 90 % of the rows fit the
+ * dominant pattern, its slots hold at least 80 % nonzeros (little zero
+ * padding), and the matrix is large enough (2^20 nonzeros) for the bytes
+ * saved to outweigh a possible sub-plan's extra launches. */
+#define G_MIN_NNZ (1 << 20)
+static int pattern_rule(spb_csr *a)
+{
+    if (a->base.element_type != SpB_FP64 || (int64_t)a->nnz < G_MIN_NNZ) return 0;
+    int64_t nreg = 0;
+    if (vnec_pattern_coverage(a, &nreg) != SpB_SUCCESS) {
+        vnec_clear_error(); /* no pattern plan: the selector decides */
+        return 0;
+    }
+    int32_t pat[VNEC_PATTERN_MAX];
+    int L = vnec_pattern_choose(a, pat);
+    double slots = (double)L * (double)a->base.row;
+    return (double)nreg >= 0.9 * (double)a->base.row && (double)a->nnz >= 0.8 * slots;
+}
+
 SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tuning *t)
 {
     if (!plan) return vnec_fail("vnec_plan_build_ext: NULL plan handle");
@@ -338,6 +364,14 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         tune = f;
     } else if (t && t->variant >= 0) {
         tune = *t;
+    } else if ((!t || ((t->variant == VNEC_SELECT_LEARNED || t->variant == VNEC_SELECT_RULE) &&
+                       t->encoding == VNEC_ENC_FP64 && t->nrhs <= 1)) &&
+               pattern_rule(a)) {
+        /* most rows follow one column-offset pattern: pattern slices move
+         * 8 bytes per regular nonzero, fewer than any column-indexed
+         * candidate */
+        vnec_tuning g = {VNEC_VAR_G, 8, 64, VNEC_ENC_FP64, 1, -1};
+        tune = g;
     } else {
         int nrhs = t && t->nrhs > 0 ? t->nrhs : 1;
         if (vnec_select_tuning(a, t ? t->variant : VNEC_SELECT_LEARNED, nrhs, &tune) != SpB_SUCCESS)
@@ -362,7 +396,11 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
         if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
     }
-    if (tune.variant == VNEC_VAR_H) {
+    if (tune.variant == VNEC_VAR_G) {
+        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant G stores FP64 values only");
+        if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
+        tune.grid = -1;
+    } else if (tune.variant == VNEC_VAR_H) {
         if (!vnec_hip_hot_supported(tune.ipt, tune.lanes))
             return vnec_fail("variant H: ipt 5 or 7 and lanes (hot entries) 4096 or 8192");
         if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant H stores FP64 values only");
@@ -406,6 +444,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         p->ntiles = 0;
         p->tune.grid = -1;
         rc = p->nrows > 0 ? build_slices(p, a) : SpB_SUCCESS;
+    } else if (tune.variant == VNEC_VAR_G) {
+        rc = vnec_pattern_build(p, a);
+        /* work units: pattern slices plus the sub-plan's tiles */
+        p->ntiles = p->g_nslices + (p->g_subplan ? p->g_subplan->ntiles : 0);
     } else if (p->ntiles > 0) {
         p->coords = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * 2 * (size_t)(p->ntiles + 1), &p->dev_bytes);
         if (tune.nrhs == 1 && tune.grid >= 0 && tune.variant != VNEC_VAR_D) {
@@ -587,6 +629,18 @@ static SpB_Info launch_d(struct vnec_plan_s *p, const void *x_dev, void *y_dev, 
 static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, const void *x_dev,
                             void *y_dev, hipStream_t s)
 {
+    if (p->tune.variant == VNEC_VAR_G) {
+        vnec_pattern_args g;
+        g.masks = p->g_masks;
+        g.gv = p->g_vals;
+        g.pat = p->g_pat;
+        g.L = p->g_L;
+        g.nrows = p->nrows;
+        g.nslices = p->g_nslices;
+        g.x = (const double *)x_dev;
+        g.y = (double *)y_dev;
+        return vnec_hip_ok(vnec_hip_spmv_pattern(&g, s), "pattern kernel");
+    }
     if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, x_dev, y_dev, s);
     if (p->tune.variant == VNEC_VAR_D) return launch_d(p, x_dev, y_dev, s);
     if (p->tune.variant == VNEC_VAR_H) {
@@ -600,9 +654,15 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, con
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
 }
 
-static SpB_Info launch_fixup(struct vnec_plan_s *p, void *y_dev, hipStream_t s)
+static SpB_Info launch_fixup(struct vnec_plan_s *p, const void *x_dev, void *y_dev, hipStream_t s)
 {
     if (p->tune.variant == VNEC_VAR_S) return SpB_SUCCESS; /* row-aligned tiles: no carries */
+    if (p->tune.variant == VNEC_VAR_G) { /* the irregular rows */
+        if (!p->g_nirr) return SpB_SUCCESS;
+        if (vnec_plan_run(p->g_subplan, x_dev, p->g_ysub) != SpB_SUCCESS) return SpB_FAILURE;
+        return vnec_hip_ok(vnec_hip_scatter_rows(p->g_ysub, p->g_irr, p->g_nirr, (double *)y_dev, s),
+                           "scatter rows");
+    }
     if (p->tune.encoding == VNEC_ENC_FP32)
         return vnec_hip_ok(vnec_hip_carry_fixup_f32(p->coords, (const float *)p->carry, p->ntiles,
                                                     (float *)y_dev, p->nrows, s), "carry fix-up");
@@ -625,7 +685,7 @@ SpB_Info vnec_plan_run(struct vnec_plan_s *p, const void *x_dev, void *y_dev)
     vnec_merge_args m;
     fill_args(p, x_dev, y_dev, &m);
     if (launch_main(p, &m, x_dev, y_dev, s) != SpB_SUCCESS) return SpB_FAILURE;
-    return launch_fixup(p, y_dev, s);
+    return launch_fixup(p, x_dev, y_dev, s);
 }
 
 SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev)
@@ -763,7 +823,7 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, i
         rc = vnec_hip_ok(hipEventRecord(ev[0], s), "record");
         if (rc == SpB_SUCCESS) rc = launch_main(plan, &m, x_dev, y_dev, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[1], s), "record");
-        if (rc == SpB_SUCCESS) rc = launch_fixup(plan, y_dev, s);
+        if (rc == SpB_SUCCESS) rc = launch_fixup(plan, x_dev, y_dev, s);
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[2], s), "record");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(ev[2]), "sync");
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&a, ev[0], ev[1]), "elapsed");

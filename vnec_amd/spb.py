@@ -20,7 +20,7 @@ SpB_FP32 = 10
 SpB_FP64 = 11
 SpB_MATRIX_FORMAT = 1
 SpB_CSR = 1
-VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4}
+VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4, "G": 5}
 VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2, "fp32": 3}
 # Selection modes (negative vnec_tuning.variant values).
 SELECT = {"learned": L.SELECT_LEARNED, "autotune": L.SELECT_AUTOTUNE, "rule": L.SELECT_RULE,
@@ -401,6 +401,26 @@ def read_graph(path: str) -> tuple[int, np.ndarray, np.ndarray]:
         lib.vnec_host_free_ext(pp)
         lib.vnec_host_free_ext(pc)
     return n.value, ptr, col
+
+
+def pattern(A: Matrix) -> tuple[np.ndarray, int]:
+    """Variant G's dominant row pattern (column offsets) and the number of
+    rows whose columns are a subset of it (vnec_pattern_ext)."""
+    pat = np.zeros(32, np.int32)
+    L_, nreg = C.c_int(), C.c_int64()
+    L.check(L.load().vnec_pattern_ext(A.h, _p(pat), C.byref(L_), C.byref(nreg)), "vnec_pattern_ext")
+    return pat[:L_.value].copy(), nreg.value
+
+
+def pattern_match(A: Matrix, pat) -> tuple[np.ndarray, np.ndarray]:
+    """(masks, flags) per row: the pattern positions the row holds, and 1
+    where the row is not a subset of the pattern (GPU)."""
+    pat = np.ascontiguousarray(pat, np.int32)
+    masks = np.zeros(A.nrows, np.uint32)
+    flags = np.zeros(A.nrows, np.int32)
+    L.check(L.load().vnec_pattern_match_ext(A.h, _p(pat), len(pat), _p(masks), _p(flags)),
+            "vnec_pattern_match_ext")
+    return masks, flags
 
 
 def matrix_from_file(path: str, values: str = "row", dtype=np.float64) -> Matrix:
