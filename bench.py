@@ -35,7 +35,8 @@ HBM_PEAK_GBS = 8000.0
 # dominant kernel of each plan shape: (variant, persistent schedule?)
 KERNEL_NAME = {("S", False): "k_spmv_slice", ("L", True): "k_spmv_stream",
                ("E", True): "k_spmv_estream", ("L", False): "k_spmv_merge1",
-               ("E", False): "k_spmv_merge1"}
+               ("E", False): "k_spmv_merge1", ("G", False): "k_spmv_pattern",
+               ("D", False): "k_spmv_d", ("H", True): "k_spmv_hot4"}
 PLAIN_CANDIDATE = 4  # L, 7 items per thread, persistent: the plain merge-path CSR kernel
 METRIC = "FP64 SpMV GB/s as % of HBM peak; learned-vs-plain speedup at equal error"
 R18_FLUSH = 1 << 29  # 512 MiB scribbled between SpMVs for an HBM (not cache) figure
@@ -297,7 +298,7 @@ def main():
     variants = {}
     variant_y = {}
     if args.workload == "stencil" and not args.no_variants:
-        for name, var, ipt, enc, vb in (("S3_fp64", "S", 3, "fp64", 8),
+        for name, var, ipt, enc, vb in (("G_fp64", "G", 8, "fp64", 8), ("S3_fp64", "S", 3, "fp64", 8),
                                         ("S5_fp64", "S", 5, "fp64", 8),
                                         ("S3_hilo", "S", 3, "hilo", 8),
                                         ("S3_hi", "S", 3, "hi", 4), ("L7_fp64", "L", 7, "fp64", 8),
@@ -315,6 +316,16 @@ def main():
 
     cp = C.c_double()  # measured streaming-copy ceiling (also the counter calibration)
     spb.L.check(lib.vnec_copy_bandwidth_ext(1 << 30, 10, C.byref(cp)), "copy bandwidth")
+    ceiling = None
+    if k == 1:
+        # what this matrix's traffic costs with no row work: its values and
+        # columns streamed, then with one x gather per nonzero
+        st, ga = C.c_double(), C.c_double()
+        spb.L.check(lib.vnec_gather_ceiling_ext(A.h, dx.ptr, 50, C.byref(st), C.byref(ga)),
+                    "gather ceiling")
+        ceiling = {"stream_only_us": round(st.value * 1e3, 2), "stream_gather_us": round(ga.value * 1e3, 2),
+                   "stream_gather_GBps": round(B / (ga.value / 1e3) / 1e9, 1),
+                   "kernel_pct_of_stream_gather": round(100.0 * achieved / (B / (ga.value / 1e3) / 1e9), 1)}
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
@@ -359,6 +370,7 @@ def main():
         "host_vectors_GBps": round(B / (host_ms / 1e3) / 1e9, 1),
         "copy_peak_gbs": round(cp.value, 1),
         "pct_of_copy_peak": round(100.0 * achieved / cp.value, 2),
+        "gather_ceiling": ceiling,
         "plan_build_ms": round(build_ms, 1),
         **extra,
         "variants": variants,

@@ -148,6 +148,12 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, i
 /* Measured streaming-copy bandwidth: bytes read + bytes written per
  * second over iters copies of a `bytes`-sized buffer (GB/s). */
 SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps);
+/* Measured ceilings of A's SpMV traffic without row work (a balanced
+ * persistent grid over its nonzeros): mean ms to stream its FP64 values
+ * and int32 columns, and to stream them and gather x_dev[col] for every
+ * nonzero (iters launches each, HIP events). */
+SpB_Info vnec_gather_ceiling_ext(const SpB_Matrix A, const double *x_dev, int iters, double *stream_ms,
+                                 double *gather_ms);
 
 /* ------------------------------------------------- device memory helpers */
 void *vnec_dev_alloc_ext(size_t bytes);

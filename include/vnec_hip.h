@@ -144,6 +144,12 @@ int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, dou
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 
+/* Gather-ceiling probe over a matrix's values and columns (nnz): with
+ * gather = 0 it streams them, with gather = 1 it also gathers x[col] per
+ * nonzero; no row structure; out holds grid * 256 partial sums. */
+int vnec_hip_gather_probe(const double *val, const int32_t *col, int64_t nnz, const double *x,
+                          int gather, double *out, int grid, void *stream);
+
 /* Streaming copy of bytes (multiple of 16) from src to dst. */
 int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
 

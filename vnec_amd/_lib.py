@@ -171,6 +171,7 @@ SIGNATURES = {
     "VNEC_S_free_fp64_ext": (Info, [vp]),
     "VNEC_D_free_fp32_ext": (Info, [vp]),
     "vnec_pattern_ext": (Info, [vp, vp, C.POINTER(C.c_int), C.POINTER(i64)]),
+    "vnec_gather_ceiling_ext": (Info, [vp, vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vnec_pattern_match_ext": (Info, [vp, vp, C.c_int, vp, vp]),
     "VNEC_S_free_fp32_ext": (Info, [vp]),
     "VNEC_L_free_ext": (Info, [vp]),
@@ -199,6 +200,7 @@ SIGNATURES = {
     "vnec_hip_spmv_slice": (C.c_int, [vp, vp]),
     "vnec_hip_scribble": (C.c_int, [vp, C.c_size_t, vp]),
     "vnec_hip_zero_f64": (C.c_int, [vp, i64, vp]),
+    "vnec_hip_gather_probe": (C.c_int, [vp, vp, i64, vp, C.c_int, vp, C.c_int, vp]),
     "vnec_hip_pattern_match": (C.c_int, [vp, vp, i64, vp, C.c_int, vp, vp, vp]),
     "vnec_hip_pattern_split": (C.c_int, [vp, vp, i64, vp, vp]),
     "vnec_hip_pattern_fill": (C.c_int, [vp, vp, vp, i64, C.c_int, vp, vp]),

@@ -499,3 +499,48 @@ extern "C" int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, vo
                        (const u32x4 *)src, (u32x4 *)dst, n16);
     return (int)hipGetLastError();
 }
+
+// ----------------------------------------------------- gather ceiling
+// What one SpMV's memory traffic costs without any row work: stream a
+// matrix's values and columns (and, with GATHER, gather x[col] for each
+// nonzero) over a persistent grid, 8 nonzeros per thread per chunk, one
+// partial sum per thread written.  Scattered patterns such as R-MAT are
+// bound by this gather rate rather than by HBM.
+template <bool GATHER>
+__global__ __launch_bounds__(256) void k_gather_probe(const double *__restrict__ val,
+                                                      const int32_t *__restrict__ col, int64_t nnz,
+                                                      const double *__restrict__ x,
+                                                      double *__restrict__ out)
+{
+    constexpr int U = 8;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t nchunks = (nnz + 256 * U - 1) / (256 * U);
+    double s = 0.0;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int64_t base = (ch * 4 + w) * 64 * U + lane;
+        double v[U];
+        int c[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t j = base + 64 * u;
+            v[u] = j < nnz ? __builtin_nontemporal_load(val + j) : 0.0;
+            c[u] = j < nnz ? __builtin_nontemporal_load(col + j) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) s += v[u] * (GATHER ? x[c[u]] : (double)c[u]);
+    }
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+extern "C" int vnec_hip_gather_probe(const double *val, const int32_t *col, int64_t nnz, const double *x,
+                                     int gather, double *out, int grid, void *stream)
+{
+    if (grid < 1) return (int)hipErrorInvalidValue;
+    if (gather)
+        hipLaunchKernelGGL(k_gather_probe<true>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+                           val, col, nnz, x, out);
+    else
+        hipLaunchKernelGGL(k_gather_probe<false>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+                           val, col, nnz, x, out);
+    return (int)hipGetLastError();
+}

@@ -881,3 +881,37 @@ SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps)
     vnec_free_dev(b);
     return rc;
 }
+
+#define GATHER_PROBE_GRID 2048
+SpB_Info vnec_gather_ceiling_ext(const SpB_Matrix A, const double *x_dev, int iters, double *stream_ms,
+                                 double *gather_ms)
+{
+    spb_csr *a = spb_as_csr(A);
+    if (!a || !x_dev || iters < 1 || !stream_ms || !gather_ms)
+        return vnec_fail("vnec_gather_ceiling_ext: bad arguments");
+    if (a->base.element_type != SpB_FP64) return vnec_fail("vnec_gather_ceiling_ext: FP64 matrices only");
+    if (spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    hipStream_t s = vnec_stream();
+    double *out = (double *)vnec_malloc_dev(sizeof(double) * GATHER_PROBE_GRID * 256, NULL);
+    hipEvent_t e0 = NULL, e1 = NULL;
+    SpB_Info rc = out ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e0), "event");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e1), "event");
+    for (int gather = 0; gather < 2 && rc == SpB_SUCCESS; gather++) {
+        rc = vnec_hip_ok(vnec_hip_gather_probe(a->d_val, a->d_col, (int64_t)a->nnz, x_dev, gather, out,
+                                               GATHER_PROBE_GRID, s), "probe"); /* warm-up */
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
+        for (int i = 0; i < iters && rc == SpB_SUCCESS; i++)
+            rc = vnec_hip_ok(vnec_hip_gather_probe(a->d_val, a->d_col, (int64_t)a->nnz, x_dev, gather, out,
+                                                   GATHER_PROBE_GRID, s), "probe");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e1, s), "record");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(e1), "sync");
+        float ms = 0.f;
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (rc == SpB_SUCCESS) *(gather ? gather_ms : stream_ms) = ms / iters;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    vnec_free_dev(out);
+    return rc;
+}
