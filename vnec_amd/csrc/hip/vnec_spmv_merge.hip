@@ -203,24 +203,34 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeK(vnec_merge_args a, i
 }
 
 // Deterministic fix-up: the carry of tile t belongs to row coords[t+1].x.
-// The first tile of each run of tiles carrying into one row sums the run
-// in tile order and adds it once (the reference's parallel `+=` races when
-// two partitions carry into one row, spmv_vnec_L_for_mxv.c:160-164).
+// One wavefront per tile boundary; the wavefront of the first tile of a run
+// of tiles carrying into one row sums the run (lanes stride through it, a
+// fixed butterfly joins them) and adds it once (the reference's parallel
+// `+=` races when two partitions carry into one row,
+// spmv_vnec_L_for_mxv.c:160-164).  Runs are long on rows of hundreds of
+// thousands of nonzeros, so the run is read by 64 lanes at a time.
 // Trailing tiles can be empty (ipt = ceil(M / P) may cover M early); their
 // carries belong to row nrows, which does not exist, and are zero.
 template <typename F>
 __global__ void k_carry_fixup(const int2 *__restrict__ co, const F *__restrict__ carry,
                               int64_t ntiles, F *__restrict__ y, int64_t nrows, int K)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VNEC_WAVE;
     if (t >= ntiles - 1) return;
-    int row = co[t + 1].x;
+    const int row = co[t + 1].x;
     if (row >= nrows) return;
     if (t > 0 && co[t].x == row) return;  // not the first of its run
+    const int lane = lane_id();
     for (int c = 0; c < K; c++) {
         F s = 0;
-        for (int64_t u = t; u < ntiles - 1 && co[u + 1].x == row; u++) s += carry[u * K + c];
-        y[(int64_t)row * K + c] += s;
+        for (int64_t base = t;; base += VNEC_WAVE) {
+            const int64_t u = base + lane;
+            const bool in = u < ntiles - 1 && co[u + 1].x == row;
+            if (in) s += carry[u * K + c];
+            if (__ballot(!in)) break;  // the run is contiguous: it ends in this batch
+        }
+        for (int o = VNEC_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VNEC_WAVE);
+        if (lane == 0) y[(int64_t)row * K + c] += s;
     }
 }
 
@@ -285,8 +295,8 @@ extern "C" int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, 
                                     double *y, int64_t nrows, int nrhs, void *stream)
 {
     if (ntiles <= 1) return 0;
-    int64_t grid = (ntiles - 1 + 255) / 256;
-    hipLaunchKernelGGL(k_carry_fixup<double>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+    int64_t grid = (ntiles - 1 + VNEC_NWAVES - 1) / VNEC_NWAVES; /* one wavefront per boundary */
+    hipLaunchKernelGGL(k_carry_fixup<double>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, nrhs);
     return (int)hipGetLastError();
 }
@@ -295,8 +305,8 @@ extern "C" int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carr
                                         float *y, int64_t nrows, void *stream)
 {
     if (ntiles <= 1) return 0;
-    int64_t grid = (ntiles - 1 + 255) / 256;
-    hipLaunchKernelGGL(k_carry_fixup<float>, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream,
+    int64_t grid = (ntiles - 1 + VNEC_NWAVES - 1) / VNEC_NWAVES;
+    hipLaunchKernelGGL(k_carry_fixup<float>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, 1);
     return (int)hipGetLastError();
 }
