@@ -203,34 +203,53 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeK(vnec_merge_args a, i
 }
 
 // Deterministic fix-up: the carry of tile t belongs to row coords[t+1].x.
-// One wavefront per tile boundary; the wavefront of the first tile of a run
-// of tiles carrying into one row sums the run (lanes stride through it, a
-// fixed butterfly joins them) and adds it once (the reference's parallel
-// `+=` races when two partitions carry into one row,
-// spmv_vnec_L_for_mxv.c:160-164).  Runs are long on rows of hundreds of
-// thousands of nonzeros, so the run is read by 64 lanes at a time.
+// One thread per tile boundary; the thread of the first tile of a run of
+// tiles carrying into one row sums the run in tile order and adds it once
+// (the reference's parallel `+=` races when two partitions carry into one
+// row, spmv_vnec_L_for_mxv.c:160-164).  Runs longer than FIX_SHORT tiles
+// (rows of hundreds of thousands of nonzeros) are summed by the whole
+// wavefront instead, 64 carries at a time joined by a fixed butterfly.
 // Trailing tiles can be empty (ipt = ceil(M / P) may cover M early); their
 // carries belong to row nrows, which does not exist, and are zero.
+#define FIX_SHORT 8
 template <typename F>
-__global__ void k_carry_fixup(const int2 *__restrict__ co, const F *__restrict__ carry,
-                              int64_t ntiles, F *__restrict__ y, int64_t nrows, int K)
+__global__ __launch_bounds__(VNEC_BLOCK) void k_carry_fixup(const int2 *__restrict__ co,
+                                                            const F *__restrict__ carry, int64_t ntiles,
+                                                            F *__restrict__ y, int64_t nrows, int K)
 {
-    const int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / VNEC_WAVE;
-    if (t >= ntiles - 1) return;
-    const int row = co[t + 1].x;
-    if (row >= nrows) return;
-    if (t > 0 && co[t].x == row) return;  // not the first of its run
-    const int lane = lane_id();
-    for (int c = 0; c < K; c++) {
-        F s = 0;
-        for (int64_t base = t;; base += VNEC_WAVE) {
-            const int64_t u = base + lane;
-            const bool in = u < ntiles - 1 && co[u + 1].x == row;
-            if (in) s += carry[u * K + c];
-            if (__ballot(!in)) break;  // the run is contiguous: it ends in this batch
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = ntiles - 1;  // boundaries
+    int row = -1;
+    bool head = false, longrun = false;
+    if (t < nb) {
+        row = co[t + 1].x;
+        head = row < nrows && (t == 0 || co[t].x != row);
+        // runs are contiguous: longer than FIX_SHORT iff tile t + FIX_SHORT still carries into row
+        longrun = head && t + FIX_SHORT < nb && co[t + FIX_SHORT + 1].x == row;
+    }
+    if (head && !longrun) {
+        for (int c = 0; c < K; c++) {
+            F s = 0;
+            for (int64_t u = t; u < nb && co[u + 1].x == row; u++) s += carry[u * K + c];
+            y[(int64_t)row * K + c] += s;
         }
-        for (int o = VNEC_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VNEC_WAVE);
-        if (lane == 0) y[(int64_t)row * K + c] += s;
+    }
+    const int lane = lane_id();
+    for (uint64_t m = __ballot(longrun); m; m &= m - 1) {
+        const int src = __builtin_ctzll(m);
+        const int r = __shfl(row, src, VNEC_WAVE);
+        const int64_t tr = (t & ~(int64_t)(VNEC_WAVE - 1)) + src;  // lane src's boundary
+        for (int c = 0; c < K; c++) {
+            F s = 0;
+            for (int64_t base = tr;; base += VNEC_WAVE) {
+                const int64_t u = base + lane;
+                const bool in = u < nb && co[u + 1].x == r;
+                if (in) s += carry[u * K + c];
+                if (__ballot(!in)) break;  // the run ends in this batch
+            }
+            for (int o = VNEC_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VNEC_WAVE);
+            if (lane == 0) y[(int64_t)r * K + c] += s;
+        }
     }
 }
 
@@ -295,7 +314,7 @@ extern "C" int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, 
                                     double *y, int64_t nrows, int nrhs, void *stream)
 {
     if (ntiles <= 1) return 0;
-    int64_t grid = (ntiles - 1 + VNEC_NWAVES - 1) / VNEC_NWAVES; /* one wavefront per boundary */
+    int64_t grid = (ntiles - 1 + VNEC_BLOCK - 1) / VNEC_BLOCK; /* one thread per boundary */
     hipLaunchKernelGGL(k_carry_fixup<double>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, nrhs);
     return (int)hipGetLastError();
@@ -305,7 +324,7 @@ extern "C" int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carr
                                         float *y, int64_t nrows, void *stream)
 {
     if (ntiles <= 1) return 0;
-    int64_t grid = (ntiles - 1 + VNEC_NWAVES - 1) / VNEC_NWAVES;
+    int64_t grid = (ntiles - 1 + VNEC_BLOCK - 1) / VNEC_BLOCK;
     hipLaunchKernelGGL(k_carry_fixup<float>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, 1);
     return (int)hipGetLastError();
