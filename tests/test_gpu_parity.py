@@ -144,6 +144,14 @@ def test_encodings_error_bounds(variant, enc):
         assert worst > 1e-12  # hi-only really drops the low bits
 
 
+def _spmm_ref(m, X, k):
+    Y0 = O.spmm_serial(m.ptr, m.col, m.val, X, k)
+    ax = np.empty(m.nrows * k)
+    for c in range(k):
+        ax[c::k] = O.spmv_abs(m.ptr, m.col, m.val, X[c::k])[0]
+    return Y0, ax
+
+
 @pytest.mark.parametrize("k", [2, 4, 8, 16])
 def test_multi_rhs(k):
     m = g.rmat(11, 16, seed=31)
@@ -151,11 +159,64 @@ def test_multi_rhs(k):
     X = rng.uniform(-1, 1, m.ncols * k)
     A = _mat(m)
     Y = _run_plan(A, X, "L", 5, nrhs=k)
-    Y0 = O.spmm_serial(m.ptr, m.col, m.val, X, k)
-    ax = np.empty(m.nrows * k)
-    for c in range(k):
-        ax[c::k] = O.spmv_abs(m.ptr, m.col, m.val, X[c::k])[0]
+    Y0, ax = _spmm_ref(m, X, k)
     assert O.check_rows(Y, Y0, ax, TOL)[0]
+
+
+def _wide_row(n=20000, seed=33):
+    """Rows of 0-6 entries and one full row of n: the full row spans 11-26
+    tiles of 768-1792 items, beyond the fix-up's 8-tile runs."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 7, n)
+    lens[n // 2] = n
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    col = np.concatenate([np.arange(n) if l == n else np.sort(rng.choice(n, l, replace=False))
+                          for l in lens]).astype(np.int32)
+    return g.from_pattern(n, n, ptr, col, seed=seed)
+
+
+@pytest.mark.parametrize("k", [4, 8, 16])
+@pytest.mark.parametrize("ipt", [3, 7])
+@pytest.mark.parametrize("name", ["widerow", "denserow", "powerlaw"])
+def test_multi_rhs_long_rows(name, ipt, k):
+    """Rows spanning several tiles, with k right-hand sides: runs the fix-up
+    sums per boundary thread (up to 8 tiles) and by the whole wavefront."""
+    m = _wide_row() if name == "widerow" else dict(CASES)[name]
+    X = np.random.default_rng(100 + k).uniform(-1, 1, m.ncols * k)
+    Y = _run_plan(_mat(m), X, "L", ipt, nrhs=k)
+    Y0, ax = _spmm_ref(m, X, k)
+    ok, worst = O.check_rows(Y, Y0, ax, TOL)
+    assert ok, worst
+
+
+@pytest.mark.parametrize("k", [4, 8, 16])
+def test_multi_rhs_unaligned_device_vectors(k):
+    """x and y only 8-byte aligned (device pointers from the caller): the
+    one-lane-per-nonzero kernel runs instead of the 16-byte grouped one;
+    both meet the gate and repeat bit for bit."""
+    m = g.rmat(12, 16, seed=32)
+    X = np.random.default_rng(k).uniform(-1, 1, m.ncols * k)
+    Y0, ax = _spmm_ref(m, X, k)
+    p = spb.Plan(_mat(m), "L", 5, nrhs=k)
+    out = []
+    for off in (0, 8):
+        dx = spb.DeviceArray(8 * m.ncols * k + 16)
+        dy = spb.DeviceArray(8 * m.nrows * k + 16)
+        L = spb.L.load()
+        spb.L.check(L.vnec_memcpy_h2d_ext(dx.ptr + off, X.ctypes.data, X.nbytes), "h2d")
+        p.spmv_dev(dx.ptr + off, dy.ptr + off)
+        Y = np.empty(m.nrows * k)
+        spb.L.check(L.vnec_memcpy_d2h_ext(Y.ctypes.data, dy.ptr + off, Y.nbytes), "d2h")
+        ok, worst = O.check_rows(Y, Y0, ax, TOL)
+        assert ok, (off, worst)
+        p.spmv_dev(dx.ptr + off, dy.ptr + off)
+        Y2 = np.empty_like(Y)
+        spb.L.check(L.vnec_memcpy_d2h_ext(Y2.ctypes.data, dy.ptr + off, Y2.nbytes), "d2h")
+        assert np.array_equal(Y, Y2)
+        out.append(Y)
+        dx.free()
+        dy.free()
+    p.free()
 
 
 def test_known_answer_is_exact():

@@ -17,6 +17,7 @@
 //   5. the tile's partial last row is left in carry[t]; a second kernel
 //      adds the carries to y in tile order (deterministic fix-up).
 #include <hip/hip_runtime.h>
+#include <climits>
 
 #include "vnec_common.h"
 #include "vnec_hip.h"
@@ -202,23 +203,211 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeK(vnec_merge_args a, i
     }
 }
 
+// VW consecutive right-hand-side entries of one x or y row, held by a lane
+// (VW = 2: one 16-byte load or store).
+template <int VW> struct rhs_vec { double e[VW]; };
+template <int VW> __device__ __forceinline__ rhs_vec<VW> load_rhs(const double *p);
+template <> __device__ __forceinline__ rhs_vec<2> load_rhs<2>(const double *p)
+{
+    const double2 d = *(const double2 *)p;
+    return {{d.x, d.y}};
+}
+__device__ __forceinline__ void store_rhs(double *p, const double (&v)[2])
+{
+    *(double2 *)p = make_double2(v[0], v[1]);
+}
+
+// Segmented scan over merge-path streams of Q lanes each (lane q of a
+// stream holds columns [q VW, q VW + VW) of the stream's (flag, tail)):
+// Hillis-Steele steps of Q, 2Q, ... lanes within the wavefront, then the
+// wavefronts chained in order through LDS.  cin = carry into the stream's
+// head row, incl = the stream's inclusive value (the last stream's closes
+// the tile).  s_v: VNEC_NWAVES * Q * VW doubles, s_f: VNEC_NWAVES ints.
+template <int Q, int VW>
+__device__ __forceinline__ void stream_seg_carry(int flag, const double (&tail)[VW], int *s_f, double *s_v,
+                                                 double (&cin)[VW], double (&incl)[VW])
+{
+    constexpr int K = Q * VW;
+    const int lane = lane_id(), w = wave_id(), q = lane % Q;
+    int f = flag;
+    double v[VW];
+#pragma unroll
+    for (int e = 0; e < VW; e++) v[e] = tail[e];
+#pragma unroll
+    for (int off = Q; off < VNEC_WAVE; off <<= 1) {
+        const int fu = __shfl_up(f, off, VNEC_WAVE);
+        double vu[VW];
+#pragma unroll
+        for (int e = 0; e < VW; e++) vu[e] = __shfl_up(v[e], off, VNEC_WAVE);
+        if (lane >= off) {
+            if (!f) {
+#pragma unroll
+                for (int e = 0; e < VW; e++) v[e] = vu[e] + v[e];
+            }
+            f |= fu;
+        }
+    }
+    if (lane >= VNEC_WAVE - Q) {
+#pragma unroll
+        for (int e = 0; e < VW; e++) s_v[w * K + q * VW + e] = v[e];
+        if (q == 0) s_f[w] = f;
+    }
+    __syncthreads();
+    if (threadIdx.x < Q) {
+        double c[VW];
+#pragma unroll
+        for (int e = 0; e < VW; e++) c[e] = 0.0;
+        for (int k = 0; k < VNEC_NWAVES; k++) {
+            const int fw = s_f[k];
+#pragma unroll
+            for (int e = 0; e < VW; e++) {
+                const double tw = s_v[k * K + q * VW + e];
+                s_v[k * K + q * VW + e] = c[e];  // carry into wavefront k
+                c[e] = fw ? tw : c[e] + tw;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < VW; e++) {
+        const double cw = s_v[w * K + q * VW + e];
+        incl[e] = f ? v[e] : v[e] + cw;
+        const double prev = __shfl_up(incl[e], Q, VNEC_WAVE);
+        cin[e] = lane < Q ? cw : prev;
+    }
+}
+
+// Multi-RHS (variant L), grouped lanes: x is [ncols][K], y is [nrows][K].
+// Values, columns and row ends are staged in LDS as in k_spmv_mergeK; the
+// tile's merge items are then cut into VNEC_BLOCK / Q streams of Q IPT
+// items, each walked by a group of Q = K / VW lanes, lane q summing columns
+// [q VW, q VW + VW).  A nonzero's K contiguous x entries are thus read by
+// one load instruction of the group (8 K bytes in one or two cache lines)
+// where k_spmv_mergeK has each lane load its own row, every instruction
+// touching 64 lines; rows are written the same way.  The walk decides U
+// items (LDS reads only) before issuing their U gathers.  x, y and the
+// carries must be 16-byte aligned.
+template <int IPT, int K, int ENC, int VW>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_mergeKg(vnec_merge_args a, int64_t grid)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    constexpr int Q = K / VW;   // lanes per stream
+    constexpr int W = Q * IPT;  // merge items per stream
+    constexpr int U = 4;        // items decided before their gathers are issued
+    __shared__ double s_val[T];
+    __shared__ int s_col[T];
+    __shared__ int s_rowend[T];
+    __shared__ int s_f[VNEC_NWAVES];
+    __shared__ double s_v[VNEC_NWAVES * K];
+
+    const int64_t t = xcd_tile(blockIdx.x, grid);
+    if (t >= a.ntiles) return;
+    const int2 *co = (const int2 *)a.coords;
+    const int2 c0 = co[t], c1 = co[t + 1];
+    const int x0 = c0.x, y0 = c0.y;
+    const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+    const int tid = threadIdx.x;
+
+    for (int i = tid; i < nrow; i += VNEC_BLOCK) s_rowend[i] = a.ptr[x0 + 1 + i];
+#pragma unroll
+    for (int u = 0; u < IPT; u++) {
+        int k = u * VNEC_BLOCK + tid;
+        if (k < nz) {
+            int64_t j = (int64_t)y0 + k;
+            s_val[k] = load_value<ENC>(a, j);
+            s_col[k] = __builtin_nontemporal_load(a.col + j);
+        }
+    }
+    __syncthreads();
+
+    const int q = tid % Q, sid = tid / Q;
+    const int items = nrow + nz;
+    const int d0 = sid * W < items ? sid * W : items;
+    const int d1 = d0 + W < items ? d0 + W : items;
+    int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+    int ny = d0 - rx;
+    int nend = rx < nrow ? s_rowend[rx] : INT_MAX;  // next row end, as a nonzero index
+    const double *__restrict__ xq = a.x + q * VW;
+    double acc[VW], head[VW];
+#pragma unroll
+    for (int e = 0; e < VW; e++) acc[e] = head[e] = 0.0;
+    int flag = 0, head_row = 0;
+    for (int d = d0; d < d1; d += U) {
+        int cc[U], er[U];  // er: -2 past the stream, -1 a nonzero, else the row it ends
+        double vv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            er[u] = -2;
+            cc[u] = 0;
+            vv[u] = 0.0;
+            if (d + u < d1) {
+                if (nend <= y0 + ny) {
+                    er[u] = rx++;
+                    nend = rx < nrow ? s_rowend[rx] : INT_MAX;
+                } else {
+                    er[u] = -1;
+                    cc[u] = s_col[ny];
+                    vv[u] = s_val[ny];
+                    ny++;
+                }
+            }
+        }
+        rhs_vec<VW> xv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (er[u] == -1) xv[u] = load_rhs<VW>(xq + (int64_t)cc[u] * K);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (er[u] == -1) {
+#pragma unroll
+                for (int e = 0; e < VW; e++) acc[e] += vv[u] * xv[u].e[e];
+            } else if (er[u] >= 0) {
+                if (!flag) {
+                    head_row = x0 + er[u];
+#pragma unroll
+                    for (int e = 0; e < VW; e++) head[e] = acc[e];
+                    flag = 1;
+                } else {
+                    store_rhs(a.y + (int64_t)(x0 + er[u]) * K + q * VW, acc);
+                }
+#pragma unroll
+                for (int e = 0; e < VW; e++) acc[e] = 0.0;
+            }
+        }
+    }
+    double cin[VW], incl[VW];
+    stream_seg_carry<Q, VW>(flag, acc, s_f, s_v, cin, incl);
+    if (flag) {
+#pragma unroll
+        for (int e = 0; e < VW; e++) head[e] += cin[e];
+        store_rhs(a.y + (int64_t)head_row * K + q * VW, head);
+    }
+    if (sid == VNEC_BLOCK / Q - 1) store_rhs(a.carry + t * K + q * VW, incl);
+}
+
 // Deterministic fix-up: the carry of tile t belongs to row coords[t+1].x.
-// One thread per tile boundary; the thread of the first tile of a run of
-// tiles carrying into one row sums the run in tile order and adds it once
+// One thread per tile boundary and right-hand side: the K threads of a
+// boundary are adjacent lanes, so its K carries and y entries are read and
+// written as one contiguous piece.  The threads of the first tile of a run
+// of tiles carrying into one row sum the run in tile order and add it once
 // (the reference's parallel `+=` races when two partitions carry into one
 // row, spmv_vnec_L_for_mxv.c:160-164).  Runs longer than FIX_SHORT tiles
 // (rows of hundreds of thousands of nonzeros) are summed by the whole
 // wavefront instead, 64 carries at a time joined by a fixed butterfly.
 // Trailing tiles can be empty (ipt = ceil(M / P) may cover M early); their
 // carries belong to row nrows, which does not exist, and are zero.
+// KT > 0 fixes K at compile time; KT = 0 takes it from Kr.
 #define FIX_SHORT 8
-template <typename F>
+template <typename F, int KT>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_carry_fixup(const int2 *__restrict__ co,
                                                             const F *__restrict__ carry, int64_t ntiles,
-                                                            F *__restrict__ y, int64_t nrows, int K)
+                                                            F *__restrict__ y, int64_t nrows, int Kr)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nb = ntiles - 1;  // boundaries
+    const int K = KT > 0 ? KT : Kr;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = g / K;        // boundary between tiles t and t + 1
+    const int c = (int)(g - t * K);  // right-hand side
+    const int64_t nb = ntiles - 1;
     int row = -1;
     bool head = false, longrun = false;
     if (t < nb) {
@@ -228,27 +417,27 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_carry_fixup(const int2 *__restri
         longrun = head && t + FIX_SHORT < nb && co[t + FIX_SHORT + 1].x == row;
     }
     if (head && !longrun) {
-        for (int c = 0; c < K; c++) {
-            F s = 0;
-            for (int64_t u = t; u < nb && co[u + 1].x == row; u++) s += carry[u * K + c];
-            y[(int64_t)row * K + c] += s;
-        }
+        int n = 1;  // tiles in the run, at most FIX_SHORT
+        while (t + n < nb && co[t + n + 1].x == row) n++;
+        F s = 0;
+        for (int i = 0; i < n; i++) s += carry[(t + i) * K + c];
+        y[(int64_t)row * K + c] += s;
     }
     const int lane = lane_id();
-    for (uint64_t m = __ballot(longrun); m; m &= m - 1) {
+    for (uint64_t m = __ballot(longrun && c == 0); m; m &= m - 1) {
         const int src = __builtin_ctzll(m);
         const int r = __shfl(row, src, VNEC_WAVE);
-        const int64_t tr = (t & ~(int64_t)(VNEC_WAVE - 1)) + src;  // lane src's boundary
-        for (int c = 0; c < K; c++) {
+        const int64_t tr = (g - lane + src) / K;  // lane src's boundary
+        for (int cc = 0; cc < K; cc++) {
             F s = 0;
             for (int64_t base = tr;; base += VNEC_WAVE) {
                 const int64_t u = base + lane;
                 const bool in = u < nb && co[u + 1].x == r;
-                if (in) s += carry[u * K + c];
+                if (in) s += carry[u * K + cc];
                 if (__ballot(!in)) break;  // the run ends in this batch
             }
             for (int o = VNEC_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, VNEC_WAVE);
-            if (lane == 0) y[(int64_t)r * K + c] += s;
+            if (lane == 0) y[(int64_t)r * K + cc] += s;
         }
     }
 }
@@ -272,10 +461,15 @@ static merge_kernel_t pick1(int ipt, bool win, int enc)
 #undef CASE1
 }
 
-static merge_kernel_t pickK(int ipt, int k, int enc)
+// k >= 4 with 16-byte aligned x and y: grouped lanes, two right-hand sides
+// per lane; otherwise (k = 2, where one lane per nonzero is faster, or
+// caller pointers that are only 8-byte aligned) the one-lane kernel.
+static merge_kernel_t pickK(int ipt, int k, int enc, bool aligned)
 {
+    const bool grouped = k >= 4 && aligned;
 #define CASEK(I, K_, E) \
-    if (ipt == I && k == K_ && enc == E) return k_spmv_mergeK<I, K_, E>;
+    if (ipt == I && k == K_ && enc == E) \
+        return grouped ? k_spmv_mergeKg<I, K_, E, 2> : k_spmv_mergeK<I, K_, E>;
 #define ENCK(I, K_) CASEK(I, K_, 0) CASEK(I, K_, 1) CASEK(I, K_, 2)
 #define KS(I, _) ENCK(I, 2) ENCK(I, 4) ENCK(I, 8) ENCK(I, 16)
     if (ipt == 3) { KS(3, 0) }
@@ -287,22 +481,23 @@ static merge_kernel_t pickK(int ipt, int k, int enc)
 #undef CASEK
 }
 
-static merge_kernel_t pick(int variant, int ipt, int enc, int nrhs)
+static merge_kernel_t pick(int variant, int ipt, int enc, int nrhs, bool aligned)
 {
     if (variant != 0 && variant != 1) return nullptr;
     if (nrhs == 1) return pick1(ipt, variant == 1, enc);
     if (variant != 0) return nullptr;
-    return pickK(ipt, nrhs, enc);
+    return pickK(ipt, nrhs, enc, aligned);
 }
 
 extern "C" int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs)
 {
-    return pick(variant, ipt, encoding, nrhs) != nullptr;
+    return pick(variant, ipt, encoding, nrhs, true) != nullptr;
 }
 
 extern "C" int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream)
 {
-    merge_kernel_t k = pick(a->variant, a->ipt, a->encoding, a->nrhs);
+    const bool aligned = (((uintptr_t)a->x | (uintptr_t)a->y | (uintptr_t)a->carry) & 15) == 0;
+    merge_kernel_t k = pick(a->variant, a->ipt, a->encoding, a->nrhs, aligned);
     if (!k) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
     int64_t grid = xcd_grid(a->ntiles);
@@ -314,8 +509,13 @@ extern "C" int vnec_hip_carry_fixup(const int32_t *coords, const double *carry, 
                                     double *y, int64_t nrows, int nrhs, void *stream)
 {
     if (ntiles <= 1) return 0;
-    int64_t grid = (ntiles - 1 + VNEC_BLOCK - 1) / VNEC_BLOCK; /* one thread per boundary */
-    hipLaunchKernelGGL(k_carry_fixup<double>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
+    /* one thread per boundary and right-hand side */
+    int64_t grid = ((ntiles - 1) * (int64_t)(nrhs > 0 ? nrhs : 1) + VNEC_BLOCK - 1) / VNEC_BLOCK;
+    void (*k)(const int2 *, const double *, int64_t, double *, int64_t, int) =
+        nrhs == 1 ? k_carry_fixup<double, 1> : nrhs == 2 ? k_carry_fixup<double, 2>
+        : nrhs == 4 ? k_carry_fixup<double, 4> : nrhs == 8 ? k_carry_fixup<double, 8>
+        : nrhs == 16 ? k_carry_fixup<double, 16> : k_carry_fixup<double, 0>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, nrhs);
     return (int)hipGetLastError();
 }
@@ -325,7 +525,7 @@ extern "C" int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carr
 {
     if (ntiles <= 1) return 0;
     int64_t grid = (ntiles - 1 + VNEC_BLOCK - 1) / VNEC_BLOCK;
-    hipLaunchKernelGGL(k_carry_fixup<float>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL((k_carry_fixup<float, 1>), dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream,
                        (const int2 *)coords, carry, ntiles, y, nrows, 1);
     return (int)hipGetLastError();
 }
