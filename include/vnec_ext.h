@@ -112,7 +112,7 @@ typedef struct {
     int ipt;      /* merge items per thread (tile = 256 * ipt items)       */
     int lanes;    /* D block width / H hot x entries in LDS                */
     int encoding; /* vnec_encoding                                         */
-    int nrhs;     /* right-hand sides per SpMV (1, or 2..16)               */
+    int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
     int grid;     /* 0: persistent, one workgroup per resident slot;
                      > 0: persistent with this many workgroups (multiple
                      of 8); -1: one workgroup per tile                     */
@@ -130,7 +130,9 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
  * Synchronous: y is valid when the call returns. */
 SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x);
 /* Same on caller device buffers, asynchronous on the library stream.
- * Device buffers hold doubles, or floats for VNEC_ENC_FP32 plans. */
+ * Device buffers hold doubles, or floats for VNEC_ENC_FP32 plans.  With
+ * nrhs >= 4, x and y 16-byte aligned run the faster grouped multi-RHS
+ * kernel; 8-byte aligned ones are accepted and run the one-lane kernel. */
 SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev);
 /* Times `iters` back-to-back SpMVs on the library stream with HIP events
  * (optionally replayed from one captured hipGraph); *ms = mean ms per SpMV.
