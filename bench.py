@@ -298,7 +298,8 @@ def main():
     variants = {}
     variant_y = {}
     if args.workload == "stencil" and not args.no_variants:
-        for name, var, ipt, enc, vb in (("G_fp64", "G", 8, "fp64", 8), ("S3_fp64", "S", 3, "fp64", 8),
+        for name, var, ipt, enc, vb in (("G_fp64", "G", 8, "fp64", 8), ("G_hilo", "G", 8, "hilo", 8),
+                                        ("G_hi", "G", 8, "hi", 4), ("S3_fp64", "S", 3, "fp64", 8),
                                         ("S5_fp64", "S", 5, "fp64", 8),
                                         ("S3_hilo", "S", 3, "hilo", 8),
                                         ("S3_hi", "S", 3, "hi", 4), ("L7_fp64", "L", 7, "fp64", 8),
@@ -363,7 +364,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": KERNEL_NAME.get((info.variant, info.grid >= 0), "?"),
+                     "kernel": ("k_spmv_mergeKg" if k >= 4 else "k_spmv_mergeK") if k > 1
+                               else KERNEL_NAME.get((info.variant, info.grid >= 0), "?"),
                      "kernel_us": round(k_ms * 1e3, 2), "fixup_us": round(fix_ms * 1e3, 2)},
         "graph_spmv_us": round(graph_ms * 1e3, 2),
         "host_vectors_spmv_us": round(host_ms * 1e3, 1),

@@ -226,7 +226,11 @@ typedef struct {
     const uint32_t *masks; /* nrows: pattern positions each row holds (0: none,
                               or a row left to the sub-plan)                 */
     const double *gv;      /* nslices * L * 64 values, slice s stored [l][lane],
-                              rows 64 s .. 64 s + 63, 0 where a bit is clear  */
+                              rows 64 s .. 64 s + 63, 0 where a bit is clear
+                              (encoding 0)                                     */
+    const float *gh, *gl;  /* the same slots as FP32 hi and lo (encoding 1), or
+                              hi only (encoding 2); summed in FP64             */
+    int encoding;          /* 0 FP64, 1 FP32 hi + lo, 2 FP32 hi               */
     const int32_t *pat;    /* L column offsets: position l of row i is column
                               i + pat[l]                                      */
     int L;
@@ -246,7 +250,8 @@ int vnec_hip_pattern_split(const int32_t *flags, const int32_t *pos, int64_t nro
  * the row does not hold it, or past nrows). */
 int vnec_hip_pattern_fill(const int32_t *ptr, const double *val, const uint32_t *masks, int64_t nrows,
                           int L, double *gv, void *stream);
-/* y[i] = sum over held positions l of gv * x[i + pat[l]], every row i. */
+/* y[i] = sum over held positions l of value * x[i + pat[l]], every row i;
+ * values from gv, or gh (+ gl) by a->encoding. */
 int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream);
 /* y[rows[i]] = ysub[i], i < n. */
 int vnec_hip_scatter_rows(const double *ysub, const int32_t *rows, int64_t n, double *y, void *stream);

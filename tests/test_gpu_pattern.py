@@ -6,7 +6,8 @@ rows at multiplicative-hash positions, the most frequent offset list, the
 first on ties; a row fits when each of its columns is row + pattern[l] for
 a distinct l, its mask holding those l).  SpMV results meet the FP64 gate
 |y - y_ref| <= 1e-12 (|A||x|)_i against the serial oracle, for the rows
-that fit and for the rest (run by the sub-plan) alike.
+that fit and for the rest (run by the sub-plan) alike; with FP32 hi-only
+slots the bound is 2^-23 (|A||x|)_i.
 """
 import numpy as np
 import pytest
@@ -156,11 +157,33 @@ def test_selection_rule_picks_g_on_large_stencils_only():
     assert spb.Plan(R, "learned").info().variant != "G"
 
 
+@pytest.mark.parametrize("enc,tol", [("hilo", TOL), ("hi", 2.0 ** -23)])
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+def test_g_value_encodings(name, m, enc, tol):
+    """FP32 hi + lo slots (a = hi + lo exactly to 2^-48 |a|) meet the FP64
+    gate; hi-only slots err by at most 2^-24 |a| per term, i.e. within
+    2^-23 (|A||x|)_i, and must really lose bits.  The irregular rows' sub-plan
+    uses the same encoding."""
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    print(f"case {name} variant G encoding {enc}", flush=True)
+    p = spb.Plan(A, "G", encoding=enc)
+    assert p.info().encoding == enc
+    y = spb.Vector(m.nrows)
+    p.spmv(y, spb.Vector.from_array(m.x))
+    y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y.to_array(), y0, ax, tol)
+    assert ok, worst
+    if enc == "hi":
+        assert worst > 1e-12
+    p.free()
+
+
 def test_g_refuses_what_it_cannot_store():
     m = g.stencil27(8, seed=12)
-    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
-    with pytest.raises(RuntimeError, match="FP64 values only"):
-        spb.Plan(A, "G", encoding="hi")
+    A32 = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val.astype(np.float32), dtype=np.float32)
+    with pytest.raises(RuntimeError, match="VNEC_ENC_FP32: variants L, S and D"):
+        spb.Plan(A32, "G")
     E = spb.Matrix.from_csr(4, 4, np.zeros(5, np.int64), np.zeros(0, np.int32), np.zeros(0))
     with pytest.raises(RuntimeError, match="no row pattern"):
         spb.Plan(E, "G")

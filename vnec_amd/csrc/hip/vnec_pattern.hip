@@ -18,9 +18,9 @@
 // a CSR sub-matrix that an ordinary plan runs into a scratch vector,
 // scattered back over those rows by k_scatter_rows afterwards.
 //
-// Per stored slot the kernel moves 8 bytes (the value) plus 4 bytes of
-// mask per row, instead of 10-12 bytes per nonzero for a column-indexed
-// format.
+// Per stored slot the kernel moves 8 bytes (the value; 4 for the FP32
+// hi-only encoding, 4 + 4 for hi + lo) plus 4 bytes of mask per row,
+// instead of 10-12 bytes per nonzero for a column-indexed format.
 #include <hip/hip_runtime.h>
 
 #include "vnec_common.h"
@@ -83,8 +83,18 @@ __global__ void k_pattern_fill(const int32_t *__restrict__ ptr, const double *__
     }
 }
 
+// Slot value k of the slices in FP64: stored FP64, FP32 hi + lo, or hi.
+template <int ENC>
+__device__ __forceinline__ double slot_value(const vnec_pattern_args &a, int64_t k)
+{
+    if (ENC == 0) return __builtin_nontemporal_load(a.gv + k);
+    if (ENC == 1)
+        return (double)__builtin_nontemporal_load(a.gh + k) + (double)__builtin_nontemporal_load(a.gl + k);
+    return (double)__builtin_nontemporal_load(a.gh + k);
+}
+
 // One wavefront per slice, four slices per workgroup, XCD-aware order.
-template <int U>
+template <int U, int ENC>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
 {
     const int64_t b = xcd_tile(blockIdx.x, grid);
@@ -93,21 +103,21 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a
     const int64_t row = s * 64 + lane_id();
     const bool live = row < a.nrows;
     const uint32_t m = live ? __builtin_nontemporal_load(a.masks + row) : 0u;
-    const double *__restrict__ v = a.gv + s * (int64_t)a.L * 64 + lane_id();
+    const int64_t v = s * (int64_t)a.L * 64 + lane_id();  // slot (slice s, l = 0, this lane)
     const double *__restrict__ x = a.x;
     double acc = 0.0;
     int l = 0;
     for (; l + U <= a.L; l += U) {
         double vv[U], xv[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) vv[u] = __builtin_nontemporal_load(v + (int64_t)(l + u) * 64);
+        for (int u = 0; u < U; u++) vv[u] = slot_value<ENC>(a, v + (int64_t)(l + u) * 64);
 #pragma unroll
         for (int u = 0; u < U; u++) xv[u] = ((m >> (l + u)) & 1u) ? x[row + a.pat[l + u]] : 0.0;
 #pragma unroll
         for (int u = 0; u < U; u++) acc += vv[u] * xv[u];
     }
     for (; l < a.L; l++) {
-        const double vv = __builtin_nontemporal_load(v + (int64_t)l * 64);
+        const double vv = slot_value<ENC>(a, v + (int64_t)l * 64);
         if ((m >> l) & 1u) acc += vv * x[row + a.pat[l]];
     }
     if (live) a.y[row] = acc;
@@ -162,10 +172,13 @@ extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
 {
     if (a->nslices <= 0) return 0;
     if (a->L < 1 || a->L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
+    if (a->encoding < 0 || a->encoding > 2) return (int)hipErrorInvalidValue;
     const int64_t blocks = (a->nslices + VNEC_NWAVES - 1) / VNEC_NWAVES;
     const int64_t grid = xcd_grid(blocks);
-    hipLaunchKernelGGL(k_spmv_pattern<8>, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0,
-                       (hipStream_t)stream, *a, grid);
+    void (*k)(vnec_pattern_args, int64_t) = a->encoding == 0   ? k_spmv_pattern<8, 0>
+                                            : a->encoding == 1 ? k_spmv_pattern<8, 1>
+                                                               : k_spmv_pattern<8, 2>;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, grid);
     return (int)hipGetLastError();
 }
 

@@ -106,7 +106,8 @@ struct vnec_plan_s {
     int g_L;
     uint32_t *g_masks;  /* nrows                                      */
     int64_t g_nslices;
-    double *g_vals;     /* nslices * L * 64                           */
+    double *g_vals;     /* nslices * L * 64 (FP64 encoding)           */
+    float *g_hi, *g_lo; /* the same slots, FP32 hi (+ lo) encodings   */
     int32_t *g_irr;     /* nirr rows that are not pattern subsets     */
     int64_t g_nirr;
     SpB_Matrix g_sub;   /* CSR of the irregular rows (owned)          */

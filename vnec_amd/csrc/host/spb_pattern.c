@@ -196,6 +196,22 @@ SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(vnec_hip_pattern_fill(a->d_ptr, a->d_val, p->g_masks, nr, L, p->g_vals, s),
                          "pattern fill");
+    const int enc = p->tune.encoding;
+    if (rc == SpB_SUCCESS && enc != VNEC_ENC_FP64) { /* narrow the slots: hi (+ lo) */
+        const int64_t n = p->g_nslices * L * 64;
+        p->g_hi = (float *)vnec_malloc_dev(sizeof(float) * (size_t)(n ? n : 1), &p->dev_bytes);
+        if (enc == VNEC_ENC_HILO)
+            p->g_lo = (float *)vnec_malloc_dev(sizeof(float) * (size_t)(n ? n : 1), &p->dev_bytes);
+        rc = (p->g_hi && (enc != VNEC_ENC_HILO || p->g_lo)) ? SpB_SUCCESS : SpB_FAILURE;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(vnec_hip_split_hilo(p->g_vals, n, p->g_hi, p->g_lo, s), "split slots");
+        if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+        if (rc == SpB_SUCCESS) {
+            vnec_free_dev(p->g_vals);
+            p->g_vals = NULL;
+            p->dev_bytes -= (int64_t)(sizeof(double) * (size_t)(n ? n : 1));
+        }
+    }
     int32_t *hirr = NULL;
     if (rc == SpB_SUCCESS && nirr) {
         hirr = (int32_t *)malloc(sizeof(int32_t) * (size_t)nirr);
@@ -213,6 +229,13 @@ SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
     if (rc == SpB_SUCCESS && nirr) {
         vnec_tuning t;
         rc = vnec_select_tuning(spb_as_csr(p->g_sub), VNEC_SELECT_LEARNED, 1, &t);
+        if (rc == SpB_SUCCESS && enc != VNEC_ENC_FP64) { /* the same value encoding */
+            if (t.variant == VNEC_VAR_D || t.variant == VNEC_VAR_H) {
+                vnec_tuning l = {VNEC_VAR_L, 7, 64, enc, 1, 0};
+                t = l;
+            }
+            t.encoding = enc;
+        }
         if (rc == SpB_SUCCESS) rc = vnec_plan_build_ext(&p->g_subplan, p->g_sub, &t);
         if (rc == SpB_SUCCESS) {
             p->g_ysub = (double *)vnec_malloc_dev(sizeof(double) * (size_t)nirr, &p->dev_bytes);

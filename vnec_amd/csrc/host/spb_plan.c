@@ -21,6 +21,8 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->g_pat);
     vnec_free_dev(p->g_masks);
     vnec_free_dev(p->g_vals);
+    vnec_free_dev(p->g_hi);
+    vnec_free_dev(p->g_lo);
     vnec_free_dev(p->g_irr);
     vnec_free_dev(p->g_ysub);
     vnec_free_dev(p->coords);
@@ -397,7 +399,6 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
     }
     if (tune.variant == VNEC_VAR_G) {
-        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant G stores FP64 values only");
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
     } else if (tune.variant == VNEC_VAR_H) {
@@ -633,6 +634,9 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, con
         vnec_pattern_args g;
         g.masks = p->g_masks;
         g.gv = p->g_vals;
+        g.gh = p->g_hi;
+        g.gl = p->g_lo;
+        g.encoding = p->tune.encoding;
         g.pat = p->g_pat;
         g.L = p->g_L;
         g.nrows = p->nrows;
