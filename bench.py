@@ -36,7 +36,8 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_NAME = {("S", False): "k_spmv_slice", ("L", True): "k_spmv_stream",
                ("E", True): "k_spmv_estream", ("L", False): "k_spmv_merge1",
                ("E", False): "k_spmv_merge1", ("G", False): "k_spmv_pattern",
-               ("D", False): "k_spmv_d", ("H", True): "k_spmv_hot4"}
+               ("D", False): "k_spmv_d", ("H", True): "k_spmv_hot4", ("C", True): "k_spmv_stream",
+               ("C", False): "k_spmv_merge1"}
 PLAIN_CANDIDATE = 4  # L, 7 items per thread, persistent: the plain merge-path CSR kernel
 METRIC = "FP64 SpMV GB/s as % of HBM peak; learned-vs-plain speedup at equal error"
 R18_FLUSH = 1 << 29  # 512 MiB scribbled between SpMVs for an HBM (not cache) figure

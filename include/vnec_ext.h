@@ -82,6 +82,9 @@ typedef enum {
     VNEC_VAR_G = 5, /* pattern slices: rows whose columns are a subset of
                        the dominant column-offset pattern store a position
                        mask and values only; the rest run a sub-plan       */
+    VNEC_VAR_C = 6, /* persistent merge-path CSR whose tiles store their
+                       nonzeros in column order (+ a 16-bit position in
+                       the tile): x is gathered in ascending column order */
     VNEC_VAR_N
 } vnec_variant;
 

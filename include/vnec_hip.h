@@ -96,6 +96,9 @@ typedef struct {
     const float *x32;         /* ncols                             */
     float *y32;               /* nrows                             */
     float *carry32;           /* ntiles scratch                    */
+    /* variant C: val / col hold each tile's nonzeros in column order and
+       spos their position within the tile's CSR range */
+    const uint16_t *spos;     /* nnz                               */
 } vnec_merge_args;
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
@@ -141,6 +144,12 @@ int vnec_hip_hot_encode(const int32_t *col, int64_t nnz, const int32_t *order, i
 /* xhot[h] = x[order[h]], h < nhot (before each variant-H SpMV). */
 int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, double *xhot,
                         void *stream);
+/* Variant C tile storage: per tile t (nonzeros coords[t].y .. coords[t+1].y)
+ * a stable sort by column; sval / scol / spos (nnz) receive the values,
+ * columns and in-tile positions in that order.  keys / idx: nnz scratch. */
+int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32_t *col, const double *val,
+                   int64_t nnz, int64_t ncols, uint64_t *keys, int32_t *idx, double *sval,
+                   int32_t *scol, uint16_t *spos, void *stream);
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 

@@ -69,11 +69,13 @@ CASES = _cases()
 
 
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 15), ("E", 5), ("E", 11)])
+@pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 15), ("E", 5), ("E", 11), ("C", 5),
+                                         ("C", 15)])
 @pytest.mark.parametrize("grid", [0, 8, -1])
 def test_spmv_fp64_matches_oracle(name, m, variant, ipt, grid):
     """grid 0: persistent pipelined kernel, 8: persistent with few workgroups
-    (long tile runs and carries across many tiles), -1: one workgroup per tile."""
+    (long tile runs and carries across many tiles), -1: one workgroup per tile.
+    Variant C is L with each tile's nonzeros stored in column order."""
     A = _mat(m)
     print(f"case {name} variant {variant} ipt {ipt} grid {grid}", flush=True)
     y = _run_plan(A, m.x, variant, ipt, grid=grid)
@@ -226,7 +228,7 @@ def test_known_answer_is_exact():
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val)
     rows = np.repeat(np.arange(m.nrows), np.diff(m.ptr))
     expect = np.bincount(rows, weights=rows * m.col.astype(np.float64), minlength=m.nrows)
-    for variant, ipt in (("L", 7), ("E", 7), ("S", 3), ("D", 7), ("H", 5)):
+    for variant, ipt in (("L", 7), ("E", 7), ("S", 3), ("D", 7), ("H", 5), ("C", 9)):
         assert np.array_equal(_run_plan(A, x, variant, ipt), expect)
 
 
@@ -240,7 +242,9 @@ def test_empty_rows_and_empty_matrix():
         assert np.allclose(_run_plan(A, m.x, variant, 3), y0, rtol=0, atol=1e-15)
     assert np.allclose(_run_plan(A, m.x, "D", 5), y0, rtol=0, atol=1e-15)
     assert np.allclose(_run_plan(A, m.x, "H", 5), y0, rtol=0, atol=1e-15)
+    assert np.allclose(_run_plan(A, m.x, "C", 5), y0, rtol=0, atol=1e-15)
     A = spb.Matrix.from_csr(5, 4, [0, 0, 0, 0, 0, 0], [], [])
+    assert np.array_equal(_run_plan(A, np.ones(4), "C", 7), np.zeros(5))
     assert np.array_equal(_run_plan(A, np.ones(4), "S", 1), np.zeros(5))
     assert np.array_equal(_run_plan(A, np.ones(4), "D", 7), np.zeros(5))
     assert np.array_equal(_run_plan(A, np.ones(4), "H", 7), np.zeros(5))
@@ -453,6 +457,39 @@ def test_repeated_columns_are_summed_by_every_variant():
     A = _mat(m)
     y0 = O.spmv_serial(m.ptr, m.col, m.val, m.x)
     ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
-    for variant, ipt in (("L", 7), ("E", 5), ("S", 3), ("D", 5), ("D", 9), ("H", 7)):
+    for variant, ipt in (("L", 7), ("E", 5), ("S", 3), ("D", 5), ("D", 9), ("H", 7), ("C", 7)):
         ok, worst = O.check_rows(_run_plan(A, m.x, variant, ipt), y0, ax, TOL)
         assert ok, (variant, worst)
+
+
+def test_column_sorted_tiles_rule_and_limits():
+    """Variant C: learned plans switch from L to C on skewed column use (at
+    most 0.8 distinct columns per nonzero over 16384-item windows, at least
+    2^20 nonzeros), not on uniform columns nor small matrices; C keeps the
+    per-row gate; it refuses what it cannot run."""
+    m = g.rmat(17, 16, seed=71)
+    ints = spb.feature_ints(_mat(m))
+    assert ints[30] <= 0.8 * ints[2] and ints[2] >= 1 << 20  # VF_NEC_LARGE, VF_NNZ
+    A = _mat(m)
+    p = spb.Plan(A, "learned")
+    info = p.info()
+    if info.variant in ("L", "C"):
+        assert info.variant == "C"
+    xv = spb.Vector.from_array(m.x)
+    yv = spb.Vector(m.nrows)
+    p.spmv(yv, xv)
+    ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    assert O.check_rows(yv.to_array(), yl, ax, TOL)[0]
+    p.free()
+    u = g.uniform(1 << 17, 16, seed=72)
+    U = _mat(u)
+    assert spb.feature_ints(U)[30] > 0.8 * spb.feature_ints(U)[2]
+    assert spb.Plan(U, "learned").info().variant != "C"
+    s = _mat(g.rmat(14, 16, seed=73))  # skewed but small
+    assert spb.Plan(s, "learned").info().variant != "C"
+    with pytest.raises(RuntimeError, match="variant C stores FP64 values only"):
+        spb.Plan(A, "C", 7, encoding="hi")
+    with pytest.raises(RuntimeError, match="variant C takes one right-hand side"):
+        spb.Plan(A, "C", 7, nrhs=4)
+    with pytest.raises(RuntimeError, match="ipt 8"):
+        spb.Plan(A, "C", 8)

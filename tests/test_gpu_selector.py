@@ -158,7 +158,14 @@ def test_selection_modes_parity(mode):
         p = spb.Plan(A, mode)
         info = p.info()
         want = spb.candidates()[c]
-        assert (info.variant, info.ipt) == (want.variant, want.ipt)
+        if info.variant == "C":
+            # the same merge-path plan with column-sorted tiles: learned and
+            # rule modes switch to it where columns are reused
+            ints = spb.feature_ints(A)
+            assert mode in ("learned", "rule") and want.variant == "L" and ints[30] <= 0.8 * ints[2]
+            assert info.ipt == want.ipt
+        else:
+            assert (info.variant, info.ipt) == (want.variant, want.ipt)
         xv = spb.Vector.from_array(m.x)
         yv = spb.Vector(m.nrows)
         p.spmv(yv, xv)

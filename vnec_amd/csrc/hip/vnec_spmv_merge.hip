@@ -40,7 +40,9 @@ __device__ __forceinline__ float *y_of(const vnec_merge_args &a, float) { return
 __device__ __forceinline__ double *carry_of(const vnec_merge_args &a, double) { return a.carry; }
 __device__ __forceinline__ float *carry_of(const vnec_merge_args &a, float) { return a.carry32; }
 
-template <int IPT, bool WIN, int ENC>
+// SORT (variant C): the tile's nonzeros arrive in column order with their
+// CSR offsets in a.spos; each product is staged at its CSR offset.
+template <int IPT, bool WIN, int ENC, bool SORT = false>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, int64_t grid)
 {
     typedef typename enc_real<ENC>::type F;
@@ -72,21 +74,23 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     for (int i = tid; i < nrow; i += VNEC_BLOCK) s_rowend[i] = a.ptr[x0 + 1 + i];
     {
         F av[IPT];
-        int cv[IPT];
+        int cv[IPT], pv[IPT];
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             int k = u * VNEC_BLOCK + tid;
+            pv[u] = k;
             if (k < nz) {
                 int64_t j = (int64_t)y0 + k;
                 av[u] = load_value<ENC>(a, j);
                 cv[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
                             : __builtin_nontemporal_load(a.col + j);
+                if (SORT) pv[u] = (int)__builtin_nontemporal_load(a.spos + j);
             }
         }
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             int k = u * VNEC_BLOCK + tid;
-            if (k < nz) s_prod[k] = av[u] * (WIN ? s_win[cv[u]] : x[cv[u]]);
+            if (k < nz) s_prod[pv[u]] = av[u] * (WIN ? s_win[cv[u]] : x[cv[u]]);
         }
     }
     __syncthreads();
@@ -448,10 +452,11 @@ typedef void (*merge_kernel_t)(vnec_merge_args, int64_t);
 #define IPT_LIST(X, ...) X(3, __VA_ARGS__) X(5, __VA_ARGS__) X(7, __VA_ARGS__) X(9, __VA_ARGS__) \
     X(11, __VA_ARGS__) X(13, __VA_ARGS__) X(15, __VA_ARGS__)
 
-static merge_kernel_t pick1(int ipt, bool win, int enc)
+static merge_kernel_t pick1(int ipt, bool win, int enc, bool sort)
 {
 #define CASE1(I, W, E) \
-    if (ipt == I && win == W && enc == E) return k_spmv_merge1<I, W, E>;
+    if (ipt == I && win == W && enc == E && !sort) return k_spmv_merge1<I, W, E>; \
+    if (ipt == I && !W && E == 0 && !win && enc == 0 && sort) return k_spmv_merge1<I, false, 0, true>;
 #define ENC3(I, W) CASE1(I, W, 0) CASE1(I, W, 1) CASE1(I, W, 2)
 #define WIN2(I, _) ENC3(I, false) ENC3(I, true) CASE1(I, false, VNEC_ENC_F32_K)
     IPT_LIST(WIN2, 0)
@@ -483,8 +488,8 @@ static merge_kernel_t pickK(int ipt, int k, int enc, bool aligned)
 
 static merge_kernel_t pick(int variant, int ipt, int enc, int nrhs, bool aligned)
 {
-    if (variant != 0 && variant != 1) return nullptr;
-    if (nrhs == 1) return pick1(ipt, variant == 1, enc);
+    if (variant != 0 && variant != 1 && variant != 6) return nullptr;
+    if (nrhs == 1) return pick1(ipt, variant == 1, enc, variant == 6);
     if (variant != 0) return nullptr;
     return pickK(ipt, nrhs, enc, aligned);
 }

@@ -31,10 +31,11 @@ struct TileRegs {
     double v[IPT];
     int c[IPT];
     int re[IPT];
+    int p[IPT];  // variant C: position of the nonzero in the tile's CSR range
 };
 
 // tid: the thread's index within its 256-thread tile pipeline.
-template <int IPT, bool WIN, int ENC>
+template <int IPT, bool WIN, int ENC, bool SORT = false>
 __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int2 c1,
                                           TileRegs<IPT, WIN> &R, int tid)
 {
@@ -47,6 +48,7 @@ __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int
             R.v[u] = ld_value<ENC>(a, j);
             R.c[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
                          : __builtin_nontemporal_load(a.col + j);
+            if (SORT) R.p[u] = (int)__builtin_nontemporal_load(a.spos + j);
         }
         if (k < nrow) R.re[u] = a.ptr[c0.x + 1 + k];
     }
@@ -63,7 +65,10 @@ __device__ __forceinline__ double gather_x(const vnec_merge_args &a, const doubl
     return a.x[c];
 }
 
-template <int IPT, bool WIN, int ENC, int HOT = 0>
+// SORT (variant C): the tile's nonzeros arrive in column order, so each
+// wavefront's x gathers are ascending and share cache lines where columns
+// are dense; each product is written to its CSR position in LDS.
+template <int IPT, bool WIN, int ENC, int HOT = 0, bool SORT = false>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, int nwg,
                                                             int32_t *__restrict__ wg_row,
                                                             double *__restrict__ wg_carry)
@@ -113,7 +118,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         }
     }
     TileRegs<IPT, WIN> cur, nxt;
-    load_tile<IPT, WIN, ENC>(a, A0, A1, cur, tid);
+    load_tile<IPT, WIN, ENC, SORT>(a, A0, A1, cur, tid);
     if (HOT > 0)
         for (int i = tid; i < a.nhot; i += VNEC_BLOCK) s_hot[i] = a.xhot[i];
     __syncthreads();
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                 nec_c = a.nec[t + 2];
             }
         }
-        if (t + 1 < te) load_tile<IPT, WIN, ENC>(a, A1, B1, nxt, tid);
+        if (t + 1 < te) load_tile<IPT, WIN, ENC, SORT>(a, A1, B1, nxt, tid);
 
         const int x0 = A0.x, y0 = A0.y;
         const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
@@ -146,7 +151,8 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
             const int k = u * VNEC_BLOCK + tid;
             if (k < nrow) s_rowend[k] = cur.re[u];
             if (k < nz)
-                s_prod[k] = cur.v[u] * (WIN ? s_win[cur.c[u]] : gather_x<HOT>(a, s_hot, cur.c[u]));
+                s_prod[SORT ? cur.p[u] : k] =
+                    cur.v[u] * (WIN ? s_win[cur.c[u]] : gather_x<HOT>(a, s_hot, cur.c[u]));
         }
         __syncthreads();
 
@@ -596,6 +602,7 @@ static stream_kernel_t pick_stream(int variant, int ipt, int enc)
     if (ipt == I && enc == E) {                                      \
         if (variant == 0) return k_spmv_stream<I, false, E>;         \
         if (variant == 1) return k_spmv_estream<I, E>;               \
+        if (variant == 6 && E == 0) return k_spmv_stream<I, false, 0, 0, true>; \
     }
 #define SIPT(I) SCASE(I, 0) SCASE(I, 1) SCASE(I, 2)
     SIPT(3) SIPT(5) SIPT(7) SIPT(9) SIPT(11) SIPT(13) SIPT(15)
@@ -606,7 +613,8 @@ static stream_kernel_t pick_stream(int variant, int ipt, int enc)
 
 extern "C" int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg)
 {
-    stream_kernel_t k = (variant == 0 || variant == 1) ? pick_stream(variant, ipt, encoding) : nullptr;
+    stream_kernel_t k = (variant == 0 || variant == 1 || variant == 6) ? pick_stream(variant, ipt, encoding)
+                                                                        : nullptr;
     if (!k) return (int)hipErrorInvalidValue;
     int per_cu = 0, dev = 0, cus = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, VNEC_BLOCK, 0);

@@ -181,6 +181,7 @@ static void matrix_drop_device(spb_csr *a)
     a->d_val = NULL;
     a->dev_ready = 0;
     a->tuned_cand = -1;
+    a->feat_ready = 0;
 }
 
 SpB_Info SpB_Matrix_free(SpB_Matrix *A)

@@ -53,6 +53,8 @@ typedef struct spb_csr {
     double *d_val;      /* nnz, FP64 (FP32 values widened)   */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
     int tuned_cand;     /* autotuner's best candidate, -1 until measured */
+    int feat_ready;     /* feat_ints holds the selector's statistics     */
+    uint64_t feat_ints[VNEC_NFEAT_INTS];
 } spb_csr;
 
 typedef struct spb_dense {
@@ -113,6 +115,10 @@ struct vnec_plan_s {
     SpB_Matrix g_sub;   /* CSR of the irregular rows (owned)          */
     struct vnec_plan_s *g_subplan;
     double *g_ysub;     /* nirr                                       */
+    /* variant C: each tile's nonzeros in column order */
+    double *cs_val;     /* nnz                                        */
+    int32_t *cs_col;    /* nnz                                        */
+    uint16_t *cs_pos;   /* nnz: offset within the tile's CSR range    */
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;

@@ -12,6 +12,7 @@
 #define TILE_THREADS 256
 
 static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
+static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a);
 
 static void plan_release(struct vnec_plan_s *p)
 {
@@ -23,6 +24,9 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->g_vals);
     vnec_free_dev(p->g_hi);
     vnec_free_dev(p->g_lo);
+    vnec_free_dev(p->cs_val);
+    vnec_free_dev(p->cs_col);
+    vnec_free_dev(p->cs_pos);
     vnec_free_dev(p->g_irr);
     vnec_free_dev(p->g_ysub);
     vnec_free_dev(p->coords);
@@ -331,6 +335,26 @@ static SpB_Info build_hot(struct vnec_plan_s *p, const spb_csr *a)
  * padding), and the matrix is large enough (2^20 nonzeros) for the bytes
  * saved to outweigh a possible sub-plan's extra launches. */
 #define G_MIN_NNZ (1 << 20)
+/* Skewed column use: at most this many distinct columns per nonzero over
+ * merge-path windows of 16384 items (R-MAT graphs: 0.45-0.78; uniform and
+ * power-law-row matrices: 0.97-1.0), on at least 2^20 nonzeros (below, the
+ * sorted staging costs more than the gathers save). */
+#define CSORT_MAX_DISTINCT 0.8
+#define CSORT_MIN_NNZ (1 << 20)
+
+/* A merge-path plan on a matrix whose tiles reuse columns stores each tile
+ * in column order (variant C): the wavefront's gathers then share cache
+ * lines among the most used columns. */
+static int csort_rule(spb_csr *a)
+{
+    uint64_t v[VNEC_NFEAT_INTS];
+    if (vnec_feature_ints_ext((SpB_Matrix)a, v) != SpB_SUCCESS) {
+        vnec_clear_error(); /* no statistics: keep the selector's plan */
+        return 0;
+    }
+    return v[VF_NNZ] >= CSORT_MIN_NNZ && (double)v[VF_NEC_LARGE] <= CSORT_MAX_DISTINCT * (double)v[VF_NNZ];
+}
+
 static int pattern_rule(spb_csr *a)
 {
     if (a->base.element_type != SpB_FP64 || (int64_t)a->nnz < G_MIN_NNZ) return 0;
@@ -384,6 +408,11 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             tune.variant = VNEC_VAR_L; /* the encodings every merge kernel has */
             tune.ipt = 7;
         }
+        const int mode = t ? t->variant : VNEC_SELECT_LEARNED;
+        if ((mode == VNEC_SELECT_LEARNED || mode == VNEC_SELECT_RULE) && tune.variant == VNEC_VAR_L &&
+            tune.nrhs == 1 && tune.encoding == VNEC_ENC_FP64 && csort_rule(a)) {
+            tune.variant = VNEC_VAR_C; /* same granularity and schedule */
+        }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;
     if (tune.variant < 0 || tune.variant >= VNEC_VAR_N)
@@ -401,6 +430,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
+    } else if (tune.variant == VNEC_VAR_C) {
+        if (!valid_ipt(tune.ipt)) return vnec_fail("ipt %d: use an odd value in [3, 15]", tune.ipt);
+        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant C stores FP64 values only");
+        if (tune.nrhs != 1) return vnec_fail("variant C takes one right-hand side");
     } else if (tune.variant == VNEC_VAR_H) {
         if (!vnec_hip_hot_supported(tune.ipt, tune.lanes))
             return vnec_fail("variant H: ipt 5 or 7 and lanes (hot entries) 4096 or 8192");
@@ -478,6 +511,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
                              "partition");
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_D) rc = build_d(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_H) rc = build_hot(p, a);
+        if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_C) rc = build_csort(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_E) {
             int32_t max_runs = 0;
             rc = build_windows(p, a, &max_runs, NULL);
@@ -521,6 +555,26 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
     return SpB_SUCCESS;
 }
 
+/* Variant C: each tile's nonzeros re-stored in column order. */
+static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
+{
+    const size_t n = (size_t)(p->nnz > 0 ? p->nnz : 1);
+    p->cs_val = (double *)vnec_malloc_dev(sizeof(double) * n, &p->dev_bytes);
+    p->cs_col = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * n, &p->dev_bytes);
+    p->cs_pos = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * n, &p->dev_bytes);
+    uint64_t *keys = (uint64_t *)vnec_malloc_dev(sizeof(uint64_t) * n, NULL);
+    int32_t *idx = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * n, NULL);
+    SpB_Info rc = (p->cs_val && p->cs_col && p->cs_pos && keys && idx) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, a->d_col, a->d_val, p->nnz, p->ncols, keys, idx,
+                                        p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
+                         "column sort");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "sync");
+    vnec_free_dev(keys);
+    vnec_free_dev(idx);
+    return rc;
+}
+
 static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_dev,
                       vnec_merge_args *out)
 {
@@ -552,6 +606,11 @@ static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_de
     m.variant = p->tune.variant;
     m.encoding = p->tune.encoding;
     m.nrhs = p->tune.nrhs;
+    if (p->tune.variant == VNEC_VAR_C) {
+        m.val = p->cs_val;
+        m.col = p->cs_col;
+        m.spos = p->cs_pos;
+    }
     if (p->tune.variant == VNEC_VAR_H) {
         m.col = p->hcol;
         m.xhot = p->xhot;

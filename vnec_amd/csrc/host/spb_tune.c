@@ -125,6 +125,10 @@ SpB_Info vnec_feature_ints_ext(const SpB_Matrix A, uint64_t ints[VNEC_NFEAT_INTS
     spb_csr *a = spb_as_csr(A);
     if (!a || !ints) return vnec_fail("vnec_feature_ints_ext: bad arguments");
     if (!vnec_runtime_ready() || spb_matrix_upload(a) != SpB_SUCCESS) return SpB_FAILURE;
+    if (a->feat_ready) { /* computed since the matrix last changed */
+        memcpy(ints, a->feat_ints, sizeof a->feat_ints);
+        return SpB_SUCCESS;
+    }
     hipStream_t s = vnec_stream();
     int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
     uint64_t *d = (uint64_t *)vnec_malloc_dev(sizeof(uint64_t) * VNEC_NFEAT_INTS, NULL);
@@ -154,6 +158,10 @@ SpB_Info vnec_feature_ints_ext(const SpB_Matrix A, uint64_t ints[VNEC_NFEAT_INTS
     vnec_free_dev(d);
     vnec_free_dev(dh);
     free(hh);
+    if (rc == SpB_SUCCESS) {
+        memcpy(a->feat_ints, ints, sizeof a->feat_ints);
+        a->feat_ready = 1;
+    }
     return rc;
 }
 

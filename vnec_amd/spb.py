@@ -20,7 +20,7 @@ SpB_FP32 = 10
 SpB_FP64 = 11
 SpB_MATRIX_FORMAT = 1
 SpB_CSR = 1
-VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4, "G": 5}
+VNEC_VAR = {"L": 0, "E": 1, "S": 2, "D": 3, "H": 4, "G": 5, "C": 6}
 VNEC_ENC = {"fp64": 0, "hilo": 1, "hi": 2, "fp32": 3}
 # Selection modes (negative vnec_tuning.variant values).
 SELECT = {"learned": L.SELECT_LEARNED, "autotune": L.SELECT_AUTOTUNE, "rule": L.SELECT_RULE,
