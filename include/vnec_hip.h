@@ -97,8 +97,10 @@ typedef struct {
     float *y32;               /* nrows                             */
     float *carry32;           /* ntiles scratch                    */
     /* variant C: val / col hold each tile's nonzeros in column order and
-       spos their position within the tile's CSR range */
+       spos their position within the tile's CSR range; with cpos_shift > 0
+       the position is packed into col instead: col = column | pos << shift */
     const uint16_t *spos;     /* nnz                               */
+    int cpos_shift;
 } vnec_merge_args;
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
@@ -146,10 +148,13 @@ int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, dou
                         void *stream);
 /* Variant C tile storage: per tile t (nonzeros coords[t].y .. coords[t+1].y)
  * a stable sort by column; sval / scol / spos (nnz) receive the values,
- * columns and in-tile positions in that order.  keys / idx: nnz scratch. */
+ * columns and in-tile positions in that order.  With pos_shift > 0 the
+ * positions go into scol's high bits (scol = column | pos << pos_shift,
+ * columns below 2^pos_shift, positions below 2^(32 - pos_shift)) and spos
+ * is not written.  keys / idx: nnz scratch. */
 int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32_t *col, const double *val,
-                   int64_t nnz, int64_t ncols, uint64_t *keys, int32_t *idx, double *sval,
-                   int32_t *scol, uint16_t *spos, void *stream);
+                   int64_t nnz, int64_t ncols, int pos_shift, uint64_t *keys, int32_t *idx,
+                   double *sval, int32_t *scol, uint16_t *spos, void *stream);
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 

@@ -493,3 +493,21 @@ def test_column_sorted_tiles_rule_and_limits():
         spb.Plan(A, "C", 7, nrhs=4)
     with pytest.raises(RuntimeError, match="ipt 8"):
         spb.Plan(A, "C", 8)
+
+
+@pytest.mark.parametrize("ipt", [3, 9, 15])
+def test_column_sorted_tiles_wide_matrix(ipt):
+    """Variant C keeps each nonzero's tile offset beside its column, packed
+    in one word while both fit 32 bits, else in a separate array: 2^23
+    columns force the separate array, 2^12 columns the packed form."""
+    rng = np.random.default_rng(80 + ipt)
+    for ncols in (1 << 23, 1 << 12):
+        n = 30000
+        lens = rng.integers(0, 20, n)
+        ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        col = np.concatenate([np.sort(rng.choice(ncols, k, replace=False)) for k in lens]).astype(np.int32)
+        m = g.from_pattern(n, ncols, ptr, col, seed=ipt)
+        y = _run_plan(_mat(m), m.x, "C", ipt)
+        ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+        ok, worst = O.check_rows(y, yl, ax, TOL)
+        assert ok, (ncols, worst)

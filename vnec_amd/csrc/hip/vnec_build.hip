@@ -401,8 +401,8 @@ __global__ void k_csort_keys(const int2 *__restrict__ co, const int32_t *__restr
 // After the sort: position j holds tile t's nonzero at in-tile offset idx[j].
 __global__ void k_csort_apply(const int2 *__restrict__ co, const uint64_t *__restrict__ keys,
                               const int32_t *__restrict__ idx, const double *__restrict__ val,
-                              int64_t nnz, int cb, double *__restrict__ sval, int32_t *__restrict__ scol,
-                              uint16_t *__restrict__ spos)
+                              int64_t nnz, int cb, int pos_shift, double *__restrict__ sval,
+                              int32_t *__restrict__ scol, uint16_t *__restrict__ spos)
 {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
          j += (int64_t)gridDim.x * blockDim.x) {
@@ -410,20 +410,27 @@ __global__ void k_csort_apply(const int2 *__restrict__ co, const uint64_t *__res
         const int64_t t = (int64_t)(k >> cb);
         const int off = idx[j];
         sval[j] = val[co[t].y + off];
-        scol[j] = (int32_t)(k & (((uint64_t)1 << cb) - 1));
-        spos[j] = (uint16_t)off;
+        const uint32_t c = (uint32_t)(k & (((uint64_t)1 << cb) - 1));
+        if (pos_shift > 0) {
+            scol[j] = (int32_t)(c | ((uint32_t)off << pos_shift));
+        } else {
+            scol[j] = (int32_t)c;
+            spos[j] = (uint16_t)off;
+        }
     }
 }
 
 extern "C" int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32_t *col, const double *val,
-                              int64_t nnz, int64_t ncols, uint64_t *keys, int32_t *idx, double *sval,
-                              int32_t *scol, uint16_t *spos, void *stream)
+                              int64_t nnz, int64_t ncols, int pos_shift, uint64_t *keys, int32_t *idx,
+                              double *sval, int32_t *scol, uint16_t *spos, void *stream)
 {
     if (nnz <= 0 || ntiles <= 0) return 0;
     int cb = 1, tb = 1;  // bits for a column, bits for a tile index
     while (cb < 31 && ((int64_t)1 << cb) < ncols) cb++;
     while (tb < 32 && ((int64_t)1 << tb) < ntiles) tb++;
     if (cb + tb > 64 || ntiles >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (pos_shift < 0 || pos_shift > 31 || (pos_shift > 0 && ncols > ((int64_t)1 << pos_shift)))
+        return (int)hipErrorInvalidValue;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_csort_keys, dim3((unsigned)ntiles), dim3(256), 0, s, (const int2 *)coords, col, cb,
                        keys, idx);
@@ -433,7 +440,7 @@ extern "C" int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32
     if (rc) return rc;
     unsigned g = (unsigned)((nnz + 255) / 256 < 65536 ? (nnz + 255) / 256 : 65536);
     hipLaunchKernelGGL(k_csort_apply, dim3(g), dim3(256), 0, s, (const int2 *)coords, keys, idx, val, nnz, cb,
-                       sval, scol, spos);
+                       pos_shift, sval, scol, spos);
     return (int)hipGetLastError();
 }
 

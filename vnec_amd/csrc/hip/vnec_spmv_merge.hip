@@ -84,13 +84,20 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
                 av[u] = load_value<ENC>(a, j);
                 cv[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
                             : __builtin_nontemporal_load(a.col + j);
-                if (SORT) pv[u] = (int)__builtin_nontemporal_load(a.spos + j);
+                if (SORT && a.cpos_shift == 0) pv[u] = (int)__builtin_nontemporal_load(a.spos + j);
             }
         }
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             int k = u * VNEC_BLOCK + tid;
-            if (k < nz) s_prod[pv[u]] = av[u] * (WIN ? s_win[cv[u]] : x[cv[u]]);
+            if (k < nz) {
+                int c = cv[u], pos = pv[u];
+                if (SORT && a.cpos_shift > 0) {  // position packed above the column
+                    pos = (int)((uint32_t)c >> a.cpos_shift);
+                    c &= (1 << a.cpos_shift) - 1;
+                }
+                s_prod[pos] = av[u] * (WIN ? s_win[c] : x[c]);
+            }
         }
     }
     __syncthreads();

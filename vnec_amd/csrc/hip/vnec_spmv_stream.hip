@@ -48,7 +48,9 @@ __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int
             R.v[u] = ld_value<ENC>(a, j);
             R.c[u] = WIN ? (int)__builtin_nontemporal_load(a.lcol + j)
                          : __builtin_nontemporal_load(a.col + j);
-            if (SORT) R.p[u] = (int)__builtin_nontemporal_load(a.spos + j);
+            // (a position packed above the column is split where it is used,
+            // so the loads of the next tile stay in flight)
+            if (SORT && a.cpos_shift == 0) R.p[u] = (int)__builtin_nontemporal_load(a.spos + j);
         }
         if (k < nrow) R.re[u] = a.ptr[c0.x + 1 + k];
     }
@@ -150,9 +152,17 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         for (int u = 0; u < IPT; u++) {
             const int k = u * VNEC_BLOCK + tid;
             if (k < nrow) s_rowend[k] = cur.re[u];
-            if (k < nz)
-                s_prod[SORT ? cur.p[u] : k] =
-                    cur.v[u] * (WIN ? s_win[cur.c[u]] : gather_x<HOT>(a, s_hot, cur.c[u]));
+            if (k < nz) {
+                int c = cur.c[u], pos = k;
+                if (SORT) {
+                    pos = cur.p[u];
+                    if (a.cpos_shift > 0) {  // position packed above the column
+                        pos = (int)((uint32_t)c >> a.cpos_shift);
+                        c &= (1 << a.cpos_shift) - 1;
+                    }
+                }
+                s_prod[pos] = cur.v[u] * (WIN ? s_win[c] : gather_x<HOT>(a, s_hot, c));
+            }
         }
         __syncthreads();
 

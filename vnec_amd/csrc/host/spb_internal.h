@@ -118,7 +118,9 @@ struct vnec_plan_s {
     /* variant C: each tile's nonzeros in column order */
     double *cs_val;     /* nnz                                        */
     int32_t *cs_col;    /* nnz                                        */
-    uint16_t *cs_pos;   /* nnz: offset within the tile's CSR range    */
+    uint16_t *cs_pos;   /* nnz: offset within the tile's CSR range,
+                           NULL when packed into cs_col's high bits  */
+    int cs_shift;       /* > 0: cs_col = column | offset << cs_shift  */
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;

@@ -559,15 +559,23 @@ SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int
 static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
 {
     const size_t n = (size_t)(p->nnz > 0 ? p->nnz : 1);
+    /* offsets within a tile stay below 256 ipt; when a column and an offset
+     * fit one 32-bit word together, the offset rides in the column's high
+     * bits (12 bytes per nonzero, as CSR) */
+    int cb = 1, ob = 1;
+    while (((int64_t)1 << cb) < p->ncols) cb++;
+    while ((1 << ob) < TILE_THREADS * p->tune.ipt) ob++;
+    p->cs_shift = cb + ob <= 32 ? cb : 0;
     p->cs_val = (double *)vnec_malloc_dev(sizeof(double) * n, &p->dev_bytes);
     p->cs_col = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * n, &p->dev_bytes);
-    p->cs_pos = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * n, &p->dev_bytes);
+    if (!p->cs_shift) p->cs_pos = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * n, &p->dev_bytes);
     uint64_t *keys = (uint64_t *)vnec_malloc_dev(sizeof(uint64_t) * n, NULL);
     int32_t *idx = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * n, NULL);
-    SpB_Info rc = (p->cs_val && p->cs_col && p->cs_pos && keys && idx) ? SpB_SUCCESS : SpB_FAILURE;
+    SpB_Info rc = (p->cs_val && p->cs_col && (p->cs_shift || p->cs_pos) && keys && idx) ? SpB_SUCCESS
+                                                                                          : SpB_FAILURE;
     if (rc == SpB_SUCCESS)
-        rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, a->d_col, a->d_val, p->nnz, p->ncols, keys, idx,
-                                        p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
+        rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, a->d_col, a->d_val, p->nnz, p->ncols, p->cs_shift,
+                                        keys, idx, p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
                          "column sort");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "sync");
     vnec_free_dev(keys);
@@ -610,6 +618,7 @@ static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_de
         m.val = p->cs_val;
         m.col = p->cs_col;
         m.spos = p->cs_pos;
+        m.cpos_shift = p->cs_shift;
     }
     if (p->tune.variant == VNEC_VAR_H) {
         m.col = p->hcol;
