@@ -15,20 +15,36 @@
 #include "vnec_common.h"
 #include "vnec_hip.h"
 
+// One nonzero's value as loaded: FP64, or FP32 hi (+ lo).  The FP32 forms
+// are widened (and added) only where the product is formed: converting at
+// the load would make the wavefront wait there for the next tile's loads,
+// which the pipeline keeps in flight while the current tile is reduced.
+template <int ENC> struct RawVal { double d; };
+template <> struct RawVal<1> { float h, l; };
+template <> struct RawVal<2> { float h; };
+
 template <int ENC>
-__device__ __forceinline__ double ld_value(const vnec_merge_args &a, int64_t j)
+__device__ __forceinline__ RawVal<ENC> ld_value(const vnec_merge_args &a, int64_t j);
+template <> __device__ __forceinline__ RawVal<0> ld_value<0>(const vnec_merge_args &a, int64_t j)
 {
-    if (ENC == 0) return __builtin_nontemporal_load(a.val + j);
-    if (ENC == 1)
-        return (double)__builtin_nontemporal_load(a.hi + j) +
-               (double)__builtin_nontemporal_load(a.lo + j);
-    return (double)__builtin_nontemporal_load(a.hi + j);
+    return {__builtin_nontemporal_load(a.val + j)};
 }
+template <> __device__ __forceinline__ RawVal<1> ld_value<1>(const vnec_merge_args &a, int64_t j)
+{
+    return {__builtin_nontemporal_load(a.hi + j), __builtin_nontemporal_load(a.lo + j)};
+}
+template <> __device__ __forceinline__ RawVal<2> ld_value<2>(const vnec_merge_args &a, int64_t j)
+{
+    return {__builtin_nontemporal_load(a.hi + j)};
+}
+__device__ __forceinline__ double widen(RawVal<0> v) { return v.d; }
+__device__ __forceinline__ double widen(RawVal<1> v) { return (double)v.h + (double)v.l; }
+__device__ __forceinline__ double widen(RawVal<2> v) { return (double)v.h; }
 
 // Registers holding one tile's streamed data (striped: item u * 256 + tid).
-template <int IPT, bool WIN>
+template <int IPT, bool WIN, int ENC = 0>
 struct TileRegs {
-    double v[IPT];
+    RawVal<ENC> v[IPT];
     int c[IPT];
     int re[IPT];
     int p[IPT];  // variant C: position of the nonzero in the tile's CSR range
@@ -37,7 +53,7 @@ struct TileRegs {
 // tid: the thread's index within its 256-thread tile pipeline.
 template <int IPT, bool WIN, int ENC, bool SORT = false>
 __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int2 c1,
-                                          TileRegs<IPT, WIN> &R, int tid)
+                                          TileRegs<IPT, WIN, ENC> &R, int tid)
 {
     const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
 #pragma unroll
@@ -119,7 +135,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
             nec_b = a.nec[tb + 1];
         }
     }
-    TileRegs<IPT, WIN> cur, nxt;
+    TileRegs<IPT, WIN, ENC> cur, nxt;
     load_tile<IPT, WIN, ENC, SORT>(a, A0, A1, cur, tid);
     if (HOT > 0)
         for (int i = tid; i < a.nhot; i += VNEC_BLOCK) s_hot[i] = a.xhot[i];
@@ -161,7 +177,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                         c &= (1 << a.cpos_shift) - 1;
                     }
                 }
-                s_prod[pos] = cur.v[u] * (WIN ? s_win[c] : gather_x<HOT>(a, s_hot, c));
+                s_prod[pos] = widen(cur.v[u]) * (WIN ? s_win[c] : gather_x<HOT>(a, s_hot, c));
             }
         }
         __syncthreads();
@@ -307,7 +323,7 @@ __global__ __launch_bounds__(VNEC_BLOCK * HOT_SUB) void k_spmv_hot4(vnec_merge_a
             if (k < nrow) s_rowend[g][k] = cur.re[u];
             if (k < nz) {
                 const int c = cur.c[u];
-                s_prod[g][k] = cur.v[u] * (c < 0 ? s_hot[-1 - c] : a.x[c]);
+                s_prod[g][k] = widen(cur.v[u]) * (c < 0 ? s_hot[-1 - c] : a.x[c]);
             }
         }
         __syncthreads();
@@ -407,9 +423,9 @@ __device__ __forceinline__ int find_run(const int2 *runs, int nr, int p)
     return lo;
 }
 
-template <int IPT>
+template <int IPT, int ENC>
 struct ERegs {
-    double v[IPT];
+    RawVal<ENC> v[IPT];
     double xw[IPT];
     int c[IPT];
     int re[IPT];
@@ -417,7 +433,7 @@ struct ERegs {
 
 template <int IPT, int ENC>
 __device__ __forceinline__ void e_load(const vnec_merge_args &a, int2 c0, int2 c1, int nec,
-                                       const int2 *runs, int nr, ERegs<IPT> &R)
+                                       const int2 *runs, int nr, ERegs<IPT, ENC> &R)
 {
     const int tid = threadIdx.x;
     const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
@@ -482,7 +498,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_estream(vnec_merge_args a, 
     if (tb + 1 < te && tid < roC - roB)
         s_runs[(tb + 1) & 1][tid] = make_int2(a.run_start[roB + tid], a.run_wpos[roB + tid]);
     __syncthreads();
-    ERegs<IPT> cur, nxt;
+    ERegs<IPT, ENC> cur, nxt;
     e_load<IPT, ENC>(a, cA, cB, necA, s_runs[tb & 1], roB - roA, cur);
 
     for (int64_t t = tb; t < te; t++) {
@@ -514,7 +530,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_estream(vnec_merge_args a, 
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             const int k = u * VNEC_BLOCK + tid;
-            if (k < nz) s_prod[k] = cur.v[u] * s_win[cur.c[u]];
+            if (k < nz) s_prod[k] = widen(cur.v[u]) * s_win[cur.c[u]];
         }
         __syncthreads();
         const int items = nrow + nz;
