@@ -227,8 +227,9 @@ def main():
                     help="held-out matrices for the learned-vs-plain comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the SpMV would shard by merge-path diagonal ranges; see DESIGN.md)",
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(the SpMV would shard by merge-path diagonal ranges, each GPU taking a "
+              "contiguous range of rows and exchanging one carry per boundary)",
               file=sys.stderr)
         sys.exit(2)
 
