@@ -367,6 +367,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
                      "kernel": ("k_spmv_mergeKg" if k >= 4 else "k_spmv_mergeK") if k > 1
+                               else "k_spmv_strided" if info.grid == -2
                                else KERNEL_NAME.get((info.variant, info.grid >= 0), "?"),
                      "kernel_us": round(k_ms * 1e3, 2), "fixup_us": round(fix_ms * 1e3, 2)},
         "graph_spmv_us": round(graph_ms * 1e3, 2),

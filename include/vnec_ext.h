@@ -118,7 +118,12 @@ typedef struct {
     int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
     int grid;     /* 0: persistent, one workgroup per resident slot;
                      > 0: persistent with this many workgroups (multiple
-                     of 8); -1: one workgroup per tile                     */
+                     of 8); -1: every tile leaves its own carry, one
+                     workgroup per tile (or, with at least 16 tiles per
+                     resident workgroup, the form of -2); -2: the same
+                     per-tile carries from a persistent launch whose
+                     workgroups sweep each XCD's tiles together (variants
+                     L and C, one right-hand side, not VNEC_ENC_FP32)    */
 } vnec_tuning;
 
 typedef struct vnec_plan_s *vnec_plan;

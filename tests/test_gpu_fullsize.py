@@ -99,6 +99,8 @@ def test_batch512_full_size(k, ipt):
     A = spb.Matrix.from_batch(m.row_off, m.col_off, m.ptr, m.col, m.val)
     print(f"batch512 nnz {m.nnz} k {k} plan L{ipt}", flush=True)
     p = spb.Plan(A, "L", ipt, nrhs=k, grid=-1)
+    if k == 1:  # hundreds of tiles per workgroup: the strided persistent launch
+        assert p.info().grid == -2
     y = _run_twice(p, m.x, m.nrows * k)
     p.free()
     col = m.col + np.repeat(m.col_off[:-1], np.diff(m.ptr[m.row_off])).astype(np.int32)

@@ -71,11 +71,14 @@ CASES = _cases()
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 15), ("E", 5), ("E", 11), ("C", 5),
                                          ("C", 15)])
-@pytest.mark.parametrize("grid", [0, 8, -1])
+@pytest.mark.parametrize("grid", [0, 8, -1, -2])
 def test_spmv_fp64_matches_oracle(name, m, variant, ipt, grid):
     """grid 0: persistent pipelined kernel, 8: persistent with few workgroups
-    (long tile runs and carries across many tiles), -1: one workgroup per tile.
-    Variant C is L with each tile's nonzeros stored in column order."""
+    (long tile runs and carries across many tiles), -1: one workgroup per tile,
+    -2: per-tile carries from the strided persistent launch (L and C; E runs
+    -1 there).  Variant C is L with each tile's nonzeros stored in column order."""
+    if grid == -2 and variant == "E":
+        grid = -1
     A = _mat(m)
     print(f"case {name} variant {variant} ipt {ipt} grid {grid}", flush=True)
     y = _run_plan(A, m.x, variant, ipt, grid=grid)
@@ -511,3 +514,23 @@ def test_column_sorted_tiles_wide_matrix(ipt):
         ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
         ok, worst = O.check_rows(y, yl, ax, TOL)
         assert ok, (ncols, worst)
+
+
+@pytest.mark.parametrize("variant,ipt,enc", [("L", 3, "fp64"), ("L", 11, "hilo"), ("L", 7, "hi"), ("C", 9, "fp64")])
+def test_strided_launch_is_bit_identical_to_one_workgroup_per_tile(variant, ipt, enc):
+    """grid -2 runs the tiles of grid -1 (same tiles, same per-tile carries,
+    same fix-up) from a persistent launch: the results agree bit for bit."""
+    m = g.rmat(15, 16, seed=91)
+    A = _mat(m)
+    p1 = spb.Plan(A, variant, ipt, encoding=enc, grid=-1)
+    p2 = spb.Plan(A, variant, ipt, encoding=enc, grid=-2)
+    assert p2.info().grid == -2
+    y1 = _run_plan(A, m.x, variant, ipt, enc, grid=-1)
+    y2 = _run_plan(A, m.x, variant, ipt, enc, grid=-2)
+    assert np.array_equal(y1, y2)
+    p1.free()
+    p2.free()
+    with pytest.raises(RuntimeError, match="grid -2"):
+        spb.Plan(A, "E", 5, grid=-2)
+    with pytest.raises(RuntimeError, match="grid -3"):
+        spb.Plan(A, "L", 5, grid=-3)

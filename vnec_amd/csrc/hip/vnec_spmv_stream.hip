@@ -257,6 +257,150 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
     }
 }
 
+// ------------------------------------------------ strided persistent form
+// One-workgroup-per-tile semantics (every tile leaves its partial last row
+// in carry[t] for k_carry_fixup; no carry flows between tiles) with a
+// persistent, software-pipelined launch: the workgroups on XCD x sweep
+// that XCD's contiguous share of the tiles together, workgroup j taking
+// tiles j, j + W, j + 2W, ... of it (W workgroups per XCD), so the tiles in
+// flight on one XCD are neighbours and share its L2, while each workgroup
+// issues its next tile's loads before reducing the current one.
+template <int IPT, int ENC, bool SORT>
+__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, int nwg)
+{
+    constexpr int T = VNEC_BLOCK * IPT;
+    __shared__ double s_prod[T];
+    __shared__ int s_rowend[T];
+    __shared__ int s_f[VNEC_NWAVES + 1];
+    __shared__ double s_v[VNEC_NWAVES + 1];
+
+    const int tid = threadIdx.x;
+    const int64_t per = nwg / VNEC_XCDS;  // workgroups per XCD
+    const int64_t xcd = blockIdx.x % VNEC_XCDS, j = blockIdx.x / VNEC_XCDS;
+    const int64_t xlo = xcd * a.ntiles / VNEC_XCDS, xhi = (xcd + 1) * a.ntiles / VNEC_XCDS;
+    const int2 *co = (const int2 *)a.coords;
+    int64_t t = xlo + j;
+    if (t >= xhi) return;
+
+    int2 A0 = co[t], A1 = co[t + 1];
+    TileRegs<IPT, false, ENC> cur, nxt;
+    load_tile<IPT, false, ENC, SORT>(a, A0, A1, cur, tid);
+
+    for (; t < xhi; t += per) {
+        // --- issue the next tile's loads
+        const int64_t tn = t + per;
+        int2 B0 = A0, B1 = A1;
+        if (tn < xhi) {
+            B0 = co[tn];
+            B1 = co[tn + 1];
+            load_tile<IPT, false, ENC, SORT>(a, B0, B1, nxt, tid);
+        }
+
+        const int x0 = A0.x, y0 = A0.y;
+        const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
+
+        // --- row ends and products into LDS
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int k = u * VNEC_BLOCK + tid;
+            if (k < nrow) s_rowend[k] = cur.re[u];
+            if (k < nz) {
+                int c = cur.c[u], pos = k;
+                if (SORT) {
+                    pos = cur.p[u];
+                    if (a.cpos_shift > 0) {  // position packed above the column
+                        pos = (int)((uint32_t)c >> a.cpos_shift);
+                        c &= (1 << a.cpos_shift) - 1;
+                    }
+                }
+                s_prod[pos] = widen(cur.v[u]) * a.x[c];
+            }
+        }
+        __syncthreads();
+
+        // --- thread-level merge walk
+        const int items = nrow + nz;
+        const int d0 = tid * IPT < items ? tid * IPT : items;
+        const int d1 = d0 + IPT < items ? d0 + IPT : items;
+        int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
+        int ny = d0 - rx;
+        double acc = 0.0, head_val = 0.0;
+        int flag = 0, head_row = 0;
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            if (d0 + u < d1) {
+                if (rx < nrow && s_rowend[rx] <= y0 + ny) {
+                    if (!flag) {
+                        head_row = x0 + rx;
+                        head_val = acc;
+                        flag = 1;
+                    } else {
+                        a.y[x0 + rx] = acc;
+                    }
+                    acc = 0.0;
+                    rx++;
+                } else {
+                    acc += s_prod[ny];
+                    ny++;
+                }
+            }
+        }
+
+        // --- segmented join across the tile's threads; the tile's carry-out
+        double incl;
+        const double cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
+        if (flag) a.y[head_row] = head_val + cin;
+        if (tid == VNEC_BLOCK - 1) a.carry[t] = incl;
+        __syncthreads();
+
+        cur = nxt;
+        A0 = B0;
+        A1 = B1;
+    }
+}
+
+typedef void (*strided_kernel_t)(vnec_merge_args, int);
+
+static strided_kernel_t pick_strided(int variant, int ipt, int enc)
+{
+#define TCASE(I, E)                                                           \
+    if (ipt == I && enc == E) {                                               \
+        if (variant == 0) return k_spmv_strided<I, E, false>;                 \
+        if (variant == 6 && E == 0) return k_spmv_strided<I, 0, true>;        \
+    }
+#define TIPT(I) TCASE(I, 0) TCASE(I, 1) TCASE(I, 2)
+    TIPT(3) TIPT(5) TIPT(7) TIPT(9) TIPT(11) TIPT(13) TIPT(15)
+    return nullptr;
+#undef TIPT
+#undef TCASE
+}
+
+extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int *nwg)
+{
+    strided_kernel_t k = pick_strided(variant, ipt, encoding);
+    if (!k) return (int)hipErrorInvalidValue;
+    int per_cu = 0, dev = 0, cus = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, VNEC_BLOCK, 0);
+    if (e != hipSuccess) return (int)e;
+    e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return (int)e;
+    int n = per_cu * cus;
+    n = n / VNEC_XCDS * VNEC_XCDS;
+    *nwg = n > 0 ? n : VNEC_XCDS;
+    return 0;
+}
+
+extern "C" int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream)
+{
+    strided_kernel_t k = pick_strided(a->variant, a->ipt, a->encoding);
+    if (!k || a->nrhs != 1 || nwg < VNEC_XCDS || nwg % VNEC_XCDS) return (int)hipErrorInvalidValue;
+    if (a->ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, nwg);
+    return (int)hipGetLastError();
+}
+
 // ----------------------------------------------------------- variant H
 // Four independent tile pipelines of 256 threads share one workgroup of
 // 1024 threads and one LDS copy of the hot x entries, so the hot table

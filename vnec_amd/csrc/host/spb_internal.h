@@ -76,6 +76,8 @@ struct vnec_plan_s {
     int32_t *coords;   /* 2 (ntiles + 1) */
     double *carry;     /* ntiles * nrhs  (one workgroup per tile)      */
     int nwg;           /* persistent workgroups (0: one per tile)       */
+    int snwg;          /* > 0: per-tile carries from the strided
+                          persistent launch of this many workgroups  */
     int32_t *wg_row;   /* nwg: row of each workgroup's trailing carry   */
     double *wg_carry;  /* nwg                                           */
     /* variant E */

@@ -12,6 +12,10 @@
 #define TILE_THREADS 256
 
 static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
+/* Tiles per resident workgroup from which per-tile-carry plans use the
+ * strided persistent launch (its pipelining pays once a workgroup sees
+ * many tiles; below, the one-workgroup-per-tile launch is as fast). */
+#define STRIDED_MIN_TILES_PER_WG 16
 static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a);
 
 static void plan_release(struct vnec_plan_s *p)
@@ -418,6 +422,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (tune.variant < 0 || tune.variant >= VNEC_VAR_N)
         return vnec_fail("variant %d is not available", tune.variant);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
+    if (tune.grid < -2) return vnec_fail("grid %d: use -2, -1, 0 or a positive multiple of 8", tune.grid);
+    if (tune.grid == -2 && !((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.nrhs <= 1 &&
+                             tune.encoding != VNEC_ENC_FP32))
+        return vnec_fail("grid -2 (strided persistent): variants L and C with one right-hand side");
     if (fp32 != (tune.encoding == VNEC_ENC_FP32))
         return vnec_fail(fp32 ? "an FP32 matrix takes encoding VNEC_ENC_FP32"
                               : "encoding VNEC_ENC_FP32 needs a matrix made with SpB_FP32");
@@ -504,6 +512,16 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             p->tune.grid = -1;
             p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(p->ntiles * tune.nrhs), &p->dev_bytes);
             if (!p->carry) rc = SpB_FAILURE;
+            /* per-tile carries from a persistent launch when asked for, or
+             * when each resident workgroup would see at least 16 tiles */
+            int snwg = 0;
+            if (rc == SpB_SUCCESS && tune.nrhs == 1 && tune.encoding != VNEC_ENC_FP32 &&
+                (tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) &&
+                vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, &snwg) == 0 &&
+                (tune.grid == -2 || p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
+                p->snwg = snwg;
+                p->tune.grid = -2;
+            }
         }
         if (!p->coords) rc = SpB_FAILURE;
         if (rc == SpB_SUCCESS)
@@ -723,6 +741,7 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, con
     }
     if (p->nwg > 0)
         return vnec_hip_ok(vnec_hip_spmv_stream(m, p->nwg, p->wg_row, p->wg_carry, s), "spmv kernel");
+    if (p->snwg > 0) return vnec_hip_ok(vnec_hip_spmv_strided(m, p->snwg, s), "spmv kernel");
     return vnec_hip_ok(vnec_hip_spmv_merge(m, s), "spmv kernel");
 }
 
