@@ -142,6 +142,12 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x);
  * nrhs >= 4, x and y 16-byte aligned run the faster grouped multi-RHS
  * kernel; 8-byte aligned ones are accepted and run the one-lane kernel. */
 SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev);
+/* Same with the right-hand sides column-major: X holds nrhs vectors of
+ * ncols doubles one after another, Y nrhs vectors of nrows.  The plan
+ * interleaves X into scratch of its own (allocated on the first call),
+ * runs, and writes Y back column-major; for nrhs = 1 this is
+ * vnec_spmv_dev_ext. */
+SpB_Info vnec_spmm_colmajor_dev_ext(vnec_plan plan, const void *X_dev, void *Y_dev);
 /* Times `iters` back-to-back SpMVs on the library stream with HIP events
  * (optionally replayed from one captured hipGraph); *ms = mean ms per SpMV.
  * If flush_bytes > 0 a buffer of that size is overwritten before each SpMV

@@ -66,6 +66,11 @@ int vnec_hip_d_fill(const int32_t *ptr, const int32_t *ecr, const double *val, i
                     const int32_t *coords, int64_t P, int L, const int32_t *v_row_ptr,
                     int32_t *col_start, double *val_align, int accumulate, void *stream);
 
+/* Right-hand-side layouts, k <= 16 vectors of n doubles: column-major
+ * (src[c n + j]) to interleaved (dst[j k + c]) when to_interleaved, else
+ * interleaved to column-major. */
+int vnec_hip_rhs_layout(const double *src, int64_t n, int k, int to_interleaved, double *dst, void *stream);
+
 /* FP64 -> FP32 hi + FP32 lo split: hi = fl32(v), lo = fl32(v - hi). */
 int vnec_hip_split_hilo(const double *val, int64_t n, float *hi, float *lo, void *stream);
 

@@ -134,6 +134,7 @@ SIGNATURES = {
     "vnec_plan_info_ext": (Info, [vp, C.POINTER(Tuning), C.POINTER(i64), C.POINTER(i64)]),
     "vnec_spmv_ext": (Info, [vp, vp, vp]),
     "vnec_spmv_dev_ext": (Info, [vp, vp, vp]),
+    "vnec_spmm_colmajor_dev_ext": (Info, [vp, vp, vp]),
     "vnec_time_dev_ext": (Info, [vp, vp, vp, C.c_int, C.c_int, C.c_size_t,
                                  C.POINTER(C.c_double)]),
     "vnec_time_kernels_ext": (Info, [vp, vp, vp, C.c_int, C.POINTER(C.c_double),
@@ -186,6 +187,7 @@ SIGNATURES = {
     "vnec_hip_d_count": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp, vp, vp]),
     "vnec_hip_d_fill": (C.c_int, [vp, vp, vp, i64, vp, i64, C.c_int, vp, vp, vp, C.c_int, vp]),
     "vnec_hip_split_hilo": (C.c_int, [vp, i64, vp, vp, vp]),
+    "vnec_hip_rhs_layout": (C.c_int, [vp, i64, C.c_int, C.c_int, vp, vp]),
     "vnec_hip_csort": (C.c_int, [vp, i64, vp, vp, i64, i64, C.c_int, vp, vp, vp, vp, vp, vp]),
     "vnec_hip_spmv_merge": (C.c_int, [vp, vp]),
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),

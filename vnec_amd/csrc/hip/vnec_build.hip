@@ -444,6 +444,54 @@ extern "C" int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32
     return (int)hipGetLastError();
 }
 
+// ------------------------------------------- right-hand-side layouts
+// Column-major k x n (k vectors of n, one after another) <-> interleaved
+// n x k (entry j of every vector together), k <= 16.  A workgroup moves 256
+// entries of each vector through LDS, so both sides are read and written
+// in contiguous runs.
+#define XPOSE_N 256
+__global__ __launch_bounds__(XPOSE_N) void k_to_interleaved(const double *__restrict__ src, int64_t n, int k,
+                                                          double *__restrict__ dst)
+{
+    __shared__ double s[16][XPOSE_N + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * XPOSE_N;
+    const int t = threadIdx.x;
+    for (int c = 0; c < k; c++)
+        if (j0 + t < n) s[c][t] = src[c * n + j0 + t];
+    __syncthreads();
+    const int64_t m = (n - j0 < XPOSE_N ? n - j0 : XPOSE_N) * k;
+    for (int64_t e = t; e < m; e += XPOSE_N) dst[j0 * k + e] = s[e % k][e / k];
+}
+
+__global__ __launch_bounds__(XPOSE_N) void k_from_interleaved(const double *__restrict__ src, int64_t n, int k,
+                                                            double *__restrict__ dst)
+{
+    __shared__ double s[16][XPOSE_N + 1];
+    const int64_t j0 = (int64_t)blockIdx.x * XPOSE_N;
+    const int t = threadIdx.x;
+    const int64_t m = (n - j0 < XPOSE_N ? n - j0 : XPOSE_N) * k;
+    for (int64_t e = t; e < m; e += XPOSE_N) s[e % k][e / k] = src[j0 * k + e];
+    __syncthreads();
+    for (int c = 0; c < k; c++)
+        if (j0 + t < n) dst[c * n + j0 + t] = s[c][t];
+}
+
+extern "C" int vnec_hip_rhs_layout(const double *src, int64_t n, int k, int to_interleaved, double *dst,
+                                   void *stream)
+{
+    if (n <= 0) return 0;
+    if (k < 1 || k > 16) return (int)hipErrorInvalidValue;
+    const int64_t g = (n + XPOSE_N - 1) / XPOSE_N;
+    if (g >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (to_interleaved)
+        hipLaunchKernelGGL(k_to_interleaved, dim3((unsigned)g), dim3(XPOSE_N), 0, (hipStream_t)stream, src, n, k,
+                           dst);
+    else
+        hipLaunchKernelGGL(k_from_interleaved, dim3((unsigned)g), dim3(XPOSE_N), 0, (hipStream_t)stream, src, n, k,
+                           dst);
+    return (int)hipGetLastError();
+}
+
 // ---------------------------------------------------------- value split
 __global__ void k_split_hilo(const double *__restrict__ v, int64_t n, float *__restrict__ hi,
                              float *__restrict__ lo)

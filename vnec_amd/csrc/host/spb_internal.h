@@ -130,6 +130,8 @@ struct vnec_plan_s {
     int64_t dev_bytes;
     /* cached graph for timing */
     hipGraphExec_t gexec;
+    /* interleaved copies of column-major right-hand sides (multi-RHS) */
+    double *xt, *yt;
     const void *g_x;
     void *g_y;
     int g_reps;

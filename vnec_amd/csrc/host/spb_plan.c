@@ -28,6 +28,8 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->g_vals);
     vnec_free_dev(p->g_hi);
     vnec_free_dev(p->g_lo);
+    vnec_free_dev(p->xt);
+    vnec_free_dev(p->yt);
     vnec_free_dev(p->cs_val);
     vnec_free_dev(p->cs_col);
     vnec_free_dev(p->cs_pos);
@@ -783,6 +785,25 @@ SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev)
 {
     if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmv_dev_ext: not a plan");
     return vnec_plan_run(plan, x_dev, y_dev);
+}
+
+SpB_Info vnec_spmm_colmajor_dev_ext(vnec_plan plan, const void *X_dev, void *Y_dev)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmm_colmajor_dev_ext: not a plan");
+    const int k = plan->tune.nrhs;
+    if (k <= 1) return vnec_plan_run(plan, X_dev, Y_dev);
+    if (!plan->xt) {
+        plan->xt = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(plan->ncols > 0 ? plan->ncols * k : 1),
+                                             &plan->dev_bytes);
+        plan->yt = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(plan->nrows > 0 ? plan->nrows * k : 1),
+                                             &plan->dev_bytes);
+        if (!plan->xt || !plan->yt) return SpB_FAILURE;
+    }
+    hipStream_t s = vnec_stream();
+    if (vnec_hip_ok(vnec_hip_rhs_layout((const double *)X_dev, plan->ncols, k, 1, plan->xt, s), "interleave X"))
+        return SpB_FAILURE;
+    if (vnec_plan_run(plan, plan->xt, plan->yt) != SpB_SUCCESS) return SpB_FAILURE;
+    return vnec_hip_ok(vnec_hip_rhs_layout(plan->yt, plan->nrows, k, 0, (double *)Y_dev, s), "de-interleave Y");
 }
 
 SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)

@@ -261,6 +261,11 @@ class Plan:
     def spmv_dev(self, x_ptr: int, y_ptr: int):
         L.check(L.load().vnec_spmv_dev_ext(self.h, x_ptr, y_ptr), "vnec_spmv_dev_ext")
 
+    def spmm_colmajor_dev(self, x_ptr: int, y_ptr: int):
+        """Right-hand sides column-major: nrhs vectors of ncols (x) and of
+        nrows (y), one after another."""
+        L.check(L.load().vnec_spmm_colmajor_dev_ext(self.h, x_ptr, y_ptr), "vnec_spmm_colmajor_dev_ext")
+
     def time(self, x_ptr: int, y_ptr: int, iters: int, graph: bool = True,
              flush_bytes: int = 0) -> float:
         ms = C.c_double()
