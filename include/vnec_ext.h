@@ -237,10 +237,11 @@ SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *i
 
 /* Candidate configurations (single right-hand side, FP64).  Copies up to
  * max of them to out (may be NULL); returns VNEC_NCAND. */
-#define VNEC_NCAND 14
+#define VNEC_NCAND 17
 int vnec_candidates_ext(vnec_tuning *out, int max);
 /* Which candidates may run on A: bit c set when candidate c applies
- * (slice candidates need padding <= 3 and fewer than 2^31 slots). */
+ * (slice candidates need padding <= 3 and fewer than 2^31 slots; D
+ * candidates need 8 * lanes * nnz <= 2^32 bytes of value blocks). */
 SpB_Info vnec_candidate_mask_ext(const SpB_Matrix A, uint32_t *mask);
 /* Exhaustive autotune: builds every applicable candidate, times it with
  * hipGraph replay (best of 3 means over reps SpMVs) and returns the times

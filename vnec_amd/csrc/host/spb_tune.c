@@ -26,14 +26,18 @@ static const vnec_tuning CANDIDATES[VNEC_NCAND] = {
     {VNEC_VAR_L, 11, 64, 0, 1, 0},  {VNEC_VAR_L, 5, 64, 0, 1, -1},  {VNEC_VAR_L, 7, 64, 0, 1, -1},
     {VNEC_VAR_L, 11, 64, 0, 1, -1}, {VNEC_VAR_E, 5, 64, 0, 1, -1},  {VNEC_VAR_E, 7, 64, 0, 1, -1},
     {VNEC_VAR_E, 5, 64, 0, 1, 0},   {VNEC_VAR_E, 7, 64, 0, 1, 0},
+    {VNEC_VAR_D, 7, 4, 0, 1, -1},   {VNEC_VAR_D, 7, 8, 0, 1, -1},   {VNEC_VAR_D, 7, 16, 0, 1, -1},
 };
 /* Candidates named by the fixed selection modes. */
 #define CAND_S3 1
 #define CAND_L7_PERSISTENT 4
-#define CAND_E5_TILE 10
+#define CAND_D8 15 /* the reference's own block width */
 
 /* Slice candidates apply only up to this padding (slots / nnz). */
 #define TUNE_MAX_PADDING 3.0
+/* D candidates apply while their value blocks (at most 8 * lanes bytes per
+ * nonzero) fit this many bytes. */
+#define TUNE_MAX_D_BYTES (1ull << 32)
 /* Row count from which the reference dispatch always takes VNEC-L
  * (test_spmv_vnec.cpp:197). */
 #define REF_L_ROWS 3000000
@@ -56,6 +60,9 @@ uint32_t vnec_candidate_mask_from_ints_ext(const uint64_t v[VNEC_NFEAT_INTS])
     uint32_t m = 0;
     for (int c = 0; c < VNEC_NCAND; c++) {
         if (CANDIDATES[c].variant == VNEC_VAR_S && !slices_ok) continue;
+        if (CANDIDATES[c].variant == VNEC_VAR_D &&
+            (nnz == 0 || nnz * 8ull * (uint64_t)CANDIDATES[c].lanes > TUNE_MAX_D_BYTES))
+            continue;
         m |= 1u << c;
     }
     return m;
@@ -576,7 +583,7 @@ SpB_Info vnec_select_ext(const SpB_Matrix A, int mode, int *candidate)
         int dec = SpB_VNEC_S;
         if (vnec_ird_ext(A, 36, 8, 0.477, NULL, &dec) != SpB_SUCCESS) return SpB_FAILURE;
         if (dec == SpB_VNEC_D) {
-            *candidate = CAND_E5_TILE;
+            *candidate = CAND_D8;
         } else {
             uint64_t ints[VNEC_NFEAT_INTS];
             if (vnec_feature_ints_ext(A, ints) != SpB_SUCCESS) return SpB_FAILURE;
