@@ -1,6 +1,8 @@
 // vnec_build.hip — GPU builders: merge-path partitioner, exclusive scan,
-// ECR column compression (per-tile LDS sort and dense-flag forms) and the
-// FP32 hi/lo value split.
+// ECR column compression (per-tile LDS sort and dense-flag forms), the
+// column-sorted tiles of variant C, the right-hand-side layout transposes,
+// the FP32 hi/lo value split and small utilities (fills, widening, the
+// gather-ceiling probe).
 #include <hip/hip_runtime.h>
 
 #include "vnec_common.h"

@@ -1,4 +1,5 @@
-// vnec_spmv_merge.hip — merge-path SpMV for gfx950 (variants L and E).
+// vnec_spmv_merge.hip — merge-path SpMV for gfx950, one workgroup per tile
+// (variants L, E and C; multi-RHS; the FP32 API's L) and the carry fix-up.
 //
 // Restates the merge-based CSR of spmv_vnec_L_for_mxv.c:124-165 for wave64
 // workgroups: a tile is a fixed share of the merge list (row ends +

@@ -1,5 +1,6 @@
 // vnec_spmv_stream.hip — persistent, software-pipelined merge-path FP64 SpMV
-// (variants L and E, one right-hand side).
+// (variants L, E, C and H, one right-hand side), and the strided persistent
+// form of the one-workgroup-per-tile kernel (k_spmv_strided, below).
 //
 // Each workgroup owns one contiguous run of merge-path tiles (workgroups on
 // one XCD own neighbouring runs, so the x entries they share stay in that
