@@ -160,9 +160,12 @@ def test_selection_modes_parity(mode):
         want = spb.candidates()[c]
         if info.variant == "C":
             # the same merge-path plan with column-sorted tiles: learned and
-            # rule modes switch to it where columns are reused
-            ints = spb.feature_ints(A)
-            assert mode in ("learned", "rule") and want.variant == "L" and ints[30] <= 0.8 * ints[2]
+            # rule modes switch to it where columns are reused, autotune mode
+            # where it measures faster
+            assert mode in ("learned", "rule", "autotune") and want.variant == "L"
+            if mode != "autotune":
+                ints = spb.feature_ints(A)
+                assert ints[30] <= 0.8 * ints[2]
             assert info.ipt == want.ipt
         else:
             assert (info.variant, info.ipt) == (want.variant, want.ipt)

@@ -53,6 +53,8 @@ typedef struct spb_csr {
     double *d_val;      /* nnz, FP64 (FP32 values widened)   */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
     int tuned_cand;     /* autotuner's best candidate, -1 until measured */
+    int tuned_csort;    /* its L plan measured against variant C: 1 C is
+                           faster, 0 not, -1 not measured              */
     int feat_ready;     /* feat_ints holds the selector's statistics     */
     uint64_t feat_ints[VNEC_NFEAT_INTS];
 } spb_csr;
@@ -176,6 +178,9 @@ double vnec_slice_padding(const spb_csr *a);
  * granularity autotuned in VNEC_SELECT_AUTOTUNE mode, else 5 items per
  * thread). */
 SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t);
+/* Times the FP64 merge-path plan l against the same plan with column-sorted
+ * tiles (variant C); *use_c = 1 when C is faster.  Cached on the matrix. */
+SpB_Info vnec_autotune_csort(spb_csr *a, const vnec_tuning *l, int *use_c);
 
 /* ---- compiled-in selector model (spb_selector_model.c, generated) ---- */
 extern const int vnec_model_trained;

@@ -415,9 +415,15 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             tune.ipt = 7;
         }
         const int mode = t ? t->variant : VNEC_SELECT_LEARNED;
-        if ((mode == VNEC_SELECT_LEARNED || mode == VNEC_SELECT_RULE) && tune.variant == VNEC_VAR_L &&
-            tune.nrhs == 1 && tune.encoding == VNEC_ENC_FP64 && csort_rule(a)) {
-            tune.variant = VNEC_VAR_C; /* same granularity and schedule */
+        if (tune.variant == VNEC_VAR_L && tune.nrhs == 1 && tune.encoding == VNEC_ENC_FP64) {
+            /* column-sorted tiles at the same granularity and schedule: by
+             * the column-reuse rule, or measured in autotune mode */
+            int use_c = 0;
+            if (mode == VNEC_SELECT_LEARNED || mode == VNEC_SELECT_RULE)
+                use_c = csort_rule(a);
+            else if (mode == VNEC_SELECT_AUTOTUNE && vnec_autotune_csort(a, &tune, &use_c) != SpB_SUCCESS)
+                return SpB_FAILURE;
+            if (use_c) tune.variant = VNEC_VAR_C;
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;

@@ -636,6 +636,45 @@ static SpB_Info autotune_multi(spb_csr *a, int nrhs, int *ipt)
     return rc;
 }
 
+SpB_Info vnec_autotune_csort(spb_csr *a, const vnec_tuning *l, int *use_c)
+{
+    if (a->tuned_csort >= 0) {
+        *use_c = a->tuned_csort;
+        return SpB_SUCCESS;
+    }
+    int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
+    double *hx = (double *)malloc(sizeof(double) * (size_t)(nc ? nc : 1));
+    double *dx = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nc ? nc : 1), NULL);
+    double *dy = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nr ? nr : 1), NULL);
+    SpB_Info rc = (hx && dx && dy) ? SpB_SUCCESS : vnec_fail("out of memory");
+    for (int64_t i = 0; rc == SpB_SUCCESS && i < nc; i++) hx[i] = 1.0 / (double)(1 + (i % 7));
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
+    double t[2] = {INFINITY, INFINITY};
+    for (int v = 0; rc == SpB_SUCCESS && v < 2; v++) {
+        vnec_tuning u = *l;
+        u.variant = v ? VNEC_VAR_C : VNEC_VAR_L;
+        vnec_plan p = NULL;
+        if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &u) != SpB_SUCCESS) {
+            vnec_clear_error(); /* stays INFINITY */
+            continue;
+        }
+        for (int r = 0; rc == SpB_SUCCESS && r < 3; r++) {
+            double ms = 0.0;
+            rc = vnec_time_dev_ext(p, dx, dy, 20, 1, 0, &ms);
+            if (rc == SpB_SUCCESS && ms < t[v]) t[v] = ms;
+        }
+        vnec_plan_free_ext(&p);
+    }
+    free(hx);
+    vnec_free_dev(dx);
+    vnec_free_dev(dy);
+    if (rc != SpB_SUCCESS) return rc;
+    a->tuned_csort = t[1] < t[0];
+    *use_c = a->tuned_csort;
+    return SpB_SUCCESS;
+}
+
 SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t)
 {
     if (nrhs > 1) {
