@@ -485,6 +485,27 @@ spb_dense *spb_as_dense(const SpB_Vector v)
     return d;
 }
 
+/* Host storage of a vector's values: page-locked when a GPU is present, so
+ * the uploads of x and downloads of y move by DMA at the link's speed
+ * instead of through the runtime's staging copies; else 64-byte aligned
+ * pageable memory.  Either way 64-byte aligned host memory. */
+static void *vector_host_alloc(size_t bytes, int *pinned)
+{
+    *pinned = 0;
+    if (bytes == 0) bytes = 64;
+    if (vnec_runtime_ready()) {
+        void *p = NULL;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess && p) {
+            *pinned = 1;
+            return p;
+        }
+        (void)hipGetLastError(); /* fall back to pageable memory */
+    } else {
+        vnec_clear_error(); /* no device: host memory only */
+    }
+    return aligned64(bytes);
+}
+
 SpB_Info SpB_Vector_new(SpB_Vector *v, SpB_Type type, SpB_Index n)
 {
     if (!v) return vnec_fail("SpB_Vector_new: NULL handle");
@@ -496,7 +517,7 @@ SpB_Info SpB_Vector_new(SpB_Vector *v, SpB_Type type, SpB_Index n)
     d->base.format_type = SpB_DENSE;
     d->base.n = n;
     d->base.element_size = type == SpB_FP32 ? sizeof(float) : sizeof(double);
-    d->values = aligned64(d->base.element_size * (size_t)n);
+    d->values = vector_host_alloc(d->base.element_size * (size_t)n, &d->pinned);
     if (!d->values) {
         free(d);
         return vnec_fail("SpB_Vector_new: out of memory");
@@ -514,7 +535,10 @@ SpB_Info SpB_Vector_free(SpB_Vector *v)
     spb_dense *d = (spb_dense *)*v;
     if (d->magic != SPB_MAGIC_VECTOR) return vnec_fail("SpB_Vector_free: not a vector");
     vnec_free_dev(d->d_values);
-    free(d->values);
+    if (d->pinned)
+        (void)hipHostFree(d->values);
+    else
+        free(d->values);
     d->magic = 0;
     free(d);
     *v = NULL;

@@ -68,6 +68,7 @@ typedef struct spb_dense {
     int resident;   /* device copy is authoritative between calls      */
     int host_stale; /* device holds newer values than host             */
     int dev_stale;  /* host holds newer values than device             */
+    int pinned;     /* values come from hipHostMalloc (page-locked)    */
 } spb_dense;
 
 struct vnec_plan_s {
