@@ -237,7 +237,7 @@ SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *i
 
 /* Candidate configurations (single right-hand side, FP64).  Copies up to
  * max of them to out (may be NULL); returns VNEC_NCAND. */
-#define VNEC_NCAND 17
+#define VNEC_NCAND 21
 int vnec_candidates_ext(vnec_tuning *out, int max);
 /* Which candidates may run on A: bit c set when candidate c applies
  * (slice candidates need padding <= 3 and fewer than 2^31 slots; D

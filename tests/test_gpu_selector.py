@@ -158,17 +158,18 @@ def test_selection_modes_parity(mode):
         p = spb.Plan(A, mode)
         info = p.info()
         want = spb.candidates()[c]
-        if info.variant == "C":
-            # the same merge-path plan with column-sorted tiles: learned and
-            # rule modes switch to it where columns are reused, autotune mode
-            # where it measures faster
-            assert mode in ("learned", "rule", "autotune") and want.variant == "L"
-            if mode != "autotune":
+        if info.variant != want.variant:
+            # the same merge-path plan, CSR-ordered (L) or column-sorted (C):
+            # the rule mode follows the column-reuse rule, the autotune mode
+            # measures both (the learned mode does so only beyond its
+            # training sizes, 2^24 nonzeros, which these matrices are not)
+            assert {info.variant, want.variant} == {"L", "C"} and info.ipt == want.ipt
+            assert mode in ("rule", "autotune")
+            if mode == "rule" and info.variant == "C":
                 ints = spb.feature_ints(A)
                 assert ints[30] <= 0.8 * ints[2]
-            assert info.ipt == want.ipt
         else:
-            assert (info.variant, info.ipt) == (want.variant, want.ipt)
+            assert info.ipt == want.ipt
         xv = spb.Vector.from_array(m.x)
         yv = spb.Vector(m.nrows)
         p.spmv(yv, xv)

@@ -238,7 +238,7 @@ SIGNATURES = {
 }
 
 # Constants of include/vnec_ext.h used by the selector bindings.
-NCAND = 17
+NCAND = 21
 NFEAT = 32
 NFEAT_INTS = 32
 MLP_HIDDEN = 64

@@ -53,8 +53,10 @@ typedef struct spb_csr {
     double *d_val;      /* nnz, FP64 (FP32 values widened)   */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
     int tuned_cand;     /* autotuner's best candidate, -1 until measured */
-    int tuned_csort;    /* its L plan measured against variant C: 1 C is
-                           faster, 0 not, -1 not measured              */
+    int refined_ipt;    /* merge-path neighbourhood measured at this ipt
+                           (0: not yet): the fastest of L / C under
+                           both schedules                            */
+    int refined_variant, refined_grid;
     int feat_ready;     /* feat_ints holds the selector's statistics     */
     uint64_t feat_ints[VNEC_NFEAT_INTS];
 } spb_csr;
@@ -179,9 +181,16 @@ double vnec_slice_padding(const spb_csr *a);
  * granularity autotuned in VNEC_SELECT_AUTOTUNE mode, else 5 items per
  * thread). */
 SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t);
-/* Times the FP64 merge-path plan l against the same plan with column-sorted
- * tiles (variant C); *use_c = 1 when C is faster.  Cached on the matrix. */
-SpB_Info vnec_autotune_csort(spb_csr *a, const vnec_tuning *l, int *use_c);
+/* Times the FP64 merge-path plan t (variant L or C) in its four forms —
+ * CSR-ordered (L) or column-sorted (C) tiles, persistent (grid 0) or
+ * per-tile carries (grid -1) — at t's granularity, and sets t's variant
+ * and grid to the fastest.  Cached on the matrix. */
+SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
+/* The structural rules of the learned and rule modes: most rows fit one
+ * column-offset pattern (variant G applies), and tiles reuse columns
+ * (variant C beats L).  FP64 matrices. */
+int vnec_pattern_rule(spb_csr *a);
+int vnec_csort_rule(spb_csr *a);
 
 /* ---- compiled-in selector model (spb_selector_model.c, generated) ---- */
 extern const int vnec_model_trained;

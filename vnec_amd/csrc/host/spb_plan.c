@@ -16,6 +16,8 @@ static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
  * strided persistent launch (its pipelining pays once a workgroup sees
  * many tiles; below, the one-workgroup-per-tile launch is as fast). */
 #define STRIDED_MIN_TILES_PER_WG 16
+/* The largest matrices the compiled selector model was trained on. */
+#define SELECTOR_TRAINED_MAX_NNZ ((int64_t)1 << 24)
 static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a);
 
 static void plan_release(struct vnec_plan_s *p)
@@ -343,25 +345,31 @@ static SpB_Info build_hot(struct vnec_plan_s *p, const spb_csr *a)
 #define G_MIN_NNZ (1 << 20)
 /* Skewed column use: at most this many distinct columns per nonzero over
  * merge-path windows of 16384 items (R-MAT graphs: 0.45-0.78; uniform and
- * power-law-row matrices: 0.97-1.0), on at least 2^20 nonzeros (below, the
- * sorted staging costs more than the gathers save). */
+ * power-law-row matrices: 0.97-1.0), scattered over at least this many runs
+ * of consecutive columns per nonzero (R-MAT: 0.29-0.65; banded matrices and
+ * stencils, whose tiles already gather contiguous columns: 0.00), on at
+ * least 2^20 nonzeros (below, the sorted staging costs more than the
+ * gathers save). */
 #define CSORT_MAX_DISTINCT 0.8
+#define CSORT_MIN_RUNS 0.2
 #define CSORT_MIN_NNZ (1 << 20)
 
 /* A merge-path plan on a matrix whose tiles reuse columns stores each tile
  * in column order (variant C): the wavefront's gathers then share cache
  * lines among the most used columns. */
-static int csort_rule(spb_csr *a)
+int vnec_csort_rule(spb_csr *a)
 {
     uint64_t v[VNEC_NFEAT_INTS];
     if (vnec_feature_ints_ext((SpB_Matrix)a, v) != SpB_SUCCESS) {
         vnec_clear_error(); /* no statistics: keep the selector's plan */
         return 0;
     }
-    return v[VF_NNZ] >= CSORT_MIN_NNZ && (double)v[VF_NEC_LARGE] <= CSORT_MAX_DISTINCT * (double)v[VF_NNZ];
+    const double z = (double)v[VF_NNZ];
+    return v[VF_NNZ] >= CSORT_MIN_NNZ && (double)v[VF_NEC_LARGE] <= CSORT_MAX_DISTINCT * z &&
+           (double)v[VF_RUNS_LARGE] >= CSORT_MIN_RUNS * z;
 }
 
-static int pattern_rule(spb_csr *a)
+int vnec_pattern_rule(spb_csr *a)
 {
     if (a->base.element_type != SpB_FP64 || (int64_t)a->nnz < G_MIN_NNZ) return 0;
     int64_t nreg = 0;
@@ -398,7 +406,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         tune = *t;
     } else if ((!t || ((t->variant == VNEC_SELECT_LEARNED || t->variant == VNEC_SELECT_RULE) &&
                        t->encoding == VNEC_ENC_FP64 && t->nrhs <= 1)) &&
-               pattern_rule(a)) {
+               vnec_pattern_rule(a)) {
         /* most rows follow one column-offset pattern: pattern slices move
          * 8 bytes per regular nonzero, fewer than any column-indexed
          * candidate */
@@ -409,21 +417,29 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (vnec_select_tuning(a, t ? t->variant : VNEC_SELECT_LEARNED, nrhs, &tune) != SpB_SUCCESS)
             return SpB_FAILURE;
         if (t) tune.encoding = t->encoding;
-        if (tune.variant != VNEC_VAR_S &&
-            !vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs)) {
-            tune.variant = VNEC_VAR_L; /* the encodings every merge kernel has */
+        /* a candidate without a kernel for the requested encoding falls back
+         * to the merge-path kernel, which has them all */
+        const int ok = tune.variant == VNEC_VAR_S ? 1
+                       : tune.variant == VNEC_VAR_D ? tune.encoding == VNEC_ENC_FP64
+                       : vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs);
+        if (!ok) {
+            tune.variant = VNEC_VAR_L;
             tune.ipt = 7;
         }
         const int mode = t ? t->variant : VNEC_SELECT_LEARNED;
-        if (tune.variant == VNEC_VAR_L && tune.nrhs == 1 && tune.encoding == VNEC_ENC_FP64) {
-            /* column-sorted tiles at the same granularity and schedule: by
-             * the column-reuse rule, or measured in autotune mode */
-            int use_c = 0;
-            if (mode == VNEC_SELECT_LEARNED || mode == VNEC_SELECT_RULE)
-                use_c = csort_rule(a);
-            else if (mode == VNEC_SELECT_AUTOTUNE && vnec_autotune_csort(a, &tune, &use_c) != SpB_SUCCESS)
-                return SpB_FAILURE;
-            if (use_c) tune.variant = VNEC_VAR_C;
+        if ((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.nrhs == 1 &&
+            tune.encoding == VNEC_ENC_FP64) {
+            /* CSR-ordered or column-sorted tiles, persistent or with
+             * per-tile carries, at the chosen granularity: the rule mode
+             * asks the column-reuse rule; the autotune mode, and the
+             * learned mode beyond the matrix sizes its model was trained
+             * on, measure the four forms */
+            if (mode == VNEC_SELECT_RULE) {
+                tune.variant = vnec_csort_rule(a) ? VNEC_VAR_C : VNEC_VAR_L;
+            } else if (mode == VNEC_SELECT_AUTOTUNE ||
+                       (mode == VNEC_SELECT_LEARNED && (int64_t)a->nnz > SELECTOR_TRAINED_MAX_NNZ)) {
+                if (vnec_refine_merge_plan(a, &tune) != SpB_SUCCESS) return SpB_FAILURE;
+            }
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;

@@ -27,6 +27,8 @@ static const vnec_tuning CANDIDATES[VNEC_NCAND] = {
     {VNEC_VAR_L, 11, 64, 0, 1, -1}, {VNEC_VAR_E, 5, 64, 0, 1, -1},  {VNEC_VAR_E, 7, 64, 0, 1, -1},
     {VNEC_VAR_E, 5, 64, 0, 1, 0},   {VNEC_VAR_E, 7, 64, 0, 1, 0},
     {VNEC_VAR_D, 7, 4, 0, 1, -1},   {VNEC_VAR_D, 7, 8, 0, 1, -1},   {VNEC_VAR_D, 7, 16, 0, 1, -1},
+    {VNEC_VAR_C, 9, 64, 0, 1, 0},   {VNEC_VAR_C, 11, 64, 0, 1, 0},  {VNEC_VAR_C, 7, 64, 0, 1, -1},
+    {VNEC_VAR_C, 11, 64, 0, 1, -1},
 };
 /* Candidates named by the fixed selection modes. */
 #define CAND_S3 1
@@ -636,10 +638,11 @@ static SpB_Info autotune_multi(spb_csr *a, int nrhs, int *ipt)
     return rc;
 }
 
-SpB_Info vnec_autotune_csort(spb_csr *a, const vnec_tuning *l, int *use_c)
+SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
 {
-    if (a->tuned_csort >= 0) {
-        *use_c = a->tuned_csort;
+    if (a->refined_ipt == t->ipt) {
+        t->variant = a->refined_variant;
+        t->grid = a->refined_grid;
         return SpB_SUCCESS;
     }
     int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
@@ -650,28 +653,38 @@ SpB_Info vnec_autotune_csort(spb_csr *a, const vnec_tuning *l, int *use_c)
     for (int64_t i = 0; rc == SpB_SUCCESS && i < nc; i++) hx[i] = 1.0 / (double)(1 + (i % 7));
     if (rc == SpB_SUCCESS && nc)
         rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
-    double t[2] = {INFINITY, INFINITY};
-    for (int v = 0; rc == SpB_SUCCESS && v < 2; v++) {
-        vnec_tuning u = *l;
-        u.variant = v ? VNEC_VAR_C : VNEC_VAR_L;
+    static const int VARIANTS[2] = {VNEC_VAR_L, VNEC_VAR_C}, GRIDS[2] = {0, -1};
+    double best = INFINITY;
+    int bv = t->variant, bg = t->grid;
+    for (int k = 0; rc == SpB_SUCCESS && k < 4; k++) {
+        vnec_tuning u = *t;
+        u.variant = VARIANTS[k / 2];
+        u.grid = GRIDS[k % 2];
         vnec_plan p = NULL;
         if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &u) != SpB_SUCCESS) {
-            vnec_clear_error(); /* stays INFINITY */
+            vnec_clear_error(); /* not a contender */
             continue;
         }
+        double tb = INFINITY;
         for (int r = 0; rc == SpB_SUCCESS && r < 3; r++) {
             double ms = 0.0;
             rc = vnec_time_dev_ext(p, dx, dy, 20, 1, 0, &ms);
-            if (rc == SpB_SUCCESS && ms < t[v]) t[v] = ms;
+            if (rc == SpB_SUCCESS && ms < tb) tb = ms;
         }
         vnec_plan_free_ext(&p);
+        if (tb < best) {
+            best = tb;
+            bv = u.variant;
+            bg = u.grid;
+        }
     }
     free(hx);
     vnec_free_dev(dx);
     vnec_free_dev(dy);
     if (rc != SpB_SUCCESS) return rc;
-    a->tuned_csort = t[1] < t[0];
-    *use_c = a->tuned_csort;
+    a->refined_ipt = t->ipt;
+    a->refined_variant = t->variant = bv;
+    a->refined_grid = t->grid = bg;
     return SpB_SUCCESS;
 }
 

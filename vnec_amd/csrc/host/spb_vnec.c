@@ -406,11 +406,18 @@ static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
     vnec_tuning t = {VNEC_SELECT_LEARNED, 7, 64, fp32 ? VNEC_ENC_FP32 : VNEC_ENC_FP64, 1, -1};
     if (fp32) t.variant = VNEC_VAR_L; /* grid -1: one workgroup per tile */
     if (kind == KIND_L) {
-        t.variant = VNEC_VAR_L;
+        /* merge-path CSR; FP64 tiles stored in column order where columns
+         * are reused (the same algorithm, fewer lines per gather) */
+        t.variant = !fp32 && vnec_csort_rule(a) ? VNEC_VAR_C : VNEC_VAR_L;
         t.grid = fp32 ? -1 : 0;
     } else if (kind == KIND_D || kind == KIND_D32) {
         t.variant = VNEC_VAR_D;
         t.lanes = fp32 ? REF_L32 : REF_L;
+    } else if (!fp32 && vnec_pattern_rule(a)) {
+        /* rows of one column-offset pattern: pattern slices, the slice
+         * layout without column indices */
+        t.variant = VNEC_VAR_G;
+        t.ipt = 8;
     } else {
         uint32_t mask = 0;
         if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return NULL;
