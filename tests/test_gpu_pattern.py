@@ -187,3 +187,19 @@ def test_g_refuses_what_it_cannot_store():
     E = spb.Matrix.from_csr(4, 4, np.zeros(5, np.int64), np.zeros(0, np.int32), np.zeros(0))
     with pytest.raises(RuntimeError, match="no row pattern"):
         spb.Plan(E, "G")
+
+
+@pytest.mark.parametrize("enc", ["hilo", "hi"])
+def test_g_encoded_sub_plan_for_every_selector_pick(enc):
+    """The irregular rows' sub-plan takes the selector's pick in the G plan's
+    encoding; picks that store FP64 values only (C, D, H) run as the
+    merge-path kernel instead.  A planted matrix whose irregular rows are
+    R-MAT-like exercises it."""
+    m = planted(n=60000, frac_irregular=0.4, seed=21)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    p = spb.Plan(A, "G", encoding=enc)
+    y = spb.Vector(m.nrows)
+    p.spmv(y, spb.Vector.from_array(m.x))
+    ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    assert O.check_rows(y.to_array(), yl, ax, TOL if enc == "hilo" else 2.0 ** -23)[0]
+    p.free()
