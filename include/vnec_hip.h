@@ -264,7 +264,17 @@ typedef struct {
     int64_t nslices;       /* ceil(nrows / 64)                                */
     const double *x;
     double *y;
+    /* x window of a workgroup's 256 rows r0 .. r0 + 255: the union of the
+       column ranges r0 + pat[l] + [0, 256), as nwin runs (offset wst[k] from
+       r0, length wlen[k], LDS start wbase[k]), staged in LDS once; position
+       l of local row t reads LDS entry wpos[l] + t.  wsize = 0: direct x
+       reads.  ncols bounds the staged columns. */
+    int nwin, wsize;
+    int32_t wst[VNEC_PATTERN_MAX], wlen[VNEC_PATTERN_MAX], wbase[VNEC_PATTERN_MAX], wpos[VNEC_PATTERN_MAX];
+    int64_t ncols;
 } vnec_pattern_args;
+/* Largest x window staged in LDS (doubles). */
+#define VNEC_PATTERN_WIN_MAX 4096
 /* masks[i] = positions of the pattern row i holds; flags[i] = 1 when row i
  * is not a subset of the pattern (a column outside it, or repeated). */
 int vnec_hip_pattern_match(const int32_t *ptr, const int32_t *col, int64_t nrows, const int32_t *pat,

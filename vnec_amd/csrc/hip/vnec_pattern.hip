@@ -94,13 +94,30 @@ __device__ __forceinline__ double slot_value(const vnec_pattern_args &a, int64_t
 }
 
 // One wavefront per slice, four slices per workgroup, XCD-aware order.
-template <int U, int ENC>
+// WIN: the workgroup first stages its x window in LDS (a.wsize doubles,
+// coalesced runs), so the rows' x reads, each entry shared by up to L rows
+// of neighbouring slices, are LDS reads instead of L2 requests.
+template <int U, int ENC, bool WIN>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
 {
+    extern __shared__ double s_xw[];
     const int64_t b = xcd_tile(blockIdx.x, grid);
+    const int64_t r0 = b * VNEC_BLOCK;  // first row of the workgroup
+    if (r0 >= a.nrows) return;           // whole workgroup
+    if (WIN) {
+        for (int k = 0; k < a.nwin; k++) {
+            const int64_t c0 = r0 + a.wst[k];
+            for (int i = threadIdx.x; i < a.wlen[k]; i += VNEC_BLOCK) {
+                const int64_t c = c0 + i;
+                s_xw[a.wbase[k] + i] = c >= 0 && c < a.ncols ? a.x[c] : 0.0;
+            }
+        }
+        __syncthreads();
+    }
     const int64_t s = b * VNEC_NWAVES + wave_id();
     if (s >= a.nslices) return;
-    const int64_t row = s * 64 + lane_id();
+    const int t = (int)threadIdx.x;  // local row
+    const int64_t row = r0 + t;
     const bool live = row < a.nrows;
     const uint32_t m = live ? __builtin_nontemporal_load(a.masks + row) : 0u;
     const int64_t v = s * (int64_t)a.L * 64 + lane_id();  // slot (slice s, l = 0, this lane)
@@ -112,13 +129,14 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a
 #pragma unroll
         for (int u = 0; u < U; u++) vv[u] = slot_value<ENC>(a, v + (int64_t)(l + u) * 64);
 #pragma unroll
-        for (int u = 0; u < U; u++) xv[u] = ((m >> (l + u)) & 1u) ? x[row + a.pat[l + u]] : 0.0;
+        for (int u = 0; u < U; u++)
+            xv[u] = ((m >> (l + u)) & 1u) ? (WIN ? s_xw[a.wpos[l + u] + t] : x[row + a.pat[l + u]]) : 0.0;
 #pragma unroll
         for (int u = 0; u < U; u++) acc += vv[u] * xv[u];
     }
     for (; l < a.L; l++) {
         const double vv = slot_value<ENC>(a, v + (int64_t)l * 64);
-        if ((m >> l) & 1u) acc += vv * x[row + a.pat[l]];
+        if ((m >> l) & 1u) acc += vv * (WIN ? s_xw[a.wpos[l] + t] : x[row + a.pat[l]]);
     }
     if (live) a.y[row] = acc;
 }
@@ -175,10 +193,16 @@ extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
     if (a->encoding < 0 || a->encoding > 2) return (int)hipErrorInvalidValue;
     const int64_t blocks = (a->nslices + VNEC_NWAVES - 1) / VNEC_NWAVES;
     const int64_t grid = xcd_grid(blocks);
-    void (*k)(vnec_pattern_args, int64_t) = a->encoding == 0   ? k_spmv_pattern<8, 0>
-                                            : a->encoding == 1 ? k_spmv_pattern<8, 1>
-                                                               : k_spmv_pattern<8, 2>;
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, grid);
+    if (a->wsize < 0 || a->wsize > VNEC_PATTERN_WIN_MAX || a->nwin < 0 || a->nwin > a->L)
+        return (int)hipErrorInvalidValue;
+    const bool win = a->wsize > 0;
+    void (*k)(vnec_pattern_args, int64_t) =
+        win ? (a->encoding == 0 ? k_spmv_pattern<8, 0, true> : a->encoding == 1 ? k_spmv_pattern<8, 1, true>
+                                                                                 : k_spmv_pattern<8, 2, true>)
+            : (a->encoding == 0 ? k_spmv_pattern<8, 0, false> : a->encoding == 1 ? k_spmv_pattern<8, 1, false>
+                                                                                  : k_spmv_pattern<8, 2, false>);
+    const size_t lds = win ? sizeof(double) * (size_t)a->wsize : 0;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), lds, (hipStream_t)stream, *a, grid);
     return (int)hipGetLastError();
 }
 

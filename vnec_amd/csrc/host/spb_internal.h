@@ -112,6 +112,7 @@ struct vnec_plan_s {
     int nhot;
     /* variant G: pattern slices + a sub-plan for the irregular rows */
     int32_t *g_pat;     /* L offsets                                  */
+    int32_t g_pat_h[VNEC_PATTERN_MAX]; /* the same offsets on the host */
     int g_L;
     uint32_t *g_masks;  /* nrows                                      */
     int64_t g_nslices;
@@ -190,6 +191,10 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
  * column-offset pattern (variant G applies), and tiles reuse columns
  * (variant C beats L).  FP64 matrices. */
 int vnec_pattern_rule(spb_csr *a);
+/* Fills g's x-window fields for pattern pat (L ascending offsets): the
+ * merged column runs a workgroup's 256 rows read, staged in LDS when they
+ * total at most VNEC_PATTERN_WIN_MAX entries (else wsize = 0). */
+void vnec_pattern_window(const int32_t *pat, int L, vnec_pattern_args *g);
 int vnec_csort_rule(spb_csr *a);
 
 /* ---- compiled-in selector model (spb_selector_model.c, generated) ---- */

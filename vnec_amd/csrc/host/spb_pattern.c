@@ -164,6 +164,31 @@ static SpB_Info irregular_submatrix(const spb_csr *a, const int32_t *irr, int64_
     return rc;
 }
 
+void vnec_pattern_window(const int32_t *pat, int L, vnec_pattern_args *g)
+{
+    const int R = 256; /* rows per workgroup of the pattern kernel */
+    int nwin = 0, wsize = 0, win_of[VNEC_PATTERN_MAX];
+    for (int l = 0; l < L; l++) { /* offsets ascend: merge overlapping or adjacent runs */
+        const int64_t lo = pat[l], hi = (int64_t)pat[l] + R; /* [lo, hi) */
+        if (nwin > 0 && lo <= (int64_t)g->wst[nwin - 1] + g->wlen[nwin - 1]) {
+            const int64_t end = (int64_t)g->wst[nwin - 1] + g->wlen[nwin - 1];
+            if (hi > end) g->wlen[nwin - 1] = (int32_t)(hi - g->wst[nwin - 1]);
+        } else {
+            g->wst[nwin] = pat[l];
+            g->wlen[nwin] = R;
+            nwin++;
+        }
+        win_of[l] = nwin - 1;
+    }
+    for (int k = 0; k < nwin; k++) {
+        g->wbase[k] = wsize;
+        wsize += g->wlen[k];
+    }
+    for (int l = 0; l < L; l++) g->wpos[l] = g->wbase[win_of[l]] + (pat[l] - g->wst[win_of[l]]);
+    g->nwin = nwin;
+    g->wsize = wsize <= VNEC_PATTERN_WIN_MAX ? wsize : 0;
+}
+
 SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
 {
     hipStream_t s = vnec_stream();
@@ -173,6 +198,7 @@ SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
     if (L < 1) return vnec_fail("variant G: no row pattern (sample rows are empty or longer than %d)",
                                 VNEC_PATTERN_MAX);
     p->g_L = L;
+    memcpy(p->g_pat_h, pat, sizeof(int32_t) * (size_t)L);
     p->g_nslices = (nr + 63) / 64;
     p->g_pat = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)L, &p->dev_bytes);
     p->g_masks = (uint32_t *)vnec_malloc_dev(sizeof(uint32_t) * (size_t)(nr ? nr : 1), &p->dev_bytes);

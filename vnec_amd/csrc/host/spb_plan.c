@@ -753,6 +753,8 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, con
         g.nslices = p->g_nslices;
         g.x = (const double *)x_dev;
         g.y = (double *)y_dev;
+        g.ncols = p->ncols;
+        vnec_pattern_window(p->g_pat_h, p->g_L, &g);
         return vnec_hip_ok(vnec_hip_spmv_pattern(&g, s), "pattern kernel");
     }
     if (p->tune.variant == VNEC_VAR_S) return launch_slice(p, x_dev, y_dev, s);
