@@ -256,7 +256,9 @@ typedef struct {
                               (encoding 0)                                     */
     const float *gh, *gl;  /* the same slots as FP32 hi and lo (encoding 1), or
                               hi only (encoding 2); summed in FP64             */
-    int encoding;          /* 0 FP64, 1 FP32 hi + lo, 2 FP32 hi               */
+    int encoding;          /* 0 FP64, 1 FP32 hi + lo, 2 FP32 hi (FP64 sums);
+                              3 the FP32 API: FP32 values in gh, FP32 x32,
+                              y32 and sums                                    */
     const int32_t *pat;    /* L column offsets: position l of row i is column
                               i + pat[l]                                      */
     int L;
@@ -272,6 +274,8 @@ typedef struct {
     int nwin, wsize;
     int32_t wst[VNEC_PATTERN_MAX], wlen[VNEC_PATTERN_MAX], wbase[VNEC_PATTERN_MAX], wpos[VNEC_PATTERN_MAX];
     int64_t ncols;
+    const float *x32;      /* encoding 3                                      */
+    float *y32;
 } vnec_pattern_args;
 /* Largest x window staged in LDS (doubles). */
 #define VNEC_PATTERN_WIN_MAX 4096
@@ -291,6 +295,7 @@ int vnec_hip_pattern_fill(const int32_t *ptr, const double *val, const uint32_t 
 int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream);
 /* y[rows[i]] = ysub[i], i < n. */
 int vnec_hip_scatter_rows(const double *ysub, const int32_t *rows, int64_t n, double *y, void *stream);
+int vnec_hip_scatter_rows_f32(const float *ysub, const int32_t *rows, int64_t n, float *y, void *stream);
 
 /* ---- VNEC-S task order and reordered matrix (builder) ---- */
 /* Stable sort of n (key, value) pairs by the low key_bits bits of the keys

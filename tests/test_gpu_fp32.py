@@ -178,5 +178,5 @@ def test_fp32_entry_points_refuse_mixed_types():
         spb.Plan(A32, "L", 7, encoding="fp64")
     with pytest.raises(RuntimeError, match="SpB_FP32"):
         spb.Plan(A64, "L", 7, encoding="fp32")
-    with pytest.raises(RuntimeError, match="variants L, S and D"):
+    with pytest.raises(RuntimeError, match="variants L, S, D and G"):
         spb.Plan(A32, "E", 7)

@@ -179,11 +179,30 @@ def test_g_value_encodings(name, m, enc, tol):
     p.free()
 
 
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+def test_g_fp32_api(name, m):
+    """The FP32 API's arithmetic (FP32 values, x, y and sums) in pattern
+    slices, irregular rows through an FP32 sub-plan: the FP32 gate
+    |y - y_ref| <= max(1e-4, 2 (len_i + 1) 2^-24) (|A||x|)_i against the FP32
+    serial oracle, and bit-identical repeats."""
+    v32, x32 = m.val.astype(np.float32), m.x.astype(np.float32)
+    A32 = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, v32, dtype=np.float32)
+    print(f"case {name} variant G FP32", flush=True)
+    p = spb.Plan(A32, "G")
+    assert p.info().encoding == "fp32"
+    xv = spb.Vector.from_array(x32, np.float32)
+    y = spb.Vector(m.nrows, np.float32)
+    p.spmv(y, xv)
+    yy = y.to_array()
+    ax, _ = O.spmv_abs(m.ptr, m.col, v32.astype(np.float64), x32.astype(np.float64))
+    ok, worst = O.check_rows_f32(yy, O.spmv_serial_f32(m.ptr, m.col, v32, x32), m.ptr, ax)
+    assert ok, worst
+    p.spmv(y, xv)
+    assert np.array_equal(y.to_array(), yy)
+    p.free()
+
+
 def test_g_refuses_what_it_cannot_store():
-    m = g.stencil27(8, seed=12)
-    A32 = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val.astype(np.float32), dtype=np.float32)
-    with pytest.raises(RuntimeError, match="VNEC_ENC_FP32: variants L, S and D"):
-        spb.Plan(A32, "G")
     E = spb.Matrix.from_csr(4, 4, np.zeros(5, np.int64), np.zeros(0, np.int32), np.zeros(0))
     with pytest.raises(RuntimeError, match="no row pattern"):
         spb.Plan(E, "G")

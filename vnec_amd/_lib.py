@@ -211,6 +211,7 @@ SIGNATURES = {
     "vnec_hip_pattern_fill": (C.c_int, [vp, vp, vp, i64, C.c_int, vp, vp]),
     "vnec_hip_spmv_pattern": (C.c_int, [vp, vp]),
     "vnec_hip_scatter_rows": (C.c_int, [vp, vp, i64, vp, vp]),
+    "vnec_hip_scatter_rows_f32": (C.c_int, [vp, vp, i64, vp, vp]),
     "vnec_hip_zero_f32": (C.c_int, [vp, i64, vp]),
     "vnec_hip_widen_f32": (C.c_int, [vp, i64, vp, vp]),
     "vnec_hip_hot_supported": (C.c_int, [C.c_int, C.c_int]),

@@ -83,15 +83,20 @@ __global__ void k_pattern_fill(const int32_t *__restrict__ ptr, const double *__
     }
 }
 
-// Slot value k of the slices in FP64: stored FP64, FP32 hi + lo, or hi.
+// Slot value k of the slices in the kernel's arithmetic: FP64 from stored
+// FP64, FP32 hi + lo or hi; FP32 for the FP32 API (encoding 3).
 template <int ENC>
-__device__ __forceinline__ double slot_value(const vnec_pattern_args &a, int64_t k)
+__device__ __forceinline__ typename enc_real<ENC>::type slot_value(const vnec_pattern_args &a, int64_t k)
 {
     if (ENC == 0) return __builtin_nontemporal_load(a.gv + k);
     if (ENC == 1)
         return (double)__builtin_nontemporal_load(a.gh + k) + (double)__builtin_nontemporal_load(a.gl + k);
-    return (double)__builtin_nontemporal_load(a.gh + k);
+    return __builtin_nontemporal_load(a.gh + k);  // hi (widened for ENC 2; FP32 for ENC 3)
 }
+__device__ __forceinline__ const double *pattern_x(const vnec_pattern_args &a, double) { return a.x; }
+__device__ __forceinline__ const float *pattern_x(const vnec_pattern_args &a, float) { return a.x32; }
+__device__ __forceinline__ double *pattern_y(const vnec_pattern_args &a, double) { return a.y; }
+__device__ __forceinline__ float *pattern_y(const vnec_pattern_args &a, float) { return a.y32; }
 
 // One wavefront per slice, four slices per workgroup, XCD-aware order.
 // WIN: the workgroup first stages its x window in LDS (a.wsize doubles,
@@ -100,7 +105,10 @@ __device__ __forceinline__ double slot_value(const vnec_pattern_args &a, int64_t
 template <int U, int ENC, bool WIN>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
 {
-    extern __shared__ double s_xw[];
+    typedef typename enc_real<ENC>::type F;
+    extern __shared__ double s_dyn[];
+    F *s_xw = (F *)s_dyn;
+    const F *__restrict__ x = pattern_x(a, F());
     const int64_t b = xcd_tile(blockIdx.x, grid);
     const int64_t r0 = b * VNEC_BLOCK;  // first row of the workgroup
     if (r0 >= a.nrows) return;           // whole workgroup
@@ -109,7 +117,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a
             const int64_t c0 = r0 + a.wst[k];
             for (int i = threadIdx.x; i < a.wlen[k]; i += VNEC_BLOCK) {
                 const int64_t c = c0 + i;
-                s_xw[a.wbase[k] + i] = c >= 0 && c < a.ncols ? a.x[c] : 0.0;
+                s_xw[a.wbase[k] + i] = c >= 0 && c < a.ncols ? x[c] : (F)0;
             }
         }
         __syncthreads();
@@ -121,28 +129,28 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a
     const bool live = row < a.nrows;
     const uint32_t m = live ? __builtin_nontemporal_load(a.masks + row) : 0u;
     const int64_t v = s * (int64_t)a.L * 64 + lane_id();  // slot (slice s, l = 0, this lane)
-    const double *__restrict__ x = a.x;
-    double acc = 0.0;
+    F acc = 0;
     int l = 0;
     for (; l + U <= a.L; l += U) {
-        double vv[U], xv[U];
+        F vv[U], xv[U];
 #pragma unroll
         for (int u = 0; u < U; u++) vv[u] = slot_value<ENC>(a, v + (int64_t)(l + u) * 64);
 #pragma unroll
         for (int u = 0; u < U; u++)
-            xv[u] = ((m >> (l + u)) & 1u) ? (WIN ? s_xw[a.wpos[l + u] + t] : x[row + a.pat[l + u]]) : 0.0;
+            xv[u] = ((m >> (l + u)) & 1u) ? (WIN ? s_xw[a.wpos[l + u] + t] : x[row + a.pat[l + u]]) : (F)0;
 #pragma unroll
         for (int u = 0; u < U; u++) acc += vv[u] * xv[u];
     }
     for (; l < a.L; l++) {
-        const double vv = slot_value<ENC>(a, v + (int64_t)l * 64);
+        const F vv = slot_value<ENC>(a, v + (int64_t)l * 64);
         if ((m >> l) & 1u) acc += vv * (WIN ? s_xw[a.wpos[l] + t] : x[row + a.pat[l]]);
     }
-    if (live) a.y[row] = acc;
+    if (live) pattern_y(a, F())[row] = acc;
 }
 
-__global__ void k_scatter_rows(const double *__restrict__ ysub, const int32_t *__restrict__ rows,
-                               int64_t n, double *__restrict__ y)
+template <typename F>
+__global__ void k_scatter_rows(const F *__restrict__ ysub, const int32_t *__restrict__ rows, int64_t n,
+                               F *__restrict__ y)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
@@ -190,18 +198,22 @@ extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
 {
     if (a->nslices <= 0) return 0;
     if (a->L < 1 || a->L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
-    if (a->encoding < 0 || a->encoding > 2) return (int)hipErrorInvalidValue;
+    if (a->encoding < 0 || a->encoding > VNEC_ENC_F32_K) return (int)hipErrorInvalidValue;
     const int64_t blocks = (a->nslices + VNEC_NWAVES - 1) / VNEC_NWAVES;
     const int64_t grid = xcd_grid(blocks);
     if (a->wsize < 0 || a->wsize > VNEC_PATTERN_WIN_MAX || a->nwin < 0 || a->nwin > a->L)
         return (int)hipErrorInvalidValue;
     const bool win = a->wsize > 0;
     void (*k)(vnec_pattern_args, int64_t) =
-        win ? (a->encoding == 0 ? k_spmv_pattern<8, 0, true> : a->encoding == 1 ? k_spmv_pattern<8, 1, true>
-                                                                                 : k_spmv_pattern<8, 2, true>)
-            : (a->encoding == 0 ? k_spmv_pattern<8, 0, false> : a->encoding == 1 ? k_spmv_pattern<8, 1, false>
-                                                                                  : k_spmv_pattern<8, 2, false>);
-    const size_t lds = win ? sizeof(double) * (size_t)a->wsize : 0;
+        win ? (a->encoding == 0   ? k_spmv_pattern<8, 0, true>
+               : a->encoding == 1 ? k_spmv_pattern<8, 1, true>
+               : a->encoding == 2 ? k_spmv_pattern<8, 2, true>
+                                  : k_spmv_pattern<8, VNEC_ENC_F32_K, true>)
+            : (a->encoding == 0   ? k_spmv_pattern<8, 0, false>
+               : a->encoding == 1 ? k_spmv_pattern<8, 1, false>
+               : a->encoding == 2 ? k_spmv_pattern<8, 2, false>
+                                  : k_spmv_pattern<8, VNEC_ENC_F32_K, false>);
+    const size_t lds = win ? (a->encoding == VNEC_ENC_F32_K ? sizeof(float) : sizeof(double)) * (size_t)a->wsize : 0;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), lds, (hipStream_t)stream, *a, grid);
     return (int)hipGetLastError();
 }
@@ -210,7 +222,16 @@ extern "C" int vnec_hip_scatter_rows(const double *ysub, const int32_t *rows, in
                                      void *stream)
 {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(k_scatter_rows, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, ysub, rows,
-                       n, y);
+    hipLaunchKernelGGL(k_scatter_rows<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, ysub,
+                       rows, n, y);
+    return (int)hipGetLastError();
+}
+
+extern "C" int vnec_hip_scatter_rows_f32(const float *ysub, const int32_t *rows, int64_t n, float *y,
+                                         void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_scatter_rows<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, ysub,
+                       rows, n, y);
     return (int)hipGetLastError();
 }

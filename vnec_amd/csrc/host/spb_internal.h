@@ -122,7 +122,7 @@ struct vnec_plan_s {
     int64_t g_nirr;
     SpB_Matrix g_sub;   /* CSR of the irregular rows (owned)          */
     struct vnec_plan_s *g_subplan;
-    double *g_ysub;     /* nirr                                       */
+    double *g_ysub;     /* nirr (floats for FP32 plans)               */
     /* variant C: each tile's nonzeros in column order */
     double *cs_val;     /* nnz                                        */
     int32_t *cs_col;    /* nnz                                        */

@@ -134,11 +134,13 @@ SpB_Info vnec_pattern_match_ext(const SpB_Matrix A, const int32_t *pattern, int 
  * library; the sub-plan runs on it. */
 static SpB_Info irregular_submatrix(const spb_csr *a, const int32_t *irr, int64_t nirr, SpB_Matrix *out)
 {
+    const int fp32 = a->base.element_type == SpB_FP32; /* the sub-matrix keeps the value type */
+    const size_t es = fp32 ? sizeof(float) : sizeof(double);
     int64_t nnz = 0;
     for (int64_t k = 0; k < nirr; k++) nnz += (int64_t)(a->ptr[irr[k] + 1] - a->ptr[irr[k]]);
     int64_t *ptr = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nirr + 1));
     int32_t *col = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
-    double *val = (double *)malloc(sizeof(double) * (size_t)(nnz ? nnz : 1));
+    char *val = (char *)malloc(es * (size_t)(nnz ? nnz : 1));
     SpB_Info rc = (ptr && col && val) ? SpB_SUCCESS : vnec_fail("pattern plan: out of memory");
     if (rc == SpB_SUCCESS) {
         int64_t z = 0;
@@ -146,16 +148,17 @@ static SpB_Info irregular_submatrix(const spb_csr *a, const int32_t *irr, int64_
         for (int64_t k = 0; k < nirr; k++) {
             for (SpB_Index j = a->ptr[irr[k]]; j < a->ptr[irr[k] + 1]; j++, z++) {
                 col[z] = (int32_t)a->indices[j];
-                val[z] = ((const double *)a->val)[j];
+                memcpy(val + es * (size_t)z, (const char *)a->val + es * (size_t)j, es);
             }
             ptr[k + 1] = z;
         }
     }
     *out = NULL;
-    if (rc == SpB_SUCCESS) rc = SpB_Matrix_new(out, SpB_FP64, (SpB_Index)nirr, a->base.col);
+    if (rc == SpB_SUCCESS) rc = SpB_Matrix_new(out, fp32 ? SpB_FP32 : SpB_FP64, (SpB_Index)nirr, a->base.col);
     if (rc == SpB_SUCCESS) {
         ((spb_csr *)*out)->base.format_type = SpB_CSR; /* whatever the global default */
-        rc = SpB_Matrix_import_csr_ext(*out, ptr, col, val);
+        rc = fp32 ? SpB_Matrix_import_csr_fp32_ext(*out, ptr, col, (const float *)val)
+                  : SpB_Matrix_import_csr_ext(*out, ptr, col, (const double *)val);
     }
     if (rc != SpB_SUCCESS && *out) SpB_Matrix_free(out);
     free(ptr);
@@ -252,7 +255,14 @@ SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
      * picks for it (explicit tuning: never variant G again) */
     if (rc == SpB_SUCCESS && nirr) rc = irregular_submatrix(a, hirr, nirr, &p->g_sub);
     free(hirr);
-    if (rc == SpB_SUCCESS && nirr) {
+    if (rc == SpB_SUCCESS && nirr && enc == VNEC_ENC_FP32) {
+        /* FP32 rows: the FP32 matrices' default plan (slices or merge-path) */
+        rc = vnec_plan_build_ext(&p->g_subplan, p->g_sub, NULL);
+        if (rc == SpB_SUCCESS) {
+            p->g_ysub = (double *)vnec_malloc_dev(sizeof(float) * (size_t)nirr, &p->dev_bytes);
+            if (!p->g_ysub) rc = SpB_FAILURE;
+        }
+    } else if (rc == SpB_SUCCESS && nirr) {
         vnec_tuning t;
         rc = vnec_select_tuning(spb_as_csr(p->g_sub), VNEC_SELECT_LEARNED, 1, &t);
         if (rc == SpB_SUCCESS && enc != VNEC_ENC_FP64) { /* the same value encoding */

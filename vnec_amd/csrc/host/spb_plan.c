@@ -371,7 +371,7 @@ int vnec_csort_rule(spb_csr *a)
 
 int vnec_pattern_rule(spb_csr *a)
 {
-    if (a->base.element_type != SpB_FP64 || (int64_t)a->nnz < G_MIN_NNZ) return 0;
+    if ((int64_t)a->nnz < G_MIN_NNZ) return 0;
     int64_t nreg = 0;
     if (vnec_pattern_coverage(a, &nreg) != SpB_SUCCESS) {
         vnec_clear_error(); /* no pattern plan: the selector decides */
@@ -454,8 +454,9 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         return vnec_fail(fp32 ? "an FP32 matrix takes encoding VNEC_ENC_FP32"
                               : "encoding VNEC_ENC_FP32 needs a matrix made with SpB_FP32");
     if (fp32) {
-        if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_S && tune.variant != VNEC_VAR_D)
-            return vnec_fail("encoding VNEC_ENC_FP32: variants L, S and D");
+        if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_S && tune.variant != VNEC_VAR_D &&
+            tune.variant != VNEC_VAR_G)
+            return vnec_fail("encoding VNEC_ENC_FP32: variants L, S, D and G");
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
         if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
     }
@@ -754,6 +755,15 @@ static SpB_Info launch_main(struct vnec_plan_s *p, const vnec_merge_args *m, con
         g.x = (const double *)x_dev;
         g.y = (double *)y_dev;
         g.ncols = p->ncols;
+        if (p->tune.encoding == VNEC_ENC_FP32) {
+            g.x = NULL;
+            g.y = NULL;
+            g.x32 = (const float *)x_dev;
+            g.y32 = (float *)y_dev;
+        } else {
+            g.x32 = NULL;
+            g.y32 = NULL;
+        }
         vnec_pattern_window(p->g_pat_h, p->g_L, &g);
         return vnec_hip_ok(vnec_hip_spmv_pattern(&g, s), "pattern kernel");
     }
@@ -777,6 +787,10 @@ static SpB_Info launch_fixup(struct vnec_plan_s *p, const void *x_dev, void *y_d
     if (p->tune.variant == VNEC_VAR_G) { /* the irregular rows */
         if (!p->g_nirr) return SpB_SUCCESS;
         if (vnec_plan_run(p->g_subplan, x_dev, p->g_ysub) != SpB_SUCCESS) return SpB_FAILURE;
+        if (p->tune.encoding == VNEC_ENC_FP32)
+            return vnec_hip_ok(vnec_hip_scatter_rows_f32((const float *)p->g_ysub, p->g_irr, p->g_nirr,
+                                                         (float *)y_dev, s),
+                               "scatter rows");
         return vnec_hip_ok(vnec_hip_scatter_rows(p->g_ysub, p->g_irr, p->g_nirr, (double *)y_dev, s),
                            "scatter rows");
     }

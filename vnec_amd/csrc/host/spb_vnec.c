@@ -413,7 +413,7 @@ static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
     } else if (kind == KIND_D || kind == KIND_D32) {
         t.variant = VNEC_VAR_D;
         t.lanes = fp32 ? REF_L32 : REF_L;
-    } else if (!fp32 && vnec_pattern_rule(a)) {
+    } else if (vnec_pattern_rule(a)) {
         /* rows of one column-offset pattern: pattern slices, the slice
          * layout without column indices */
         t.variant = VNEC_VAR_G;
