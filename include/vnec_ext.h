@@ -83,8 +83,9 @@ typedef enum {
                        the dominant column-offset pattern store a position
                        mask and values only; the rest run a sub-plan       */
     VNEC_VAR_C = 6, /* persistent merge-path CSR whose tiles store their
-                       nonzeros in column order (+ a 16-bit position in
-                       the tile): x is gathered in ascending column order */
+                       nonzeros in column order (+ its offset in the
+                       tile): x is gathered in ascending column order;
+                       FP64, one right-hand side                          */
     VNEC_VAR_N
 } vnec_variant;
 
@@ -94,7 +95,7 @@ typedef enum {
     VNEC_ENC_HI = 2,   /* FP32 hi only, FP64 accumulate                 */
     VNEC_ENC_FP32 = 3, /* the FP32 API: FP32 values, x, y and sums (FP32
                           matrices only; variants L (one workgroup per
-                          tile), S and D, one right-hand side)          */
+                          tile), S, D and G, one right-hand side)       */
     VNEC_ENC_N
 } vnec_encoding;
 
