@@ -157,8 +157,9 @@ SpB_Info vnec_spmm_colmajor_dev_ext(vnec_plan plan, const void *X_dev, void *Y_d
 SpB_Info vnec_time_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                            int use_graph, size_t flush_bytes, double *ms);
 
-/* Mean duration (ms) of the SpMV kernel and of the carry fix-up kernel,
- * each timed between its own HIP events on the library stream. */
+/* Mean duration (ms) of the SpMV kernel and of the carry fix-up kernel:
+ * iters launches of each, back to back between one pair of HIP events on the
+ * library stream.  y holds A·x on return. */
 SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                                double *ms_main, double *ms_fixup);
 

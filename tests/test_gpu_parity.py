@@ -563,3 +563,24 @@ def test_multi_rhs_column_major(k):
     for a in (dx, dy, xi, yi):
         a.free()
     p.free()
+
+
+@pytest.mark.parametrize("variant,ipt,grid", [("L", 7, 0), ("L", 7, -1), ("C", 9, 0),
+                                              ("S", 3, 0), ("D", 7, 0), ("G", 8, 0)])
+def test_time_kernels_leaves_y_right(variant, ipt, grid):
+    """The kernel timer repeats each kernel back to back; y must still come
+    back as A·x, and both means are non-negative (the SpMV kernel's positive)."""
+    m = g.stencil27(24, seed=31) if variant == "G" else g.rmat(13, 16, seed=31)
+    A = _mat(m)
+    p = spb.Plan(A, variant, ipt, lanes=8 if variant == "D" else 64, grid=grid)
+    dx = spb.DeviceArray.from_array(m.x)
+    dy = spb.DeviceArray(8 * m.nrows)
+    k_ms, f_ms = p.time_kernels(dx.ptr, dy.ptr, 25)
+    y = dy.to_array(np.float64, m.nrows)
+    dx.free()
+    dy.free()
+    p.free()
+    assert k_ms > 0 and f_ms >= 0
+    ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, yl, ax, TOL)
+    assert ok, worst

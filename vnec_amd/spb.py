@@ -274,7 +274,7 @@ class Plan:
         return ms.value
 
     def time_kernels(self, x_ptr: int, y_ptr: int, iters: int) -> tuple[float, float]:
-        """(SpMV kernel ms, carry fix-up ms), each between its own events."""
+        """(SpMV kernel ms, carry fix-up ms): mean of iters back-to-back launches."""
         a, b = C.c_double(), C.c_double()
         L.check(L.load().vnec_time_kernels_ext(self.h, x_ptr, y_ptr, iters, C.byref(a),
                                                C.byref(b)), "vnec_time_kernels_ext")
