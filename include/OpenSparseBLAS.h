@@ -80,6 +80,12 @@ extern SpB_Info SpB_Vector_setElement_FP64(SpB_Vector w, double x, SpB_Index i);
 /* include/OpenSparseBLAS.h:730-735, 741-746 */
 extern SpB_Info SpB_Vector_extractElement_FP32(float *x, const SpB_Vector v, SpB_Index i);
 extern SpB_Info SpB_Vector_extractElement_FP64(double *x, const SpB_Vector v, SpB_Index i);
+/* All n entries as (i, v[i]) (SpB_Vector.c:423-457, SpB_Vector_Dense.c:162-184):
+ * *p_nvals must equal n; I or X may be NULL. */
+extern SpB_Info SpB_Vector_extractTuples_FP32(SpB_Index *I, float *X, SpB_Index *p_nvals,
+                                              const SpB_Vector v);
+extern SpB_Info SpB_Vector_extractTuples_FP64(SpB_Index *I, double *X, SpB_Index *p_nvals,
+                                              const SpB_Vector v);
 
 /* include/OpenSparseBLAS.h:971, 981, 992, 1000 */
 SpB_Info SpB_Matrix_nrows(SpB_Index *n, SpB_Matrix m);
@@ -224,5 +230,33 @@ typedef union {
 SpB_Info SpB_Global_Option_set_ext(SpB_Option_Field_ext field, SpB_Object_Format_ext value);
 
 #ifdef __cplusplus
+}
+
+/* C++ overloads by value type (OpenSparseBLAS.h:2651-2701 of the reference,
+ * there defined in SpB_Vector.cpp), for the value types this library's
+ * vectors hold; header-only, so the library itself stays C. */
+inline SpB_Info SpB_Vector_setElement(SpB_Vector v, double x, SpB_Index i)
+{
+    return SpB_Vector_setElement_FP64(v, x, i);
+}
+inline SpB_Info SpB_Vector_setElement(SpB_Vector v, float x, SpB_Index i)
+{
+    return SpB_Vector_setElement_FP32(v, x, i);
+}
+inline SpB_Info SpB_Vector_extractElement(double *x, const SpB_Vector v, SpB_Index i)
+{
+    return SpB_Vector_extractElement_FP64(x, v, i);
+}
+inline SpB_Info SpB_Vector_extractElement(float *x, const SpB_Vector v, SpB_Index i)
+{
+    return SpB_Vector_extractElement_FP32(x, v, i);
+}
+inline SpB_Info SpB_Vector_extractTuples(SpB_Index *I, double *X, SpB_Index *p_nvals, const SpB_Vector v)
+{
+    return SpB_Vector_extractTuples_FP64(I, X, p_nvals, v);
+}
+inline SpB_Info SpB_Vector_extractTuples(SpB_Index *I, float *X, SpB_Index *p_nvals, const SpB_Vector v)
+{
+    return SpB_Vector_extractTuples_FP32(I, X, p_nvals, v);
 }
 #endif

@@ -4,6 +4,7 @@ reference's (struct layout prefix, CSR build order, error returns)."""
 import ctypes as C
 import os
 import re
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -178,3 +179,54 @@ def test_fp32_objects_and_type_checks_on_the_host():
     y64 = spb.Vector(n)
     assert lib.SpB_mxv(y64.h, A.h, x.h, None) == 1 and "FP32" in L.last_error()
     assert lib.IRD_VNEC(A.h, 4) == 1 and "SpB_FP64 or SpB_FP32" in L.last_error()
+
+
+def test_vector_extract_tuples():
+    """SpB_Vector_extractTuples_FP64/FP32 on dense vectors (reference
+    SpB_Vector_Dense.c:162-184): *p_nvals must equal n; (i, v[i]) for all i."""
+    import numpy as np_
+    lib = L.load()
+    n = 7
+    x = spb.Vector(n)
+    for i in range(n):
+        x.set(i, 0.5 * i - 1.0)
+    I = (C.c_uint64 * n)()
+    X = (C.c_double * n)()
+    nv = C.c_uint64(n)
+    assert lib.SpB_Vector_extractTuples_FP64(I, X, C.byref(nv), x.h) == 0
+    assert list(I) == list(range(n)) and list(X) == [0.5 * i - 1.0 for i in range(n)]
+    X2 = (C.c_double * n)()
+    assert lib.SpB_Vector_extractTuples_FP64(None, X2, C.byref(nv), x.h) == 0
+    assert list(X2) == list(X)
+    nv_bad = C.c_uint64(n - 1)
+    assert lib.SpB_Vector_extractTuples_FP64(I, X, C.byref(nv_bad), x.h) == 1
+    assert "p_nvals" in L.last_error()
+    X32 = (C.c_float * n)()
+    assert lib.SpB_Vector_extractTuples_FP32(I, X32, C.byref(nv), x.h) == 1  # FP64 vector
+    y = spb.Vector(n, np_.float32)
+    y.set(2, 2.5)
+    assert lib.SpB_Vector_extractTuples_FP32(None, X32, C.byref(nv), y.h) == 0
+    assert list(X32) == [0, 0, 2.5, 0, 0, 0, 0]
+
+
+def test_cpp_overloads_compile(tmp_path):
+    """The header's C++ overloads resolve by value type (g++, syntax only)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "ov.cpp"
+    src.write_text(
+        '#include "OpenSparseBLAS.h"\n'
+        "SpB_Info f(SpB_Vector v, SpB_Vector w) {\n"
+        "  double d = 0; float s = 0; SpB_Index I[4], n = 4; double X[4]; float Y[4];\n"
+        "  SpB_Info r = SpB_Vector_setElement(v, 1.0, 0);\n"
+        "  r = (SpB_Info)(r | SpB_Vector_setElement(w, 1.0f, 0));\n"
+        "  r = (SpB_Info)(r | SpB_Vector_extractElement(&d, v, 0));\n"
+        "  r = (SpB_Info)(r | SpB_Vector_extractElement(&s, w, 0));\n"
+        "  r = (SpB_Info)(r | SpB_Vector_extractTuples(I, X, &n, v));\n"
+        "  return (SpB_Info)(r | SpB_Vector_extractTuples(I, Y, &n, w));\n"
+        "}\n")
+    inc = Path(__file__).resolve().parents[1] / "include"
+    subprocess.run([gxx, "-fsyntax-only", "-Wall", "-Werror", f"-I{inc}", str(src)], check=True)

@@ -93,6 +93,8 @@ SIGNATURES = {
     "SpB_Vector_nvals": (Info, [C.POINTER(u64), vp]),
     "SpB_Vector_setElement_FP64": (Info, [vp, C.c_double, u64]),
     "SpB_Vector_extractElement_FP64": (Info, [C.POINTER(C.c_double), vp, u64]),
+    "SpB_Vector_extractTuples_FP32": (Info, [vp, vp, C.POINTER(u64), vp]),
+    "SpB_Vector_extractTuples_FP64": (Info, [vp, vp, C.POINTER(u64), vp]),
     "SpB_Vector_setElement_FP32": (Info, [vp, C.c_float, u64]),
     "SpB_Vector_extractElement_FP32": (Info, [C.POINTER(C.c_float), vp, u64]),
     "SpB_Matrix_nrows": (Info, [C.POINTER(u64), vp]),

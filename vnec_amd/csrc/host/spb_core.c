@@ -642,6 +642,29 @@ SpB_Info SpB_Vector_extractElement_FP32(float *x, const SpB_Vector v, SpB_Index 
     return SpB_SUCCESS;
 }
 
+/* All n entries of a dense vector as (index, value) tuples
+ * (SpB_Vector_Dense.c:162-184): *p_nvals must equal n, else SpB_FAILURE;
+ * I[i] = i and X[i] = v[i].  Either array may be NULL to skip it.  The
+ * vector's type must match the call's. */
+#define SPB_EXTRACT_TUPLES(T, ctype, spbtype)                                                     \
+    SpB_Info SpB_Vector_extractTuples_##T(SpB_Index *I, ctype *X, SpB_Index *p_nvals,             \
+                                          const SpB_Vector v)                                     \
+    {                                                                                             \
+        spb_dense *d = typed_vector(v, spbtype, "SpB_Vector_extractTuples_" #T);                   \
+        if (!d || !p_nvals) return SpB_FAILURE;                                                   \
+        if (*p_nvals != d->base.n)                                                                \
+            return vnec_fail("SpB_Vector_extractTuples_" #T ": *p_nvals differs from the size");  \
+        if (X && d->host_stale && spb_vector_from_device(d) != SpB_SUCCESS) return SpB_FAILURE;   \
+        for (SpB_Index i = 0; i < d->base.n; i++) {                                               \
+            if (I) I[i] = i;                                                                      \
+            if (X) X[i] = ((const ctype *)d->values)[i];                                          \
+        }                                                                                         \
+        return SpB_SUCCESS;                                                                       \
+    }
+SPB_EXTRACT_TUPLES(FP64, double, SpB_FP64)
+SPB_EXTRACT_TUPLES(FP32, float, SpB_FP32)
+#undef SPB_EXTRACT_TUPLES
+
 SpB_Info SpB_Vector_import_ext(SpB_Vector v, const double *values)
 {
     spb_dense *d = typed_vector(v, SpB_FP64, "SpB_Vector_import_ext");
