@@ -10,8 +10,9 @@ A "step" is one SpMV over the whole workload with A, x and y resident in
 HBM.  `value` is algorithmic GB/s over the timed steps, with
   B = (vb + 4) nnz + 4 (nrows + 1) + 8 k ncols + 8 k nrows   (vb = 8 for FP64)
 bytes per SpMV, whatever the on-device format moves.  The roofline object
-uses the same B over the SpMV kernel's mean duration (HIP events on the
-library's stream).  The CPU baseline is the reference VNEC code (compiled
+uses the same B over the SpMV kernel's mean duration (back-to-back launches
+between one pair of HIP events on the library's stream), and reports the
+physical rate (counter bytes over the same duration) beside it.  The CPU baseline is the reference VNEC code (compiled
 from its sources into oracle/_ref) with its own dispatch, timed like the
 reference harness (1 warm-up + 40 SpMVs), on this host's cores.
 
