@@ -103,9 +103,11 @@ typedef struct {
     float *carry32;           /* ntiles scratch                    */
     /* variant C: val / col hold each tile's nonzeros in column order and
        spos their position within the tile's CSR range; with cpos_shift > 0
-       the position is packed into col instead: col = column | pos << shift */
+       the position is packed into col instead: col = column | pos << shift,
+       the column taken relative to cbase[t] when cbase is given */
     const uint16_t *spos;     /* nnz                               */
     int cpos_shift;
+    const int32_t *cbase;     /* ntiles, or NULL                   */
 } vnec_merge_args;
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
@@ -163,10 +165,17 @@ int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, dou
  * columns and in-tile positions in that order.  With pos_shift > 0 the
  * positions go into scol's high bits (scol = column | pos << pos_shift,
  * columns below 2^pos_shift, positions below 2^(32 - pos_shift)) and spos
- * is not written.  keys / idx: nnz scratch. */
+ * is not written; with cbase (ntiles, from vnec_hip_tile_colspan) the packed
+ * column is column - cbase[t], which must stay below 2^pos_shift.  keys /
+ * idx: nnz scratch. */
 int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32_t *col, const double *val,
-                   int64_t nnz, int64_t ncols, int pos_shift, uint64_t *keys, int32_t *idx,
-                   double *sval, int32_t *scol, uint16_t *spos, void *stream);
+                   int64_t nnz, int64_t ncols, int pos_shift, const int32_t *cbase, uint64_t *keys,
+                   int32_t *idx, double *sval, int32_t *scol, uint16_t *spos, void *stream);
+/* cmin[t] = the smallest column of tile t (0 for a tile with no nonzeros);
+ * *span_host = the largest cmax - cmin + 1 over the tiles (after a stream
+ * synchronisation). */
+int vnec_hip_tile_colspan(const int32_t *coords, int64_t ntiles, const int32_t *col, int32_t *cmin,
+                          int32_t *span_host, void *stream);
 /* Whether a (variant, ipt, encoding, nrhs) combination is compiled in. */
 int vnec_hip_merge_supported(int variant, int ipt, int encoding, int nrhs);
 

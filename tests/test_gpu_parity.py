@@ -584,3 +584,44 @@ def test_time_kernels_leaves_y_right(variant, ipt, grid):
     ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     ok, worst = O.check_rows(y, yl, ax, TOL)
     assert ok, worst
+
+
+_WIDE_BATCH = []
+
+
+def _wide_batch():
+    """64 members of 16k-20k rows: more than 2^20 columns in all, so a
+    column and a 12-bit tile offset no longer share 32 bits, but every tile
+    spans at most two members' columns."""
+    if not _WIDE_BATCH:
+        _WIDE_BATCH.append(g.batch(64, 16_000, 20_000, 4.0, 16.0, seed=21))
+    return _WIDE_BATCH[0]
+
+
+@pytest.mark.parametrize("ipt,grid", [(9, 512), (11, 512), (11, -1), (9, -2)])
+def test_c_tile_relative_columns(ipt, grid):
+    """Variant C packs each offset above a column taken relative to its
+    tile's smallest column when absolute columns are too wide: the plan
+    holds values, packed words and one base per tile (no offset array), and
+    y passes the row gate."""
+    m = _wide_batch()
+    assert m.ncols > 1 << 20
+    A = spb.Matrix.from_batch(m.row_off, m.col_off, m.ptr, m.col, m.val)
+    pl = spb.Plan(A, "L", ipt, grid=grid)
+    pc = spb.Plan(A, "C", ipt, grid=grid)
+    il, ic = pl.info(), pc.info()
+    pl.free()
+    assert ic.ntiles == il.ntiles and ic.grid == il.grid
+    assert ic.dev_bytes - il.dev_bytes == 12 * m.nnz + 4 * ic.ntiles
+    print(f"wide batch nnz {m.nnz} ncols {m.ncols} C{ipt} grid {ic.grid}", flush=True)
+    dx = spb.DeviceArray.from_array(m.x)
+    dy = spb.DeviceArray(8 * m.nrows)
+    pc.spmv_dev(dx.ptr, dy.ptr)
+    y = dy.to_array(np.float64, m.nrows)
+    dx.free()
+    dy.free()
+    pc.free()
+    col = m.col + np.repeat(m.col_off[:-1], np.diff(m.ptr[m.row_off])).astype(np.int32)
+    ax, yl = O.spmv_abs(m.ptr, col, m.val, m.x)
+    ok, worst = O.check_rows(y, yl, ax, TOL)
+    assert ok, worst

@@ -190,7 +190,8 @@ SIGNATURES = {
     "vnec_hip_d_fill": (C.c_int, [vp, vp, vp, i64, vp, i64, C.c_int, vp, vp, vp, C.c_int, vp]),
     "vnec_hip_split_hilo": (C.c_int, [vp, i64, vp, vp, vp]),
     "vnec_hip_rhs_layout": (C.c_int, [vp, i64, C.c_int, C.c_int, vp, vp]),
-    "vnec_hip_csort": (C.c_int, [vp, i64, vp, vp, i64, i64, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "vnec_hip_csort": (C.c_int, [vp, i64, vp, vp, i64, i64, C.c_int, vp, vp, vp, vp, vp, vp, vp]),
+    "vnec_hip_tile_colspan": (C.c_int, [vp, i64, vp, vp, vp, vp]),
     "vnec_hip_spmv_merge": (C.c_int, [vp, vp]),
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "vnec_hip_carry_fixup_f32": (C.c_int, [vp, vp, i64, vp, i64, vp]),

@@ -5,6 +5,8 @@
 // gather-ceiling probe).
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "vnec_common.h"
 #include "vnec_hip.h"
 
@@ -403,8 +405,9 @@ __global__ void k_csort_keys(const int2 *__restrict__ co, const int32_t *__restr
 // After the sort: position j holds tile t's nonzero at in-tile offset idx[j].
 __global__ void k_csort_apply(const int2 *__restrict__ co, const uint64_t *__restrict__ keys,
                               const int32_t *__restrict__ idx, const double *__restrict__ val,
-                              int64_t nnz, int cb, int pos_shift, double *__restrict__ sval,
-                              int32_t *__restrict__ scol, uint16_t *__restrict__ spos)
+                              int64_t nnz, int cb, int pos_shift, const int32_t *__restrict__ cbase,
+                              double *__restrict__ sval, int32_t *__restrict__ scol,
+                              uint16_t *__restrict__ spos)
 {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
          j += (int64_t)gridDim.x * blockDim.x) {
@@ -414,7 +417,8 @@ __global__ void k_csort_apply(const int2 *__restrict__ co, const uint64_t *__res
         sval[j] = val[co[t].y + off];
         const uint32_t c = (uint32_t)(k & (((uint64_t)1 << cb) - 1));
         if (pos_shift > 0) {
-            scol[j] = (int32_t)(c | ((uint32_t)off << pos_shift));
+            const uint32_t cr = cbase ? c - (uint32_t)cbase[t] : c;
+            scol[j] = (int32_t)(cr | ((uint32_t)off << pos_shift));
         } else {
             scol[j] = (int32_t)c;
             spos[j] = (uint16_t)off;
@@ -423,15 +427,16 @@ __global__ void k_csort_apply(const int2 *__restrict__ co, const uint64_t *__res
 }
 
 extern "C" int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32_t *col, const double *val,
-                              int64_t nnz, int64_t ncols, int pos_shift, uint64_t *keys, int32_t *idx,
-                              double *sval, int32_t *scol, uint16_t *spos, void *stream)
+                              int64_t nnz, int64_t ncols, int pos_shift, const int32_t *cbase,
+                              uint64_t *keys, int32_t *idx, double *sval, int32_t *scol, uint16_t *spos,
+                              void *stream)
 {
     if (nnz <= 0 || ntiles <= 0) return 0;
     int cb = 1, tb = 1;  // bits for a column, bits for a tile index
     while (cb < 31 && ((int64_t)1 << cb) < ncols) cb++;
     while (tb < 32 && ((int64_t)1 << tb) < ntiles) tb++;
     if (cb + tb > 64 || ntiles >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
-    if (pos_shift < 0 || pos_shift > 31 || (pos_shift > 0 && ncols > ((int64_t)1 << pos_shift)))
+    if (pos_shift < 0 || pos_shift > 31 || (pos_shift > 0 && !cbase && ncols > ((int64_t)1 << pos_shift)))
         return (int)hipErrorInvalidValue;
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_csort_keys, dim3((unsigned)ntiles), dim3(256), 0, s, (const int2 *)coords, col, cb,
@@ -442,8 +447,65 @@ extern "C" int vnec_hip_csort(const int32_t *coords, int64_t ntiles, const int32
     if (rc) return rc;
     unsigned g = (unsigned)((nnz + 255) / 256 < 65536 ? (nnz + 255) / 256 : 65536);
     hipLaunchKernelGGL(k_csort_apply, dim3(g), dim3(256), 0, s, (const int2 *)coords, keys, idx, val, nnz, cb,
-                       pos_shift, sval, scol, spos);
+                       pos_shift, cbase, sval, scol, spos);
     return (int)hipGetLastError();
+}
+
+// One workgroup per tile: its smallest column, and the widest column span
+// of any tile (atomicMax into *span).
+__global__ __launch_bounds__(VNEC_BLOCK) void k_tile_colspan(const int2 *__restrict__ co,
+                                                             const int32_t *__restrict__ col,
+                                                             int32_t *__restrict__ cmin,
+                                                             int32_t *__restrict__ span)
+{
+    __shared__ int s_mn[VNEC_NWAVES], s_mx[VNEC_NWAVES];
+    const int64_t t = blockIdx.x;
+    const int y0 = co[t].y, y1 = co[t + 1].y;
+    int mn = INT_MAX, mx = -1;
+    for (int j = y0 + (int)threadIdx.x; j < y1; j += VNEC_BLOCK) {
+        const int c = col[j];
+        mn = c < mn ? c : mn;
+        mx = c > mx ? c : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_down(mn, o, VNEC_WAVE), b = __shfl_down(mx, o, VNEC_WAVE);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+    }
+    const int w = threadIdx.x / VNEC_WAVE;
+    if (lane_id() == 0) {
+        s_mn[w] = mn;
+        s_mx[w] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < VNEC_NWAVES; i++) {
+            mn = s_mn[i] < mn ? s_mn[i] : mn;
+            mx = s_mx[i] > mx ? s_mx[i] : mx;
+        }
+        const bool any = y1 > y0;
+        cmin[t] = any ? mn : 0;
+        if (any) atomicMax(span, mx - mn + 1);
+    }
+}
+
+extern "C" int vnec_hip_tile_colspan(const int32_t *coords, int64_t ntiles, const int32_t *col, int32_t *cmin,
+                                     int32_t *span_host, void *stream)
+{
+    if (ntiles <= 0 || ntiles >= ((int64_t)1 << 31) || !span_host) return (int)hipErrorInvalidValue;
+    hipStream_t s = (hipStream_t)stream;
+    int32_t *d = nullptr;
+    HIPCHK(hipMalloc(&d, 4));
+    int rc = (int)hipMemsetAsync(d, 0, 4, s);
+    if (!rc) {
+        hipLaunchKernelGGL(k_tile_colspan, dim3((unsigned)ntiles), dim3(VNEC_BLOCK), 0, s, (const int2 *)coords,
+                           col, cmin, d);
+        rc = (int)hipGetLastError();
+    }
+    if (!rc) rc = (int)hipMemcpyAsync(span_host, d, 4, hipMemcpyDeviceToHost, s);
+    if (!rc) rc = (int)hipStreamSynchronize(s);
+    hipError_t f = hipFree(d);
+    return rc ? rc : (int)f;
 }
 
 // ------------------------------------------- right-hand-side layouts

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
     F *__restrict__ y = y_of(a, F());
     const int2 *co = (const int2 *)a.coords;
     const int2 c0 = co[t], c1 = co[t + 1];
+    const int cbt = (SORT && a.cbase) ? a.cbase[t] : 0;  // column base of packed relative columns
     const int x0 = c0.x, y0 = c0.y;
     const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
     const int tid = threadIdx.x;
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_merge1(vnec_merge_args a, i
                 int c = cv[u], pos = pv[u];
                 if (SORT && a.cpos_shift > 0) {  // position packed above the column
                     pos = (int)((uint32_t)c >> a.cpos_shift);
-                    c &= (1 << a.cpos_shift) - 1;
+                    c = (c & ((1 << a.cpos_shift) - 1)) + cbt;
                 }
                 s_prod[pos] = av[u] * (WIN ? s_win[c] : x[c]);
             }

@@ -49,14 +49,16 @@ struct TileRegs {
     int c[IPT];
     int re[IPT];
     int p[IPT];  // variant C: position of the nonzero in the tile's CSR range
+    int cb;      // variant C: the tile's column base (packed relative columns)
 };
 
 // tid: the thread's index within its 256-thread tile pipeline.
 template <int IPT, bool WIN, int ENC, bool SORT = false>
-__device__ __forceinline__ void load_tile(const vnec_merge_args &a, int2 c0, int2 c1,
+__device__ __forceinline__ void load_tile(const vnec_merge_args &a, int64_t t, int2 c0, int2 c1,
                                           TileRegs<IPT, WIN, ENC> &R, int tid)
 {
     const int nrow = c1.x - c0.x, nz = c1.y - c0.y;
+    if (SORT) R.cb = a.cbase ? a.cbase[t] : 0;
 #pragma unroll
     for (int u = 0; u < IPT; u++) {
         const int k = u * VNEC_BLOCK + tid;
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         }
     }
     TileRegs<IPT, WIN, ENC> cur, nxt;
-    load_tile<IPT, WIN, ENC, SORT>(a, A0, A1, cur, tid);
+    load_tile<IPT, WIN, ENC, SORT>(a, tb, A0, A1, cur, tid);
     if (HOT > 0)
         for (int i = tid; i < a.nhot; i += VNEC_BLOCK) s_hot[i] = a.xhot[i];
     __syncthreads();
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                 nec_c = a.nec[t + 2];
             }
         }
-        if (t + 1 < te) load_tile<IPT, WIN, ENC, SORT>(a, A1, B1, nxt, tid);
+        if (t + 1 < te) load_tile<IPT, WIN, ENC, SORT>(a, t + 1, A1, B1, nxt, tid);
 
         const int x0 = A0.x, y0 = A0.y;
         const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                     pos = cur.p[u];
                     if (a.cpos_shift > 0) {  // position packed above the column
                         pos = (int)((uint32_t)c >> a.cpos_shift);
-                        c &= (1 << a.cpos_shift) - 1;
+                        c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
                     }
                 }
                 s_prod[pos] = widen(cur.v[u]) * (WIN ? s_win[c] : gather_x<HOT>(a, s_hot, c));
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
 
     int2 A0 = co[t], A1 = co[t + 1];
     TileRegs<IPT, false, ENC> cur, nxt;
-    load_tile<IPT, false, ENC, SORT>(a, A0, A1, cur, tid);
+    load_tile<IPT, false, ENC, SORT>(a, t, A0, A1, cur, tid);
 
     for (; t < xhi; t += per) {
         // --- issue the next tile's loads
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
         if (tn < xhi) {
             B0 = co[tn];
             B1 = co[tn + 1];
-            load_tile<IPT, false, ENC, SORT>(a, B0, B1, nxt, tid);
+            load_tile<IPT, false, ENC, SORT>(a, tn, B0, B1, nxt, tid);
         }
 
         const int x0 = A0.x, y0 = A0.y;
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
                     pos = cur.p[u];
                     if (a.cpos_shift > 0) {  // position packed above the column
                         pos = (int)((uint32_t)c >> a.cpos_shift);
-                        c &= (1 << a.cpos_shift) - 1;
+                        c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
                     }
                 }
                 s_prod[pos] = widen(cur.v[u]) * a.x[c];
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(VNEC_BLOCK * HOT_SUB) void k_spmv_hot4(vnec_merge_a
         A1 = co[tb + 1];
         B1 = A1;
         if (tb + 1 < te) B1 = co[tb + 2];
-        load_tile<IPT, false, 0>(a, A0, A1, cur, tid);
+        load_tile<IPT, false, 0>(a, tb, A0, A1, cur, tid);
     }
     __syncthreads();
 
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(VNEC_BLOCK * HOT_SUB) void k_spmv_hot4(vnec_merge_a
         const bool act = t < te;
         int2 B2 = B1;
         if (act && t + 2 < te) B2 = co[t + 3];
-        if (act && t + 1 < te) load_tile<IPT, false, 0>(a, A1, B1, nxt, tid);
+        if (act && t + 1 < te) load_tile<IPT, false, 0>(a, t + 1, A1, B1, nxt, tid);
 
         const int x0 = A0.x, y0 = A0.y;
         const int nrow = act ? A1.x - A0.x : 0, nz = act ? A1.y - A0.y : 0;

@@ -129,6 +129,8 @@ struct vnec_plan_s {
     uint16_t *cs_pos;   /* nnz: offset within the tile's CSR range,
                            NULL when packed into cs_col's high bits  */
     int cs_shift;       /* > 0: cs_col = column | offset << cs_shift  */
+    int32_t *cs_base;   /* ntiles, or NULL: with cs_shift > 0, the packed
+                           column is relative to its tile's smallest one  */
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;

@@ -35,6 +35,7 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->cs_val);
     vnec_free_dev(p->cs_col);
     vnec_free_dev(p->cs_pos);
+    vnec_free_dev(p->cs_base);
     vnec_free_dev(p->g_irr);
     vnec_free_dev(p->g_ysub);
     vnec_free_dev(p->coords);
@@ -609,6 +610,26 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
     while (((int64_t)1 << cb) < p->ncols) cb++;
     while ((1 << ob) < TILE_THREADS * p->tune.ipt) ob++;
     p->cs_shift = cb + ob <= 32 ? cb : 0;
+    if (!p->cs_shift && p->nnz > 0) {
+        /* too many columns to pack: columns relative to each tile's smallest
+         * one still fit when every tile spans few enough of them (a batch of
+         * block-diagonal members: a tile lies within one or two members) */
+        p->cs_base = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)p->ntiles, &p->dev_bytes);
+        int32_t span = 0;
+        if (!p->cs_base ||
+            vnec_hip_ok(vnec_hip_tile_colspan(p->coords, p->ntiles, a->d_col, p->cs_base, &span, vnec_stream()),
+                        "tile column span"))
+            return SpB_FAILURE;
+        int rb = 1;
+        while (((int64_t)1 << rb) < (int64_t)span) rb++;
+        if (rb + ob <= 32) {
+            p->cs_shift = rb;
+        } else {
+            vnec_free_dev(p->cs_base);
+            p->cs_base = NULL;
+            p->dev_bytes -= (int64_t)sizeof(int32_t) * p->ntiles;
+        }
+    }
     p->cs_val = (double *)vnec_malloc_dev(sizeof(double) * n, &p->dev_bytes);
     p->cs_col = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * n, &p->dev_bytes);
     if (!p->cs_shift) p->cs_pos = (uint16_t *)vnec_malloc_dev(sizeof(uint16_t) * n, &p->dev_bytes);
@@ -618,7 +639,7 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
                                                                                           : SpB_FAILURE;
     if (rc == SpB_SUCCESS)
         rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, a->d_col, a->d_val, p->nnz, p->ncols, p->cs_shift,
-                                        keys, idx, p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
+                                        p->cs_base, keys, idx, p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
                          "column sort");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "sync");
     vnec_free_dev(keys);
@@ -662,6 +683,7 @@ static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_de
         m.col = p->cs_col;
         m.spos = p->cs_pos;
         m.cpos_shift = p->cs_shift;
+        m.cbase = p->cs_base;
     }
     if (p->tune.variant == VNEC_VAR_H) {
         m.col = p->hcol;
