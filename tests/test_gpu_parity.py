@@ -625,3 +625,38 @@ def test_c_tile_relative_columns(ipt, grid):
     ax, yl = O.spmv_abs(m.ptr, col, m.val, m.x)
     ok, worst = O.check_rows(y, yl, ax, TOL)
     assert ok, worst
+
+
+@pytest.mark.parametrize("name", ["wide_batch", "rmat12", "powerlaw"])
+def test_tile_colspan_matches_numpy(name):
+    """vnec_hip_tile_colspan: each tile's smallest column and the widest
+    span, exactly as numpy finds them (empty tiles report base 0)."""
+    import ctypes as C
+    from vnec_amd import _lib as L
+    if name == "wide_batch":
+        m = _wide_batch()
+        col = m.col + np.repeat(m.col_off[:-1], np.diff(m.ptr[m.row_off])).astype(np.int32)
+    else:
+        m = dict(_cases())[name]
+        col = m.col
+    A = _mat(m) if name != "wide_batch" else spb.Matrix.from_batch(m.row_off, m.col_off, m.ptr, m.col, m.val)
+    P = -(-(m.nrows + m.nnz) // (256 * 11))
+    co = spb.partition(A, P)
+    dco = spb.DeviceArray.from_array(np.ascontiguousarray(co, dtype=np.int32))
+    dcol = spb.DeviceArray.from_array(np.ascontiguousarray(col, dtype=np.int32))
+    dmin = spb.DeviceArray(4 * P)
+    span = C.c_int32(0)
+    rc = L.load().vnec_hip_tile_colspan(dco.ptr, P, dcol.ptr, dmin.ptr, C.byref(span), None)
+    assert rc == 0
+    got = dmin.to_array(np.int32, P)
+    for d in (dco, dcol, dmin):
+        d.free()
+    y = co[:, 1].astype(np.int64)
+    ref_min = np.zeros(P, np.int64)
+    ref_span = 0
+    for t in range(P):
+        seg = col[y[t]:y[t + 1]]
+        if len(seg):
+            ref_min[t] = seg.min()
+            ref_span = max(ref_span, int(seg.max()) - int(seg.min()) + 1)
+    assert np.array_equal(got, ref_min) and span.value == ref_span
