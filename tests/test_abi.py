@@ -230,3 +230,18 @@ def test_cpp_overloads_compile(tmp_path):
         "}\n")
     inc = Path(__file__).resolve().parents[1] / "include"
     subprocess.run([gxx, "-fsyntax-only", "-Wall", "-Werror", f"-I{inc}", str(src)], check=True)
+
+
+def test_column_sort_argument_checks():
+    """The variant C builders reject shapes they cannot store before any
+    device work: packed absolute columns wider than the shift, a packed
+    shift beyond 31, and span queries over no tiles."""
+    lib = L.load()
+    span = C.c_int32(0)
+    # ncols 2^21 does not fit a 20-bit packed column without a per-tile base
+    assert lib.vnec_hip_csort(None, 4, None, None, 100, 1 << 21, 20, None, None, None, None, None,
+                              None, None) != 0
+    assert lib.vnec_hip_csort(None, 4, None, None, 100, 1000, 32, None, None, None, None, None,
+                              None, None) != 0
+    assert lib.vnec_hip_tile_colspan(None, 0, None, None, C.byref(span), None) != 0
+    assert lib.vnec_hip_tile_colspan(None, 4, None, None, None, None) != 0
