@@ -145,7 +145,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a
         const F vv = slot_value<ENC>(a, v + (int64_t)l * 64);
         if ((m >> l) & 1u) acc += vv * (WIN ? s_xw[a.wpos[l] + t] : x[row + a.pat[l]]);
     }
-    if (live) pattern_y(a, F())[row] = acc;
+    if (live) __builtin_nontemporal_store(acc, pattern_y(a, F()) + row);  // y is not read back here
 }
 
 template <typename F>
