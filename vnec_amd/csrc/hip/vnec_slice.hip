@@ -202,7 +202,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_slice(vnec_slice_args a, in
                 acc += v * (WIN ? s_win[c] : x[c]);
             }
         }
-        if (row < a.nrows) sy_of(a, F())[row] = acc;
+        if (row < a.nrows) __builtin_nontemporal_store(acc, sy_of(a, F()) + row);  // not read back here
     }
 }
 
