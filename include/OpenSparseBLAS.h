@@ -50,10 +50,71 @@ typedef enum {
     SpB_Type_N
 } SpB_Type;
 
-/* include/OpenSparseBLAS.h:78-139 (only the first enumerator is used on
- * this path: SpB_BINARYOP_NULL keeps duplicates). */
+/* include/OpenSparseBLAS.h:78-139: every enumerator, with the reference's
+ * values (0, 1, 2, ... in this order), so code naming any of them compiles
+ * against this header.  SpB_Matrix_build_* accepts only SpB_BINARYOP_NULL
+ * (duplicates kept) and returns SpB_FAILURE for any other operator, as the
+ * reference's build does (src/core/src/SpB_Matrix_CSC_or_CSR.c:197,252). */
 typedef enum {
-    SpB_BINARYOP_NULL = 0
+    SpB_BINARYOP_NULL,
+    SpB_BINARYOP_BOOL_ST,
+    SpB_LOR,
+    SpB_LAND,
+    SpB_LXOR,
+    SpB_LXNOR,
+    SpB_BINARYOP_BOOL_ED,
+    SpB_BINARYOP_INT8_ST,
+    SpB_PLUS_INT8,
+    SpB_MINUS_INT8,
+    SpB_DIV_INT8,
+    SpB_BINARYOP_INT8_ED,
+    SpB_BINARYOP_INT16_ST,
+    SpB_PLUS_INT16,
+    SpB_MINUS_INT16,
+    SpB_DIV_INT16,
+    SpB_BINARYOP_INT16_ED,
+    SpB_BINARYOP_INT32_ST,
+    SpB_PLUS_INT32,
+    SpB_MINUS_INT32,
+    SpB_DIV_INT32,
+    SpB_BINARYOP_INT32_ED,
+    SpB_BINARYOP_INT64_ST,
+    SpB_PLUS_INT64,
+    SpB_MINUS_INT64,
+    SpB_DIV_INT64,
+    SpB_BINARYOP_INT64_ED,
+    SpB_BINARYOP_UINT8_ST,
+    SpB_PLUS_UINT8,
+    SpB_MINUS_UINT8,
+    SpB_DIV_UINT8,
+    SpB_BINARYOP_UINT8_ED,
+    SpB_BINARYOP_UINT16_ST,
+    SpB_PLUS_UINT16,
+    SpB_MINUS_UINT16,
+    SpB_DIV_UINT16,
+    SpB_BINARYOP_UINT16_ED,
+    SpB_BINARYOP_UINT32_ST,
+    SpB_PLUS_UINT32,
+    SpB_MINUS_UINT32,
+    SpB_DIV_UINT32,
+    SpB_BINARYOP_UINT32_ED,
+    SpB_BINARYOP_UINT64_ST,
+    SpB_PLUS_UINT64,
+    SpB_MINUS_UINT64,
+    SpB_DIV_UINT64,
+    SpB_BINARYOP_UINT64_ED,
+    SpB_BINARYOP_FP32_ST,
+    SpB_PLUS_FP32,
+    SpB_TIMES_FP32,
+    SpB_DIV_FP32,
+    SpB_MAX_FP32,
+    SpB_MINUS_FP32,
+    SpB_BINARYOP_FP32_ED,
+    SpB_BINARYOP_FP64_ST,
+    SpB_PLUS_FP64,
+    SpB_MINUS_FP64,
+    SpB_DIV_FP64,
+    SpB_BINARYOP_FP64_ED
 } SpB_BinaryOp;
 
 /* include/OpenSparseBLAS.h:252 */

@@ -134,6 +134,13 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
 SpB_Info vnec_plan_free_ext(vnec_plan *plan);
 /* Tuning in force, tile count and device bytes held by the plan. */
 SpB_Info vnec_plan_info_ext(vnec_plan plan, vnec_tuning *t, int64_t *ntiles, int64_t *dev_bytes);
+/* Bytes the plan's own layout must move per SpMV: every array its kernels
+ * read counted once, x read once and y written once (padding slots
+ * included, caches ignored).  *main_bytes: the main SpMV kernel alone;
+ * *spmv_bytes: the whole SpMV (variant G adds its irregular-row sub-plan).
+ * Either may be NULL.  The format-agnostic figure, CSR with int32 columns,
+ * is (vb + 4) nnz + 4 (nrows + 1) + 8 k (ncols + nrows). */
+SpB_Info vnec_plan_bytes_ext(vnec_plan plan, int64_t *main_bytes, int64_t *spmv_bytes);
 
 /* y <- A x (x: ncols * nrhs, y: nrows * nrhs, row-major [i][rhs]).
  * Synchronous: y is valid when the call returns. */
@@ -166,6 +173,9 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, i
 /* Measured streaming-copy bandwidth: bytes read + bytes written per
  * second over iters copies of a `bytes`-sized buffer (GB/s). */
 SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps);
+/* Measured streaming-read bandwidth: bytes read per second over iters
+ * reads of a `bytes`-sized buffer (GB/s); an SpMV is read-dominated. */
+SpB_Info vnec_read_bandwidth_ext(size_t bytes, int iters, double *gbps);
 /* Measured ceilings of A's SpMV traffic without row work (a balanced
  * persistent grid over its nonzeros): mean ms to stream its FP64 values
  * and int32 columns, and to stream them and gather x_dev[col] for every

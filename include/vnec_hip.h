@@ -187,6 +187,10 @@ int vnec_hip_gather_probe(const double *val, const int32_t *col, int64_t nnz, co
 
 /* Streaming copy of bytes (multiple of 16) from src to dst. */
 int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
+/* Streaming read of bytes (a multiple of 16) from src; out (>= 4096 words)
+ * is written only for an input whose words XOR-fold to 0x9E3779B9 per
+ * thread (a probe: pass bytes of a memset pattern). */
+int vnec_hip_stream_read(const void *src, size_t bytes, void *out, void *stream);
 
 /* ---- variant S: 64-row slices, [l][lane] layout ---- */
 typedef struct {

@@ -245,3 +245,56 @@ def test_column_sort_argument_checks():
                               None, None) != 0
     assert lib.vnec_hip_tile_colspan(None, 0, None, None, C.byref(span), None) != 0
     assert lib.vnec_hip_tile_colspan(None, 4, None, None, None, None) != 0
+
+
+# include/OpenSparseBLAS.h:78-139 in the VNEC repository: the enumerators in
+# order, numbered from 0 (no explicit values there).
+_BINARYOPS = (["SpB_BINARYOP_NULL", "SpB_BINARYOP_BOOL_ST", "SpB_LOR", "SpB_LAND", "SpB_LXOR",
+               "SpB_LXNOR", "SpB_BINARYOP_BOOL_ED"]
+              + [n for t in ("INT8", "INT16", "INT32", "INT64", "UINT8", "UINT16", "UINT32",
+                             "UINT64")
+                 for n in (f"SpB_BINARYOP_{t}_ST", f"SpB_PLUS_{t}", f"SpB_MINUS_{t}",
+                           f"SpB_DIV_{t}", f"SpB_BINARYOP_{t}_ED")]
+              + ["SpB_BINARYOP_FP32_ST", "SpB_PLUS_FP32", "SpB_TIMES_FP32", "SpB_DIV_FP32",
+                 "SpB_MAX_FP32", "SpB_MINUS_FP32", "SpB_BINARYOP_FP32_ED",
+                 "SpB_BINARYOP_FP64_ST", "SpB_PLUS_FP64", "SpB_MINUS_FP64", "SpB_DIV_FP64",
+                 "SpB_BINARYOP_FP64_ED"])
+
+
+def _enum_values(header_dir, tmp_path, tag):
+    import subprocess
+    src = tmp_path / f"enum_{tag}.c"
+    src.write_text("#include <stdio.h>\n#include \"OpenSparseBLAS.h\"\nint main(void){\n"
+                   + "".join(f'printf("%d\\n", (int){n});\n' for n in _BINARYOPS)
+                   + "return 0;}\n")
+    exe = tmp_path / f"enum_{tag}"
+    subprocess.run(["gcc", "-std=gnu11", "-w", "-I", str(header_dir), "-o", str(exe), str(src)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    return [int(v) for v in out.split()]
+
+
+def test_binaryop_enum_matches_the_reference(tmp_path):
+    """Every SpB_BinaryOp enumerator compiles against the drop-in header with
+    the reference's value; where the reference's header is present, the two
+    headers give the same values."""
+    ours = _enum_values(os.path.join(ROOT, "include"), tmp_path, "ours")
+    assert ours == list(range(len(_BINARYOPS))) and len(ours) == 59
+    ref_inc = "/root/reference/include"
+    if os.path.exists(os.path.join(ref_inc, "OpenSparseBLAS.h")):
+        assert _enum_values(ref_inc, tmp_path, "ref") == ours
+
+
+def test_build_refuses_every_operator_but_null():
+    """SpB_Matrix_build_* keeps duplicates with SpB_BINARYOP_NULL and fails
+    for any other operator (SpB_Matrix_CSC_or_CSR.c:197,252)."""
+    lib = spb.init()
+    rr = np.array([0, 0], np.uint64)
+    cc = np.array([1, 1], np.uint64)
+    vv = np.array([1.0, 2.0])
+    for op in (1, 55, 58):  # SpB_BINARYOP_BOOL_ST, SpB_PLUS_FP64, SpB_BINARYOP_FP64_ED
+        A = spb.Matrix(2, 2)
+        assert lib.SpB_Matrix_build_FP64(A.h, rr.ctypes.data, cc.ctypes.data, vv.ctypes.data, 2,
+                                         op) == 1
+        assert "SpB_BINARYOP_NULL" in L.last_error()
+        A.free()

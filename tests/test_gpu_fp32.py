@@ -65,8 +65,6 @@ CASES = _cases()
                                                 ("D", 9, 4)])
 def test_fp32_plans_match_the_fp32_serial_oracle(name, m, variant, ipt, lanes):
     ptr, col, val, x = _f32(m)
-    if variant == "D" and name == "longrow" and lanes == 4:
-        pytest.skip("covered at the other widths")
     A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
     p = spb.Plan(A, variant, ipt, lanes=lanes, grid=-1)
     assert p.info().encoding == "fp32"
@@ -128,6 +126,9 @@ def test_fp32_reference_api_bit_exact_structures_and_gated_results(name):
             G = len(d["col_start"])
             if G:
                 assert np.array_equal(np.ctypeslib.as_array(p.col_start, (G,)), d["col_start"])
+                # val_align: the reference's G * 16 filled entries
+                assert np.array_equal(np.ctypeslib.as_array(p.val_align, (G * 16,)),
+                                      d["val_align"][:G * 16])
             assert np.array_equal(np.ctypeslib.as_array(p.NEC_NUM, (36,)), d["nec"])
         if kind == "S" and "s_tasks" in d:
             assert np.array_equal(np.ctypeslib.as_array(p.NEC_NUM, (36,)), d["s_nec"])

@@ -378,6 +378,9 @@ def test_reference_api_build_and_spmv(name):
             G = len(d["col_start"])
             if G:
                 assert np.array_equal(np.ctypeslib.as_array(p.col_start, (G,)), d["col_start"])
+                # val_align: the reference's G * 8 filled entries (it allocates nnz * 8)
+                assert np.array_equal(np.ctypeslib.as_array(p.val_align, (G * 8,)),
+                                      d["val_align"][:G * 8])
             nec = np.ctypeslib.as_array(p.NEC_NUM, (36,))
             assert np.array_equal(nec, d["nec"])
         if kind == "S" and "s_tasks" in d:

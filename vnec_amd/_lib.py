@@ -142,6 +142,8 @@ SIGNATURES = {
     "vnec_time_kernels_ext": (Info, [vp, vp, vp, C.c_int, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
     "vnec_copy_bandwidth_ext": (Info, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "vnec_read_bandwidth_ext": (Info, [C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
+    "vnec_plan_bytes_ext": (Info, [vp, C.POINTER(i64), C.POINTER(i64)]),
     "vnec_dev_alloc_ext": (vp, [C.c_size_t]),
     "vnec_dev_free_ext": (Info, [vp]),
     "vnec_memcpy_h2d_ext": (Info, [vp, vp, C.c_size_t]),
@@ -202,6 +204,7 @@ SIGNATURES = {
     "vnec_hip_strided_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, vp]),
     "vnec_hip_spmv_strided": (C.c_int, [vp, C.c_int, vp]),
     "vnec_hip_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
+    "vnec_hip_stream_read": (C.c_int, [vp, C.c_size_t, vp, vp]),
     "vnec_hip_slice_len": (C.c_int, [vp, i64, i64, vp, vp, vp]),
     "vnec_hip_slice_tile_rows": (C.c_int, [vp, i64, vp, i64, vp, vp]),
     "vnec_hip_slice_fill": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),

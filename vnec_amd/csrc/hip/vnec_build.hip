@@ -655,20 +655,62 @@ extern "C" int vnec_hip_scribble(void *p, size_t n, void *stream)
 // hardware byte counters (read n bytes, write n bytes per launch).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// Each thread keeps STREAM_U 16-byte loads in flight before it stores them.
+#define STREAM_U 4
+
 __global__ __launch_bounds__(256) void k_stream_copy(const u32x4 *__restrict__ src,
                                                      u32x4 *__restrict__ dst, int64_t n16)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
-         i += (int64_t)gridDim.x * blockDim.x)
-        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (STREAM_U - 1) * stride < n16; i += STREAM_U * stride) {
+        u32x4 v[STREAM_U];
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < STREAM_U; u++) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+    }
+    for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 extern "C" int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream)
 {
     int64_t n16 = (int64_t)(bytes / 16);
     if (n16 <= 0) return 0;
-    hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_stream_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream,
                        (const u32x4 *)src, (u32x4 *)dst, n16);
+    return (int)hipGetLastError();
+}
+
+// Streaming read of n16 16-byte words (2 * STREAM_U loads in flight per
+// thread), folded into one word per thread that is stored only if it
+// equals a value the probe's input never produces: the read-only HBM
+// ceiling (an SpMV reads far more than it writes).
+__global__ __launch_bounds__(256) void k_stream_read(const u32x4 *__restrict__ src, int64_t n16,
+                                                     uint32_t key, uint32_t *__restrict__ out)
+{
+    constexpr int U = 2 * STREAM_U;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    u32x4 acc = {0u, 0u, 0u, 0u};
+    for (; i + (U - 1) * stride < n16; i += U * stride) {
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < U; u++) acc ^= v[u];
+    }
+    for (; i < n16; i += stride) acc ^= __builtin_nontemporal_load(src + i);
+    const uint32_t r = acc.x ^ acc.y ^ acc.z ^ acc.w;
+    if (r == key) out[blockIdx.x] = r;
+}
+
+extern "C" int vnec_hip_stream_read(const void *src, size_t bytes, void *out, void *stream)
+{
+    int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 <= 0) return 0;
+    hipLaunchKernelGGL(k_stream_read, dim3(4096), dim3(256), 0, (hipStream_t)stream,
+                       (const u32x4 *)src, n16, 0x9E3779B9u, (uint32_t *)out);
     return (int)hipGetLastError();
 }
 

@@ -99,6 +99,8 @@ struct vnec_plan_s {
     uint16_t *slcol;
     int32_t max_nec;
     int win;
+    int64_t s_padded;   /* variant S: slots of the padded slice layout     */
+    int64_t nruns;      /* window runs over all tiles (variants E, D, S)   */
     /* variant D: groups of `lanes` window positions */
     int32_t *vrp;       /* nrows + 1 */
     int32_t *col_start; /* groups    */

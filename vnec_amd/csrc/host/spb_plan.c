@@ -125,6 +125,7 @@ static SpB_Info build_windows(struct vnec_plan_s *p, const spb_csr *a, int32_t *
         rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass2(p->coords, P, uniq, p->nec, p->run_off,
                                                   p->run_start, p->run_wpos, s), "ecr pass 2");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "windows sync");
+    p->nruns = total;
     vnec_free_dev(uniq);
     vnec_free_dev(nruns);
     return rc;
@@ -152,6 +153,7 @@ static SpB_Info build_slices(struct vnec_plan_s *p, const spb_csr *a)
         rc = vnec_hip_ok(vnec_hip_scan_excl_i32(p->slice_ptr, p->slice_ptr, ns + 1, &padded, s), "slice scan");
     if (rc != SpB_SUCCESS) return rc;
     if (padded + ns >= INT32_MAX) return vnec_fail("padded slice layout exceeds 2^31 entries");
+    p->s_padded = padded;
     /* tiles of whole slices, balanced over the padded layout */
     int64_t M = ns + padded;
     int64_t T = 2048 * (int64_t)p->tune.ipt;
@@ -210,7 +212,10 @@ static SpB_Info build_slices(struct vnec_plan_s *p, const spb_csr *a)
             if (rc == SpB_SUCCESS)
                 rc = vnec_hip_ok(vnec_hip_ecr_tiles_pass2(p->coords, P, uniq, p->nec, p->run_off,
                                                           p->run_start, p->run_wpos, s), "ecr pass 2");
-            if (rc == SpB_SUCCESS) p->win = 1;
+            if (rc == SpB_SUCCESS) {
+                p->win = 1;
+                p->nruns = total;
+            }
         }
         vnec_free_dev(uniq);
         vnec_free_dev(nruns);
@@ -587,6 +592,56 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         return SpB_FAILURE;
     }
     *plan = p;
+    return SpB_SUCCESS;
+}
+
+/* Bytes of one value slot in the plan's encoding. */
+static int64_t value_bytes(int enc) { return enc == VNEC_ENC_HI || enc == VNEC_ENC_FP32 ? 4 : 8; }
+
+/* Bytes the main kernel of plan p must move per launch: every array of its
+ * layout read once, x read once (every column), y written once.  Work
+ * indices that are not per nonzero or per row (tile coordinates, window
+ * counts, workgroup carries) are counted too; they are below 0.1 %. */
+static int64_t main_kernel_bytes(const struct vnec_plan_s *p)
+{
+    const int enc = p->tune.encoding, k = p->tune.nrhs > 0 ? p->tune.nrhs : 1;
+    const int64_t vb = value_bytes(enc), xb = enc == VNEC_ENC_FP32 ? 4 : 8;
+    const int64_t nr = p->nrows, nc = p->ncols, nnz = p->nnz;
+    const int64_t vecs = xb * k * (nc + nr), coords = 8 * (p->ntiles + 1);
+    const int64_t windows = 8 * p->nruns + 4 * (p->ntiles + 1) + 4 * p->ntiles; /* runs, run_off, nec */
+    switch (p->tune.variant) {
+    case VNEC_VAR_G:
+        return p->g_nslices * 64 * (int64_t)p->g_L * vb + 4 * nr + vecs;
+    case VNEC_VAR_S:
+        return p->s_padded * (vb + (p->win ? 2 : 4)) + 8 * (p->nslices + 1) + coords +
+               (p->win ? windows : 0) + vecs;
+    case VNEC_VAR_D:
+        return ((p->groups + 63) / 64) * 64 * (int64_t)p->tune.lanes * vb + 4 * p->groups +
+               4 * (nr + 1) + coords + windows + vecs;
+    case VNEC_VAR_E:
+        return nnz * (vb + 2) + 4 * (nr + 1) + coords + windows + vecs;
+    case VNEC_VAR_C:
+        return nnz * (8 + 4 + (p->cs_pos ? 2 : 0)) + 4 * (nr + 1) + coords +
+               (p->cs_base ? 4 * p->ntiles : 0) + vecs;
+    default: /* L, H: CSR with int32 columns */
+        return nnz * (vb + 4) + 4 * (nr + 1) + coords + vecs;
+    }
+}
+
+SpB_Info vnec_plan_bytes_ext(vnec_plan plan, int64_t *main_bytes, int64_t *spmv_bytes)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_plan_bytes_ext: not a plan");
+    int64_t mb = plan->ntiles > 0 ? main_kernel_bytes(plan) : 0, tot = mb;
+    if (plan->tune.variant == VNEC_VAR_G && plan->g_nirr && plan->g_subplan) {
+        /* the irregular rows: their plan (x already counted), its rows
+         * written to scratch, read back and scattered with their indices */
+        const int64_t xb = plan->tune.encoding == VNEC_ENC_FP32 ? 4 : 8, n = plan->g_nirr;
+        int64_t sub = 0;
+        vnec_plan_bytes_ext(plan->g_subplan, NULL, &sub);
+        tot += sub - xb * plan->g_subplan->ncols + (2 * xb + 4) * n;
+    }
+    if (main_bytes) *main_bytes = mb;
+    if (spmv_bytes) *spmv_bytes = tot;
     return SpB_SUCCESS;
 }
 
@@ -1028,32 +1083,47 @@ double vnec_slice_padding(const spb_csr *a)
     return a->nnz ? (double)slots / (double)a->nnz : 1.0;
 }
 
-SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps)
+/* Streaming probe over two buffers of `bytes`: a copy (read + write) or a
+ * read alone; *gbps counts the bytes moved per launch over the mean launch
+ * time of iters back-to-back launches. */
+static SpB_Info probe_bandwidth(size_t bytes, int iters, int read_only, double *gbps, const char *fn)
 {
-    if (!gbps || iters < 1 || bytes < 16) return vnec_fail("vnec_copy_bandwidth_ext: bad arguments");
+    if (!gbps || iters < 1 || bytes < 16) return vnec_fail("%s: bad arguments", fn);
     if (!vnec_runtime_ready()) return SpB_FAILURE;
     bytes = bytes / 16 * 16;
     void *a = vnec_malloc_dev(bytes, NULL), *b = vnec_malloc_dev(bytes, NULL);
     hipStream_t s = vnec_stream();
-    hipEvent_t e0, e1;
+    hipEvent_t e0 = NULL, e1 = NULL;
     SpB_Info rc = (a && b) ? SpB_SUCCESS : SpB_FAILURE;
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipMemsetAsync(a, 1, bytes, s), "memset");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e0), "event");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventCreate(&e1), "event");
-    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_stream_copy(a, b, bytes, s), "copy");
-    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
-    for (int i = 0; rc == SpB_SUCCESS && i < iters; i++)
-        rc = vnec_hip_ok(vnec_hip_stream_copy(a, b, bytes, s), "copy");
+    for (int i = -1; rc == SpB_SUCCESS && i < iters; i++) { /* i = -1: warm-up */
+        if (i == 0) rc = vnec_hip_ok(hipEventRecord(e0, s), "record");
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(read_only ? vnec_hip_stream_read(a, bytes, b, s) : vnec_hip_stream_copy(a, b, bytes, s),
+                             "probe");
+    }
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(e1, s), "record");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventSynchronize(e1), "sync");
     float ms = 0.f;
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-    if (rc == SpB_SUCCESS) *gbps = 2.0 * (double)bytes * iters / (ms / 1e3) / 1e9;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (rc == SpB_SUCCESS) *gbps = (read_only ? 1.0 : 2.0) * (double)bytes * iters / (ms / 1e3) / 1e9;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
     vnec_free_dev(a);
     vnec_free_dev(b);
     return rc;
+}
+
+SpB_Info vnec_copy_bandwidth_ext(size_t bytes, int iters, double *gbps)
+{
+    return probe_bandwidth(bytes, iters, 0, gbps, "vnec_copy_bandwidth_ext");
+}
+
+SpB_Info vnec_read_bandwidth_ext(size_t bytes, int iters, double *gbps)
+{
+    return probe_bandwidth(bytes, iters, 1, gbps, "vnec_read_bandwidth_ext");
 }
 
 #define GATHER_PROBE_GRID 2048

@@ -9,9 +9,9 @@
  * and keeps behind it a GPU execution plan laid out for wave64 tiles.
  * SpMV_VNEC_* runs that GPU plan; y is valid on return.  The FP32 entry
  * points compute as the reference's FP32 kernels do: FP32 values, x, y and
- * sums.  The D plans' val_align stays NULL: the device plan holds the
- * values, and the reference's val_align would take L values per nonzero
- * group of host memory.
+ * sums.  The D plans' val_align holds G * L values (G = the group count,
+ * 64-byte aligned, freed with free()), the reference's layout without its
+ * unused tail: the reference allocates nnz * L entries and fills G * L.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -493,6 +493,8 @@ struct VNEC_L *Build_VNEC_L(const SpB_Matrix A)
 typedef struct {
     coord *tcs, *tce;
     int *ecr_indices, *NEC_NUM, **use_x_indices, *v_row_ptr, *col_start;
+    double *val_align; /* G * L, FP64 (exact for FP32 matrices) */
+    int64_t G;
 } d_fields;
 
 static void d_fields_free(d_fields *f)
@@ -505,17 +507,30 @@ static void d_fields_free(d_fields *f)
     free(f->use_x_indices);
     free(f->v_row_ptr);
     free(f->col_start);
+    free(f->val_align);
     memset(f, 0, sizeof *f);
 }
 
+/* 64-byte aligned host array of n bytes (SpB_Alloc.c:53-66 aligns the
+ * reference's arrays the same way); released with free(). */
+static void *host_aligned(size_t n)
+{
+    void *p = NULL;
+    if (posix_memalign(&p, 64, n ? n : 64) != 0) return NULL;
+    return p;
+}
+
 /* spmv_vnec_D_for_mxv.c:242-398 (FP64) and :2-241 (FP32): partitions,
- * ECR and the group layout at P = 36 and L lanes, computed on the GPU. */
+ * ECR, the group layout and val_align (spmv_vnec_D_for_mxv.c:345-392:
+ * group g's values at [g * L, (g + 1) * L), zero-padded, a repeated
+ * column's last value kept) at P = 36 and L lanes, computed on the GPU. */
 static SpB_Info d_public(spb_csr *a, int L, const char *fn, d_fields *f)
 {
     memset(f, 0, sizeof *f);
     int64_t nr = (int64_t)a->base.row, nnz = (int64_t)a->nnz;
     dlay d;
-    if (d_layout_device(a, REF_P, L, 0, &d) != SpB_SUCCESS) return SpB_FAILURE;
+    if (d_layout_device(a, REF_P, L, 1, &d) != SpB_SUCCESS) return SpB_FAILURE;
+    f->G = d.G;
     int32_t *co = (int32_t *)malloc(sizeof(int32_t) * 2 * (REF_P + 1));
     int64_t *off = (int64_t *)malloc(sizeof(int64_t) * (REF_P + 1));
     int32_t *win = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz ? nnz : 1));
@@ -527,8 +542,9 @@ static SpB_Info d_public(spb_csr *a, int L, const char *fn, d_fields *f)
     f->use_x_indices = (int **)calloc(REF_P + 1, sizeof(int *));
     f->v_row_ptr = (int *)malloc(sizeof(int) * (size_t)(nr + 1));
     f->col_start = (int *)malloc(sizeof(int) * (size_t)(d.G ? d.G : 1));
+    f->val_align = (double *)host_aligned(sizeof(double) * (size_t)(d.G * L));
     SpB_Info rc = (co && off && win && f->tcs && f->tce && f->ecr_indices && f->NEC_NUM &&
-                   f->use_x_indices && f->v_row_ptr && f->col_start)
+                   f->use_x_indices && f->v_row_ptr && f->col_start && f->val_align)
                       ? SpB_SUCCESS
                       : vnec_fail("%s: out of memory", fn);
     if (rc == SpB_SUCCESS)
@@ -541,6 +557,9 @@ static SpB_Info d_public(spb_csr *a, int L, const char *fn, d_fields *f)
         rc = vnec_hip_ok(hipMemcpyAsync(f->v_row_ptr, d.vrp, sizeof(int32_t) * (size_t)(nr + 1), hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS && d.G)
         rc = vnec_hip_ok(hipMemcpyAsync(f->col_start, d.col_start, sizeof(int32_t) * (size_t)d.G, hipMemcpyDeviceToHost, s), "d2h");
+    if (rc == SpB_SUCCESS && d.G)
+        rc = vnec_hip_ok(hipMemcpyAsync(f->val_align, d.val_align, sizeof(double) * (size_t)(d.G * L),
+                                        hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS && nnz)
         rc = vnec_hip_ok(hipMemcpyAsync(win, d.win, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost, s), "d2h");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
@@ -595,6 +614,7 @@ struct VNEC_D_FP64 *Build_VNEC_D_FP64(const SpB_Matrix A)
     p->ecr_xx_val = xx;
     p->v_row_ptr = f.v_row_ptr;
     p->col_start = f.col_start;
+    p->val_align = f.val_align;
     struct vnec_plan_s *g = gpu_plan_for(a, KIND_D);
     if (!g) {
         VNEC_D_free_fp64_ext(p);
@@ -636,6 +656,17 @@ struct VNEC_D_FP32 *Build_VNEC_D_FP32(const SpB_Matrix A)
     p->ecr_xx_val = xx;
     p->v_row_ptr = f.v_row_ptr;
     p->col_start = f.col_start;
+    /* the device layout holds the FP32 values widened, so narrowing is exact */
+    size_t nva = (size_t)(f.G * REF_L32);
+    p->val_align = (float *)host_aligned(sizeof(float) * nva);
+    if (!p->val_align) {
+        vnec_fail("Build_VNEC_D_FP32: out of memory");
+        free(f.val_align);
+        VNEC_D_free_fp32_ext(p);
+        return NULL;
+    }
+    for (size_t i = 0; i < nva; i++) p->val_align[i] = (float)f.val_align[i];
+    free(f.val_align);
     struct vnec_plan_s *g = gpu_plan_for(a, KIND_D32);
     if (!g) {
         VNEC_D_free_fp32_ext(p);

@@ -255,6 +255,13 @@ class Plan:
         return TuningInfo(inv[t.variant], t.ipt, t.lanes, inve[t.encoding], t.nrhs, t.grid,
                           nt.value, db.value)
 
+    def format_bytes(self) -> tuple[int, int]:
+        """(main kernel, whole SpMV) bytes the plan's own layout must move
+        per SpMV (vnec_plan_bytes_ext)."""
+        a, b = C.c_int64(), C.c_int64()
+        L.check(L.load().vnec_plan_bytes_ext(self.h, C.byref(a), C.byref(b)), "vnec_plan_bytes_ext")
+        return a.value, b.value
+
     def spmv(self, y: Vector, x: Vector):
         L.check(L.load().vnec_spmv_ext(self.h, y.h, x.h), "vnec_spmv_ext")
 
