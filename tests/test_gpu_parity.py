@@ -6,6 +6,8 @@ decision) must be bit-exact; FP64 results must satisfy, per row,
 |y - y_ref| <= 1e-12 * (|A||x|)_i (FP32 hi+lo too); FP32 hi-only is
 bounded by 2^-23 * (|A||x|)_i.  Integer-valued known answers must be exact.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -663,3 +665,31 @@ def test_tile_colspan_matches_numpy(name):
             ref_min[t] = seg.min()
             ref_span = max(ref_span, int(seg.max()) - int(seg.min()) + 1)
     assert np.array_equal(got, ref_min) and span.value == ref_span
+
+
+def _ird_large_cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ird_large.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _ird_large_cases(), ids=lambda c: c["name"])
+def test_ird_above_2_24_nonzeros_bit_exact_with_reference(case):
+    """IRD on the GPU past 2^24 nonzeros (ird_calc.c:103-104 rounds nnz and
+    G * L to FP32 there): the GPU group count equals the reference's, the
+    ratio is the reference's formula over it bit for bit and prints as the
+    reference printed it, and IRD_VNEC decides as the reference did."""
+    m = getattr(g, case["gen"])(*case["args"], seed=case["seed"])
+    assert (m.nrows, m.nnz) == (case["nrows"], case["nnz"]) and m.nnz > 1 << 24
+    print(f"IRD {case['name']}: nrows {m.nrows} nnz {m.nnz}", flush=True)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    G = C.c_int64()
+    spb.L.check(spb.L.load().vnec_d_layout_ext(A.h, 36, 8, None, None, None, C.byref(G)), "d layout")
+    assert G.value == case["groups"]
+    dec, r = spb.ird_ext(A)
+    want = np.float32(case["nnz"]) / np.float32(case["groups"] * 8)
+    assert np.float32(r).tobytes() == want.tobytes()
+    assert "%f" % r == case["ird_printed"]
+    assert dec == case["decision"] == spb.ird_vnec(A)
+    A.free()

@@ -13,7 +13,10 @@ What pins each oracle function:
                                        to one group per distinct column run
   orc_s_build (a7/a8)                  golden tasks/row ranges/s_col/s_val,
                                        permutation and equal-length invariants
-  orc_ird (a10)                        golden decision, float32 recomputation
+  orc_ird (a10)                        golden decision, float32 recomputation;
+                                       above 2^24 nonzeros the reference's own
+                                       group count, printed ratio and decision
+                                       (tests/golden/ird_large.json)
   orc_spmv_serial (a13)                reference KAT y_i = i * sum_j j (exact)
   orc_spmv_L/D/S (a11/a5/a9)           golden y of the reference kernels, and
                                        the serial oracle, within 1e-12 (|A||x|)_i
@@ -242,3 +245,25 @@ def test_generator_stream_is_fixed():
     assert np.array_equal(g.stream(1, 4), s)
     st = g.stencil27(6, seed=1)
     assert st.nnz == (3 * 6 - 2) ** 3
+
+
+def _ird_large_cases():
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "ird_large.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("case", _ird_large_cases(), ids=lambda c: c["name"])
+def test_ird_above_2_24_nonzeros_matches_reference(case):
+    """ird_calc.c:103-104 divides (float)nnz by (float)(G * L): past 2^24
+    nonzeros the operands round.  The oracle's ratio must be the reference's
+    formula over the reference's own group count, bit for bit, print as the
+    reference printed it, and decide as the reference decided."""
+    m = getattr(g, case["gen"])(*case["args"], seed=case["seed"])
+    assert (m.nrows, m.nnz) == (case["nrows"], case["nnz"]) and m.nnz > 1 << 24
+    dec, r = O.ird(m.ptr, m.col, 36, 8, 0.477)
+    want = np.float32(case["nnz"]) / np.float32(case["groups"] * 8)
+    assert np.float32(r).tobytes() == want.tobytes()
+    assert "%f" % r == case["ird_printed"]
+    assert dec == case["decision"]
