@@ -337,6 +337,25 @@ SpB_Info VNEC_L_free_ext(VNEC_L *plan);
 /* Device plan behind a Build_VNEC_* result (owned by it). */
 vnec_plan VNEC_plan_of_ext(const void *reference_plan);
 
+/* ------------------------------------------- SpMV that allocates nothing
+ * No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): y_dev <- A x_dev for an FP64 CSR matrix
+ * given as caller device arrays (ptr_dev: nrows + 1 int32 row offsets,
+ * col_dev / val_dev: nnz int32 columns and FP64 values), enqueued on
+ * `stream` (NULL: the library stream) and not waited for.  It runs the
+ * kernels of vnec_spmv_dev_ext on an L plan at 7 items per thread with one
+ * workgroup per tile (same tiles, same sums, bit-identical y), keeping the
+ * tile coordinates and carries in a fixed device array of about 1 MiB per
+ * GPU instead of plan memory: at most VNEC_NOALLOC_MAX_TILES tiles of 1792
+ * merge items, i.e. nrows + nnz <= 117,440,512; larger inputs fail with a
+ * message before any device work.  Calls that overlap on one GPU race on
+ * that array. */
+#define VNEC_NOALLOC_MAX_TILES 65536
+SpB_Info vnec_csr_spmv_noalloc_ext(int64_t nrows, int64_t ncols, int64_t nnz, const int32_t *ptr_dev,
+                                   const int32_t *col_dev, const double *val_dev, const double *x_dev,
+                                   double *y_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -298,3 +298,21 @@ def test_build_refuses_every_operator_but_null():
                                          op) == 1
         assert "SpB_BINARYOP_NULL" in L.last_error()
         A.free()
+
+
+def test_noalloc_spmv_checks_sizes_before_any_device_work():
+    """vnec_csr_spmv_noalloc_ext refuses what its fixed scratch cannot hold
+    (and malformed sizes) with a message, before touching a device."""
+    lib = L.load()
+    # 65536 tiles of 1792 merge items: nrows + nnz <= 117,440,512
+    assert lib.vnec_csr_spmv_noalloc_ext(60_000_000, 60_000_000, 57_440_513, None, None, None, None,
+                                         None, None) == 1
+    assert "scratch holds 65536" in L.last_error()
+    assert lib.vnec_csr_spmv_noalloc_ext(-1, 4, 0, None, None, None, None, None, None) == 1
+    assert "negative" in L.last_error()
+    assert lib.vnec_csr_spmv_noalloc_ext(0, 4, 3, None, None, None, None, None, None) == 1
+    assert "no rows" in L.last_error()
+    assert lib.vnec_csr_spmv_noalloc_ext(10, 10, 20, None, None, None, None, None, None) == 1
+    assert "NULL array" in L.last_error()
+    # an empty matrix is a no-op
+    assert lib.vnec_csr_spmv_noalloc_ext(0, 0, 0, None, None, None, None, None, None) == 0

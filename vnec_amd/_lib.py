@@ -181,6 +181,7 @@ SIGNATURES = {
     "VNEC_S_free_fp32_ext": (Info, [vp]),
     "VNEC_L_free_ext": (Info, [vp]),
     "VNEC_plan_of_ext": (vp, [vp]),
+    "vnec_csr_spmv_noalloc_ext": (Info, [i64, i64, i64, vp, vp, vp, vp, vp, vp]),
     # vnec_hip.h (thin C-ABI to the kernels)
     "vnec_hip_partition": (C.c_int, [vp, i64, i64, i64, C.c_int, vp, vp]),
     "vnec_hip_scan_excl_i32": (C.c_int, [vp, vp, i64, C.POINTER(i64), vp]),
