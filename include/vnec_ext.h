@@ -150,6 +150,13 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x);
  * nrhs >= 4, x and y 16-byte aligned run the faster grouped multi-RHS
  * kernel; 8-byte aligned ones are accepted and run the one-lane kernel. */
 SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev);
+/* Same, replayed from a HIP graph: the plan captures its launch sequence
+ * for this (x_dev, y_dev) pair on the first call and replays it after
+ * (bit-identical to vnec_spmv_dev_ext; x_dev's contents may change between
+ * calls, its address may not).  A plan keeps graphs for 4 pairs; a fifth
+ * recaptures over the least recently used.  vnec_spmv_ext and SpB_mxv use
+ * it when x and y are device-resident (SpB_Vector_device_resident_ext). */
+SpB_Info vnec_spmv_graph_ext(vnec_plan plan, const void *x_dev, void *y_dev);
 /* Same with the right-hand sides column-major: X holds nrhs vectors of
  * ncols doubles one after another, Y nrhs vectors of nrows.  The plan
  * interleaves X into scratch of its own (allocated on the first call),

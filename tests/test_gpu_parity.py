@@ -693,3 +693,31 @@ def test_ird_above_2_24_nonzeros_bit_exact_with_reference(case):
     assert "%f" % r == case["ird_printed"]
     assert dec == case["decision"] == spb.ird_vnec(A)
     A.free()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_reference_d_entry_runs_slice_layouts_on_dense_column_runs(fp32):
+    """IRD sends matrices of dense column runs to D; the D entry runs them
+    with pattern slices (banded) or 64-row slices (block-diagonal), which
+    stream about twice as fast as D blocks there, and its public fields stay
+    the reference's (checked against the fixtures elsewhere)."""
+    dt = np.float32 if fp32 else np.float64
+    for m, want in ((g.banded(1 << 17, 5, seed=81), "G"), (g.blockdiag(1 << 16, 16, seed=82), "S")):
+        print(f"D entry: {m.name} fp32={fp32}", flush=True)
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val.astype(dt), dtype=dt)
+        assert spb.ird_vnec(A) == 0  # SpB_VNEC_D: the reference would run D blocks
+        plan = spb.build_vnec("D", A)
+        assert spb.reference_plan_info(plan).variant == want
+        x = spb.Vector.from_array(m.x.astype(dt), dt)
+        y = spb.Vector(m.nrows, dt)
+        assert spb.spmv_vnec("D", y, A, x, plan) == ""
+        if fp32:
+            v32, x32 = m.val.astype(np.float32), m.x.astype(np.float32)
+            ax, _ = O.spmv_abs(m.ptr, m.col, v32.astype(np.float64), x32.astype(np.float64))
+            assert O.check_rows_f32(y.to_array(), O.spmv_serial_f32(m.ptr, m.col, v32, x32),
+                                    m.ptr, ax)[0]
+        else:
+            ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+            assert O.check_rows(y.to_array(), O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
+        spb.free_vnec("D", plan, fp32=fp32)
+        A.free()

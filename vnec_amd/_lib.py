@@ -136,6 +136,7 @@ SIGNATURES = {
     "vnec_plan_info_ext": (Info, [vp, C.POINTER(Tuning), C.POINTER(i64), C.POINTER(i64)]),
     "vnec_spmv_ext": (Info, [vp, vp, vp]),
     "vnec_spmv_dev_ext": (Info, [vp, vp, vp]),
+    "vnec_spmv_graph_ext": (Info, [vp, vp, vp]),
     "vnec_spmm_colmajor_dev_ext": (Info, [vp, vp, vp]),
     "vnec_time_dev_ext": (Info, [vp, vp, vp, C.c_int, C.c_int, C.c_size_t,
                                  C.POINTER(C.c_double)]),

@@ -73,6 +73,9 @@ typedef struct spb_dense {
     int pinned;     /* values come from hipHostMalloc (page-locked)    */
 } spb_dense;
 
+/* Captured SpMV graphs a plan keeps (distinct x / y device pointer pairs). */
+#define VNEC_PLAN_GRAPHS 4
+
 struct vnec_plan_s {
     uint32_t magic;
     vnec_tuning tune;
@@ -140,6 +143,15 @@ struct vnec_plan_s {
     int64_t dev_bytes;
     /* cached graph for timing */
     hipGraphExec_t gexec;
+    /* captured single SpMVs for replay (vnec_spmv_graph_ext), one per
+       (x, y) device pointer pair, least recently used replaced */
+    struct {
+        hipGraphExec_t exec;
+        const void *x;
+        void *y;
+        uint64_t used;
+    } sgraph[VNEC_PLAN_GRAPHS];
+    uint64_t sgraph_clock;
     /* interleaved copies of column-major right-hand sides (multi-RHS) */
     double *xt, *yt;
     const void *g_x;

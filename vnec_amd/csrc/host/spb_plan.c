@@ -23,6 +23,8 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a);
 static void plan_release(struct vnec_plan_s *p)
 {
     if (p->gexec) hipGraphExecDestroy(p->gexec);
+    for (int i = 0; i < VNEC_PLAN_GRAPHS; i++)
+        if (p->sgraph[i].exec) hipGraphExecDestroy(p->sgraph[i].exec);
     if (p->g_subplan) vnec_plan_free_ext(&p->g_subplan);
     if (p->g_sub) SpB_Matrix_free(&p->g_sub);
     vnec_free_dev(p->g_pat);
@@ -902,6 +904,51 @@ SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev)
     return vnec_plan_run(plan, x_dev, y_dev);
 }
 
+/* One SpMV of p on (x_dev, y_dev) replayed from a HIP graph: captured on
+ * the library stream the first time the pair is seen (the plan keeps
+ * VNEC_PLAN_GRAPHS pairs, the least recently used one is recaptured), then
+ * launched.  The launch sequence records no memset node. */
+static SpB_Info plan_run_graph(struct vnec_plan_s *p, const void *x_dev, void *y_dev)
+{
+    hipStream_t s = vnec_stream();
+    int slot = -1, lru = 0;
+    for (int i = 0; i < VNEC_PLAN_GRAPHS; i++) {
+        if (p->sgraph[i].exec && p->sgraph[i].x == x_dev && p->sgraph[i].y == y_dev) slot = i;
+        if (p->sgraph[i].used < p->sgraph[lru].used) lru = i;
+    }
+    if (slot < 0) {
+        slot = lru;
+        if (p->sgraph[slot].exec) {
+            hipGraphExecDestroy(p->sgraph[slot].exec);
+            p->sgraph[slot].exec = NULL;
+        }
+        hipGraph_t g = NULL;
+        SpB_Info rc = vnec_hip_ok(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+        if (rc != SpB_SUCCESS) return rc;
+        rc = vnec_plan_run(p, x_dev, y_dev);
+        SpB_Info rc2 = vnec_hip_ok(hipStreamEndCapture(s, &g), "end capture");
+        if (rc == SpB_SUCCESS) rc = rc2;
+        if (rc == SpB_SUCCESS)
+            rc = vnec_hip_ok(hipGraphInstantiate(&p->sgraph[slot].exec, g, NULL, NULL, 0), "instantiate");
+        if (g) hipGraphDestroy(g);
+        if (rc != SpB_SUCCESS) {
+            p->sgraph[slot].exec = NULL;
+            return rc;
+        }
+        p->sgraph[slot].x = x_dev;
+        p->sgraph[slot].y = y_dev;
+    }
+    p->sgraph[slot].used = ++p->sgraph_clock;
+    return vnec_hip_ok(hipGraphLaunch(p->sgraph[slot].exec, s), "graph launch");
+}
+
+SpB_Info vnec_spmv_graph_ext(vnec_plan plan, const void *x_dev, void *y_dev)
+{
+    if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmv_graph_ext: not a plan");
+    if (!x_dev || !y_dev) return vnec_fail("vnec_spmv_graph_ext: NULL vector");
+    return plan_run_graph(plan, x_dev, y_dev);
+}
+
 SpB_Info vnec_spmm_colmajor_dev_ext(vnec_plan plan, const void *X_dev, void *Y_dev)
 {
     if (!plan || plan->magic != SPB_MAGIC_PLAN) return vnec_fail("vnec_spmm_colmajor_dev_ext: not a plan");
@@ -936,7 +983,11 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)
                          xv->base.n, (long)(plan->ncols * k), yv->base.n, (long)(plan->nrows * k));
     if (spb_vector_to_device(xv) != SpB_SUCCESS) return SpB_FAILURE;
     if (spb_vector_device_alloc(yv) != SpB_SUCCESS) return SpB_FAILURE;
-    if (vnec_plan_run(plan, xv->d_values, yv->d_values) != SpB_SUCCESS) return SpB_FAILURE;
+    /* device-resident vectors keep their device arrays between calls, so a
+     * repeated SpMV replays one captured launch sequence */
+    if ((xv->resident && yv->resident ? plan_run_graph(plan, xv->d_values, yv->d_values)
+                                      : vnec_plan_run(plan, xv->d_values, yv->d_values)) != SpB_SUCCESS)
+        return SpB_FAILURE;
     yv->dev_stale = 0;
     yv->host_stale = 1;
     if (!yv->resident) return spb_vector_from_device(yv);

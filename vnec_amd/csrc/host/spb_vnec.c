@@ -410,25 +410,32 @@ static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
          * are reused (the same algorithm, fewer lines per gather) */
         t.variant = !fp32 && vnec_csort_rule(a) ? VNEC_VAR_C : VNEC_VAR_L;
         t.grid = fp32 ? -1 : 0;
-    } else if (kind == KIND_D || kind == KIND_D32) {
-        t.variant = VNEC_VAR_D;
-        t.lanes = fp32 ? REF_L32 : REF_L;
     } else if (vnec_pattern_rule(a)) {
         /* rows of one column-offset pattern: pattern slices, the slice
-         * layout without column indices */
+         * layout without column indices.  This serves the D entry too: IRD
+         * sends matrices of dense column runs (banded, block-diagonal) to
+         * D, whose blocks the GPU streams at about half the rate of the
+         * slice layouts on them */
         t.variant = VNEC_VAR_G;
         t.ipt = 8;
     } else {
         uint32_t mask = 0;
+        int slices = 0;
         if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return NULL;
         vnec_tuning c[VNEC_NCAND];
         vnec_candidates_ext(c, VNEC_NCAND);
         for (int i = 0; i < VNEC_NCAND; i++)
-            if (c[i].variant == VNEC_VAR_S && ((mask >> i) & 1u)) {
-                t.variant = VNEC_VAR_S;
-                t.ipt = 3;
-                break;
-            }
+            if (c[i].variant == VNEC_VAR_S && ((mask >> i) & 1u)) slices = 1;
+        if (slices) {
+            t.variant = VNEC_VAR_S;
+            t.ipt = 3;
+        } else if (kind == KIND_D32) {
+            /* FP32 D blocks (the FP32 API has no selector) */
+            t.variant = VNEC_VAR_D;
+            t.lanes = REF_L32;
+        }
+        /* FP64 otherwise: the learned selector, whose candidates include
+         * the D blocks at widths 4, 8 and 16 */
     }
     if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &t) != SpB_SUCCESS) return NULL;
     return p;

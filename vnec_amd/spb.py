@@ -268,6 +268,11 @@ class Plan:
     def spmv_dev(self, x_ptr: int, y_ptr: int):
         L.check(L.load().vnec_spmv_dev_ext(self.h, x_ptr, y_ptr), "vnec_spmv_dev_ext")
 
+    def spmv_graph(self, x_ptr: int, y_ptr: int):
+        """vnec_spmv_graph_ext: the same SpMV replayed from the plan's
+        captured graph for this pointer pair."""
+        L.check(L.load().vnec_spmv_graph_ext(self.h, x_ptr, y_ptr), "vnec_spmv_graph_ext")
+
     def spmm_colmajor_dev(self, x_ptr: int, y_ptr: int):
         """Right-hand sides column-major: nrhs vectors of ncols (x) and of
         nrows (y), one after another."""
@@ -331,6 +336,21 @@ def spmv_vnec(kind: str, y: Vector, A: Matrix, x: Vector, plan, fp32: bool | Non
     getattr(L.load(), f)(y.h, A.h, x.h, C.cast(plan, C.c_void_p))
     err = L.last_error()
     return err
+
+
+def reference_plan_info(plan) -> TuningInfo:
+    """Tuning of the device plan behind a Build_VNEC_* result."""
+    lib = L.load()
+    h = lib.VNEC_plan_of_ext(C.cast(plan, C.c_void_p))
+    if not h:
+        raise RuntimeError("not a plan of this library")
+    t = L.Tuning()
+    nt, db = C.c_int64(), C.c_int64()
+    L.check(lib.vnec_plan_info_ext(h, C.byref(t), C.byref(nt), C.byref(db)), "vnec_plan_info_ext")
+    inv = {v: k for k, v in VNEC_VAR.items()}
+    inve = {v: k for k, v in VNEC_ENC.items()}
+    return TuningInfo(inv[t.variant], t.ipt, t.lanes, inve[t.encoding], t.nrhs, t.grid, nt.value,
+                      db.value)
 
 
 def free_vnec(kind: str, plan, fp32: bool = False):
