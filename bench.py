@@ -241,13 +241,19 @@ def selector_eval(spb, n):
             "max_rel_row_diff_learned_vs_plain": worst}
 
 
-def selection_label(mode, info):
-    """How the plan was chosen."""
+def selection_label(spb, A, mode, info, k):
+    """How the plan was chosen: forced, the MLP's own pick, or the MLP's pick
+    refined by timing its neighbours (beyond the model's training sizes)."""
     if mode is None:
         return "forced"
-    if info.variant == "G" and mode in ("learned", "rule"):
-        return "pattern rule (ahead of the selector)"
-    return mode
+    if mode == "rule" and info.variant == "G":
+        return "rule (pattern rule)"
+    if mode != "learned" or k > 1:
+        return mode
+    c = spb.candidates()[spb.select(A, "learned")]
+    if (c.variant, c.ipt) == (info.variant, info.ipt) and (c.grid == info.grid or c.grid >= 0 and info.grid >= 0):
+        return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
+    return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {info.variant}{info.ipt})"
 
 
 def kernel_name(info, k):
@@ -322,7 +328,7 @@ def measure(spb, lib, name, args, variant=None, mode=None, steps=None, warmup=No
     rec = {
         "workload": desc, "nrows": m.nrows, "nnz": m.nnz, "nrhs": k,
         "plan": {"variant": info.variant, "ipt": info.ipt, "grid": info.grid, "tiles": info.ntiles,
-                 "selection": selection_label(None if variant else mode, info)},
+                 "selection": selection_label(spb, A, None if variant else mode, info, k)},
         "format_bytes": fb, "csr_bytes": B,
         "ms_per_step": round(ms_step, 5),
         "GBps": round(fb / (ms_step / 1e3) / 1e9, 1),

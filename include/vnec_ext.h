@@ -256,11 +256,13 @@ SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *i
 
 /* Candidate configurations (single right-hand side, FP64).  Copies up to
  * max of them to out (may be NULL); returns VNEC_NCAND. */
-#define VNEC_NCAND 21
+#define VNEC_NCAND 22
 int vnec_candidates_ext(vnec_tuning *out, int max);
 /* Which candidates may run on A: bit c set when candidate c applies
  * (slice candidates need padding <= 3 and fewer than 2^31 slots; D
- * candidates need 8 * lanes * nnz <= 2^32 bytes of value blocks). */
+ * candidates need 8 * lanes * nnz <= 2^32 bytes of value blocks; the
+ * pattern-slice candidate (21) needs at least half of the rows on the
+ * dominant column-offset pattern and fewer than 2^31 slots). */
 SpB_Info vnec_candidate_mask_ext(const SpB_Matrix A, uint32_t *mask);
 /* Exhaustive autotune: builds every applicable candidate, times it with
  * hipGraph replay (best of 3 means over reps SpMVs) and returns the times
@@ -295,8 +297,10 @@ enum {
 SpB_Info vnec_feature_ints_ext(const SpB_Matrix A, uint64_t ints[VNEC_NFEAT_INTS]);
 /* The selector's 32 FP32 inputs from the integer statistics. */
 void vnec_features_from_ints_ext(const uint64_t ints[VNEC_NFEAT_INTS], float f[VNEC_NFEAT]);
-/* Candidate mask from the integer statistics (same rule as
- * vnec_candidate_mask_ext). */
+/* Candidate mask from the integer statistics (the rules of
+ * vnec_candidate_mask_ext that the statistics decide: the pattern-slice
+ * bit is set whenever the slots fit, as the row pattern is not among the
+ * statistics). */
 uint32_t vnec_candidate_mask_from_ints_ext(const uint64_t ints[VNEC_NFEAT_INTS]);
 
 /* MLP 32 -> 64 -> 64 -> VNEC_NCAND, ReLU, FP32.  Output c predicts

@@ -28,7 +28,10 @@ i64 = C.c_int64
 
 
 def _ptr(a):
-    return None if a is None else a.ctypes.data
+    """Pointer to a's data that keeps a alive for the call: callers pass
+    temporaries (np.ascontiguousarray of a strided view makes a copy), and
+    a bare address would let that copy be freed before the C code reads it."""
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 def lib():

@@ -139,9 +139,15 @@ def test_g_known_answer_is_exact():
     assert np.array_equal(y.to_array(), O.spmv_serial(m.ptr, m.col, val, x))
 
 
-def test_selection_rule_picks_g_on_large_stencils_only():
+def test_learned_selector_picks_g_on_stencils_and_rule_mode_keeps_its_rule():
+    """Pattern slices are candidate 21 of the learned selector: the MLP picks
+    them on a stencil; on R-MAT they do not apply (no row pattern covers
+    half of the rows); the rule mode's structural rule still picks them on
+    large stencils only."""
     big = g.stencil27(64, seed=9)  # 262,144 rows, 6.9M nonzeros; every row fits the pattern
     A = spb.Matrix.from_csr(big.nrows, big.ncols, big.ptr, big.col, big.val)
+    assert (spb.candidate_mask(A) >> 21) & 1
+    assert spb.candidates()[spb.select(A, "learned")].variant == "G"
     p = spb.Plan(A, "learned")
     assert p.info().variant == "G"
     y = spb.Vector(big.nrows)
@@ -150,11 +156,14 @@ def test_selection_rule_picks_g_on_large_stencils_only():
     ax, _ = O.spmv_abs(big.ptr, big.col, big.val, big.x)
     assert O.check_rows(y.to_array(), y0, ax, TOL)[0]
     p.free()
-    small = g.stencil27(16, seed=10)  # below 2^20 nonzeros: the selector decides
+    assert spb.Plan(A, "rule").info().variant == "G"
+    small = g.stencil27(16, seed=10)  # below 2^20 nonzeros: the rule leaves it to slices / L
     B = spb.Matrix.from_csr(small.nrows, small.ncols, small.ptr, small.col, small.val)
-    assert spb.Plan(B, "learned").info().variant != "G"
+    assert spb.Plan(B, "rule").info().variant != "G"
     R = spb.Matrix.from_csr(*(lambda r: (r.nrows, r.ncols, r.ptr, r.col, r.val))(g.rmat(17, 16, seed=11)))
+    assert not (spb.candidate_mask(R) >> 21) & 1
     assert spb.Plan(R, "learned").info().variant != "G"
+    assert spb.Plan(R, "rule").info().variant != "G"
 
 
 @pytest.mark.parametrize("enc,tol", [("hilo", TOL), ("hi", 2.0 ** -23)])

@@ -171,9 +171,10 @@ def test_dataset_labels_and_masks():
     t = d["times"]
     assert t.shape[1] == L.NCAND and len(np.unique(d["index"])) == len(d["index"])
     np.testing.assert_array_equal(d["best"], np.argmin(t, axis=1))
+    g_bit = 1 << 21  # pattern slices: gated by the row pattern, not the statistics
     for i in range(len(t)):
         m = int(d["mask"][i])
-        assert m == spb.mask_from_ints(d["ints"][i])
+        assert m | g_bit == spb.mask_from_ints(d["ints"][i]) | g_bit
         finite = np.isfinite(t[i])
         assert not (finite & ~((m >> np.arange(L.NCAND)) & 1).astype(bool)).any()
         assert (t[i][finite] >= t[i][d["best"][i]]).all()
@@ -186,8 +187,10 @@ def test_compiled_model_regret_on_held_out():
     if not trained:
         pytest.skip("no trained model compiled in")
     from vnec_amd.selector import model_file
+    from vnec_amd.selector import specs
     params, mu, sd = model_file.load()
-    held = d["index"] >= 2000
+    held = specs.is_held(d["index"])
+    assert held.sum() == 200
     t = d["times"][held]
     ch, _ = MO.select(params, mu, sd, d["feats"][held], d["mask"][held])
     best = t.min(axis=1)
@@ -198,6 +201,13 @@ def test_compiled_model_regret_on_held_out():
     ref = t[np.arange(len(t)), d["reference"][held]]
     assert agg <= 1.10 and geo <= 1.10, (agg, geo)
     assert np.exp(np.mean(np.log(ref / best))) > geo     # gains over the reference dispatch
+    # held-out stencils: pattern slices (candidate 21) are the labels the model must find
+    hx = specs.is_held_extra(d["index"])
+    if hx.any():
+        te = d["times"][hx]
+        che, _ = MO.select(params, mu, sd, d["feats"][hx], d["mask"][hx])
+        chosen_e = te[np.arange(len(te)), che]
+        assert chosen_e.sum() / te.min(axis=1).sum() <= 1.10
 
 
 def test_compiled_model_source_matches_npz():
@@ -210,4 +220,4 @@ def test_compiled_model_source_matches_npz():
     np.testing.assert_array_equal(p, P)
     np.testing.assert_array_equal(mu, MU)
     np.testing.assert_array_equal(sd, SD)
-    assert spb.selector_info() == (True, 2000)
+    assert spb.selector_info() == (True, 2100)

@@ -247,7 +247,7 @@ SIGNATURES = {
 }
 
 # Constants of include/vnec_ext.h used by the selector bindings.
-NCAND = 21
+NCAND = 22
 NFEAT = 32
 NFEAT_INTS = 32
 MLP_HIDDEN = 64

@@ -286,6 +286,47 @@ vg_csr *vg_stencil27(int n, uint64_t seed)
     return m;
 }
 
+/* Grid stencil on an nx x ny x nz grid (nz = 1: two-dimensional), row
+ * (i*ny + j)*nz + k, truncated at the boundary.  pts = 5 or 9 (2-D star /
+ * box), 7 or 27 (3-D star / box).  Diagonal (pts - 1) + U[0,1),
+ * off-diagonals U[-1,0). */
+vg_csr *vg_stencil(int nx, int ny, int nz, int pts, uint64_t seed)
+{
+    const int box = pts == 9 || pts == 27;
+    const int dz = nz > 1 ? 1 : 0;
+    if (nx < 1 || ny < 1 || nz < 1 || !(pts == 5 || pts == 9 || pts == 7 || pts == 27) ||
+        ((pts == 5 || pts == 9) && nz != 1) || ((pts == 7 || pts == 27) && nz < 2))
+        return NULL;
+    vg_rng r;
+    rng_seed(&r, seed);
+    int64_t N = (int64_t)nx * ny * nz;
+    vg_csr *m = csr_alloc(N, N, N * pts);
+    int64_t p = 0;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                int64_t row = ((int64_t)i * ny + j) * nz + k;
+                for (int di = -1; di <= 1; di++)
+                    for (int dj = -1; dj <= 1; dj++)
+                        for (int dk = -dz; dk <= dz; dk++) {
+                            int nzero = (di != 0) + (dj != 0) + (dk != 0);
+                            if (!box && nzero > 1) continue;
+                            int ii = i + di, jj = j + dj, kk = k + dk;
+                            if (ii < 0 || ii >= nx || jj < 0 || jj >= ny || kk < 0 || kk >= nz) continue;
+                            m->col[p++] = (int32_t)(((int64_t)ii * ny + jj) * nz + kk);
+                        }
+                m->ptr[row + 1] = p;
+            }
+    m->nnz = p;
+    for (int64_t row = 0; row < N; row++)
+        for (int64_t q = m->ptr[row]; q < m->ptr[row + 1]; q++) {
+            double u = rng_u01(&r);
+            m->val[q] = (m->col[q] == row) ? (double)(pts - 1) + u : u - 1.0;
+        }
+    fill_x_pm1(m, &r, 1);
+    return m;
+}
+
 /* --------------------------------------------- per-row random patterns */
 static int cmp_i32(const void *a, const void *b)
 {

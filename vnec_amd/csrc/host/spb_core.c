@@ -183,6 +183,8 @@ static void matrix_drop_device(spb_csr *a)
     a->tuned_cand = -1;
     a->refined_ipt = 0;
     a->feat_ready = 0;
+    a->pattern_ready = 0;
+    a->refined_slice_ipt = 0;
 }
 
 SpB_Info SpB_Matrix_free(SpB_Matrix *A)

@@ -57,7 +57,12 @@ typedef struct spb_csr {
                            (0: not yet): the fastest of L / C under
                            both schedules                            */
     int refined_variant, refined_grid;
+    int refined_slice_ipt; /* slice-family pick measured at this ipt (0: not
+                              yet): the fastest of S at it and variant G  */
+    int refined_slice_variant, refined_slice_ipt_out;
     int feat_ready;     /* feat_ints holds the selector's statistics     */
+    int pattern_ready;  /* pattern_rows holds the count below            */
+    int64_t pattern_rows; /* rows that fit the dominant column-offset pattern */
     uint64_t feat_ints[VNEC_NFEAT_INTS];
 } spb_csr;
 
@@ -198,6 +203,9 @@ double vnec_slice_padding(const spb_csr *a);
  * granularity autotuned in VNEC_SELECT_AUTOTUNE mode, else 5 items per
  * thread). */
 SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t);
+/* The learned selector's pick among every candidate but pattern slices
+ * (the plan for the rows a pattern plan leaves over). */
+SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t);
 /* Times the FP64 merge-path plan t (variant L or C) in its four forms —
  * CSR-ordered (L) or column-sorted (C) tiles, persistent (grid 0) or
  * per-tile carries (grid -1) — at t's granularity, and sets t's variant
@@ -207,6 +215,10 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
  * column-offset pattern (variant G applies), and tiles reuse columns
  * (variant C beats L).  FP64 matrices. */
 int vnec_pattern_rule(spb_csr *a);
+/* Times the FP64 slice-family plan t (variant S, or G) as 64-row slices at
+ * t's granularity (3 items per thread for G) and as pattern slices where
+ * they apply, and sets t to the faster.  Cached on the matrix. */
+SpB_Info vnec_refine_slice_plan(spb_csr *a, vnec_tuning *t);
 /* Fills g's x-window fields for pattern pat (L ascending offsets): the
  * merged column runs a workgroup's 256 rows read, staged in LDS when they
  * total at most VNEC_PATTERN_WIN_MAX entries (else wsize = 0). */

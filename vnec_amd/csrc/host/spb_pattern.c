@@ -264,7 +264,7 @@ SpB_Info vnec_pattern_build(struct vnec_plan_s *p, spb_csr *a)
         }
     } else if (rc == SpB_SUCCESS && nirr) {
         vnec_tuning t;
-        rc = vnec_select_tuning(spb_as_csr(p->g_sub), VNEC_SELECT_LEARNED, 1, &t);
+        rc = vnec_select_tuning_no_pattern(spb_as_csr(p->g_sub), &t);
         if (rc == SpB_SUCCESS && enc != VNEC_ENC_FP64) { /* the same value encoding */
             if (t.variant == VNEC_VAR_D || t.variant == VNEC_VAR_H || t.variant == VNEC_VAR_C) { /* FP64 only */
                 vnec_tuning l = {VNEC_VAR_L, 7, 64, enc, 1, 0};

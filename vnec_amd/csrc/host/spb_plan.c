@@ -412,12 +412,12 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         tune = f;
     } else if (t && t->variant >= 0) {
         tune = *t;
-    } else if ((!t || ((t->variant == VNEC_SELECT_LEARNED || t->variant == VNEC_SELECT_RULE) &&
-                       t->encoding == VNEC_ENC_FP64 && t->nrhs <= 1)) &&
+    } else if (t && t->variant == VNEC_SELECT_RULE && t->encoding == VNEC_ENC_FP64 && t->nrhs <= 1 &&
                vnec_pattern_rule(a)) {
-        /* most rows follow one column-offset pattern: pattern slices move
-         * 8 bytes per regular nonzero, fewer than any column-indexed
-         * candidate */
+        /* the rule mode: most rows follow one column-offset pattern, so
+         * pattern slices move 8 bytes per regular nonzero, fewer than any
+         * column-indexed candidate (the learned mode has pattern slices
+         * among its candidates instead) */
         vnec_tuning g = {VNEC_VAR_G, 8, 64, VNEC_ENC_FP64, 1, -1};
         tune = g;
     } else {
@@ -427,7 +427,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (t) tune.encoding = t->encoding;
         /* a candidate without a kernel for the requested encoding falls back
          * to the merge-path kernel, which has them all */
-        const int ok = tune.variant == VNEC_VAR_S ? 1
+        const int ok = tune.variant == VNEC_VAR_S || tune.variant == VNEC_VAR_G ? 1
                        : tune.variant == VNEC_VAR_D ? tune.encoding == VNEC_ENC_FP64
                        : vnec_hip_merge_supported(tune.variant, tune.ipt, tune.encoding, tune.nrhs);
         if (!ok) {
@@ -448,6 +448,14 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
                        (mode == VNEC_SELECT_LEARNED && (int64_t)a->nnz > SELECTOR_TRAINED_MAX_NNZ)) {
                 if (vnec_refine_merge_plan(a, &tune) != SpB_SUCCESS) return SpB_FAILURE;
             }
+        }
+        if ((tune.variant == VNEC_VAR_S || tune.variant == VNEC_VAR_G) && tune.nrhs == 1 &&
+            tune.encoding == VNEC_ENC_FP64 && mode == VNEC_SELECT_LEARNED &&
+            (int64_t)a->nnz > SELECTOR_TRAINED_MAX_NNZ) {
+            /* beyond the matrix sizes the model was trained on, a pick
+             * among the slice layouts is measured: 64-row slices or
+             * pattern slices, whichever is faster */
+            if (vnec_refine_slice_plan(a, &tune) != SpB_SUCCESS) return SpB_FAILURE;
         }
     }
     if (tune.nrhs <= 0) tune.nrhs = 1;

@@ -28,12 +28,16 @@ static const vnec_tuning CANDIDATES[VNEC_NCAND] = {
     {VNEC_VAR_E, 5, 64, 0, 1, 0},   {VNEC_VAR_E, 7, 64, 0, 1, 0},
     {VNEC_VAR_D, 7, 4, 0, 1, -1},   {VNEC_VAR_D, 7, 8, 0, 1, -1},   {VNEC_VAR_D, 7, 16, 0, 1, -1},
     {VNEC_VAR_C, 9, 64, 0, 1, 0},   {VNEC_VAR_C, 11, 64, 0, 1, 0},  {VNEC_VAR_C, 7, 64, 0, 1, -1},
-    {VNEC_VAR_C, 11, 64, 0, 1, -1},
+    {VNEC_VAR_C, 11, 64, 0, 1, -1}, {VNEC_VAR_G, 8, 64, 0, 1, -1},
 };
 /* Candidates named by the fixed selection modes. */
 #define CAND_S3 1
 #define CAND_L7_PERSISTENT 4
 #define CAND_D8 15 /* the reference's own block width */
+#define CAND_G 21  /* pattern slices */
+/* Pattern slices apply when at least this fraction of the rows fits the
+ * matrix's dominant column-offset pattern (the others run as a sub-plan). */
+#define G_MIN_ROW_FRACTION 0.5
 
 /* Slice candidates apply only up to this padding (slots / nnz). */
 #define TUNE_MAX_PADDING 3.0
@@ -64,6 +68,10 @@ uint32_t vnec_candidate_mask_from_ints_ext(const uint64_t v[VNEC_NFEAT_INTS])
         if (CANDIDATES[c].variant == VNEC_VAR_S && !slices_ok) continue;
         if (CANDIDATES[c].variant == VNEC_VAR_D &&
             (nnz == 0 || nnz * 8ull * (uint64_t)CANDIDATES[c].lanes > TUNE_MAX_D_BYTES))
+            continue;
+        /* pattern slices: every row holds VNEC_PATTERN_MAX slots at most */
+        if (CANDIDATES[c].variant == VNEC_VAR_G &&
+            (nnz == 0 || ns * 64ull * VNEC_PATTERN_MAX >= (uint64_t)INT32_MAX))
             continue;
         m |= 1u << c;
     }
@@ -210,6 +218,15 @@ SpB_Info vnec_candidate_mask_ext(const SpB_Matrix A, uint32_t *mask)
     if (!mask) return vnec_fail("vnec_candidate_mask_ext: NULL output");
     if (vnec_feature_ints_ext(A, ints) != SpB_SUCCESS) return SpB_FAILURE;
     *mask = vnec_candidate_mask_from_ints_ext(ints);
+    if ((*mask >> CAND_G) & 1u) {
+        /* pattern slices need most rows on one column-offset pattern */
+        spb_csr *a = spb_as_csr(A);
+        if (!a->pattern_ready) {
+            if (vnec_pattern_coverage(a, &a->pattern_rows) != SpB_SUCCESS) return SpB_FAILURE;
+            a->pattern_ready = 1;
+        }
+        if ((double)a->pattern_rows < G_MIN_ROW_FRACTION * (double)a->base.row) *mask &= ~(1u << CAND_G);
+    }
     return SpB_SUCCESS;
 }
 
@@ -220,9 +237,8 @@ SpB_Info vnec_autotune_ext(const SpB_Matrix A, int reps, double times_ms[VNEC_NC
     if (!a || !times_ms || reps < 1) return vnec_fail("vnec_autotune_ext: bad arguments");
     if (a->base.element_type != SpB_FP64)
         return vnec_fail("vnec_autotune_ext: the candidates are FP64 kernels; A must hold FP64 values");
-    uint64_t ints[VNEC_NFEAT_INTS];
-    if (vnec_feature_ints_ext(A, ints) != SpB_SUCCESS) return SpB_FAILURE;
-    uint32_t mask = vnec_candidate_mask_from_ints_ext(ints);
+    uint32_t mask = 0;
+    if (vnec_candidate_mask_ext(A, &mask) != SpB_SUCCESS) return SpB_FAILURE;
     int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
     double *hx = (double *)malloc(sizeof(double) * (size_t)(nc ? nc : 1));
     double *dx = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nc ? nc : 1), NULL);
@@ -530,13 +546,17 @@ static SpB_Info selector_device_model(void)
     return rc;
 }
 
-static SpB_Info select_learned(spb_csr *a, int *cand)
+/* The MLP's pick among the applicable candidates that `allow` keeps. */
+static SpB_Info select_learned(spb_csr *a, uint32_t allow, int *cand)
 {
     uint64_t ints[VNEC_NFEAT_INTS];
     float f[VNEC_NFEAT];
+    uint32_t mask = 0;
     if (vnec_feature_ints_ext((SpB_Matrix)a, ints) != SpB_SUCCESS) return SpB_FAILURE;
     vnec_features_from_ints_ext(ints, f);
-    uint32_t mask = vnec_candidate_mask_from_ints_ext(ints);
+    if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return SpB_FAILURE;
+    mask &= allow;
+    if (!mask) return vnec_fail("selector: no candidate applies");
     if (selector_device_model() != SpB_SUCCESS) return SpB_FAILURE;
     hipStream_t s = vnec_stream();
     float *df = (float *)vnec_malloc_dev(sizeof f + sizeof(uint32_t) + sizeof(int32_t), NULL);
@@ -565,7 +585,7 @@ SpB_Info vnec_select_ext(const SpB_Matrix A, int mode, int *candidate)
     if (mode == VNEC_SELECT_LEARNED && !model_trained()) mode = VNEC_SELECT_RULE;
     switch (mode) {
     case VNEC_SELECT_LEARNED:
-        return select_learned(a, candidate);
+        return select_learned(a, ~0u, candidate);
     case VNEC_SELECT_AUTOTUNE: {
         if (a->tuned_cand >= 0) {
             *candidate = a->tuned_cand;
@@ -638,6 +658,72 @@ static SpB_Info autotune_multi(spb_csr *a, int nrhs, int *ipt)
     return rc;
 }
 
+/* Best of three hipGraph-timed means of `iters` SpMVs of plan tuning u on
+ * a (INFINITY when u does not build). */
+static SpB_Info time_tuning(spb_csr *a, const vnec_tuning *u, const double *dx, double *dy, int iters,
+                            double *ms_out)
+{
+    *ms_out = INFINITY;
+    vnec_plan p = NULL;
+    if (vnec_plan_build_ext(&p, (SpB_Matrix)a, u) != SpB_SUCCESS) {
+        vnec_clear_error(); /* not a contender */
+        return SpB_SUCCESS;
+    }
+    SpB_Info rc = SpB_SUCCESS;
+    for (int r = 0; rc == SpB_SUCCESS && r < 3; r++) {
+        double ms = 0.0;
+        rc = vnec_time_dev_ext(p, dx, dy, iters, 1, 0, &ms);
+        if (rc == SpB_SUCCESS && ms < *ms_out) *ms_out = ms;
+    }
+    vnec_plan_free_ext(&p);
+    return rc;
+}
+
+SpB_Info vnec_refine_slice_plan(spb_csr *a, vnec_tuning *t)
+{
+    const int ipt = t->variant == VNEC_VAR_S ? t->ipt : 3;
+    if (a->refined_slice_ipt == ipt) {
+        t->variant = a->refined_slice_variant;
+        t->ipt = a->refined_slice_ipt_out;
+        t->grid = -1;
+        return SpB_SUCCESS;
+    }
+    uint32_t mask = 0;
+    if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return SpB_FAILURE;
+    int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
+    double *hx = (double *)malloc(sizeof(double) * (size_t)(nc ? nc : 1));
+    double *dx = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nc ? nc : 1), NULL);
+    double *dy = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nr ? nr : 1), NULL);
+    SpB_Info rc = (hx && dx && dy) ? SpB_SUCCESS : vnec_fail("out of memory");
+    for (int64_t i = 0; rc == SpB_SUCCESS && i < nc; i++) hx[i] = 1.0 / (double)(1 + (i % 7));
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
+    vnec_tuning s = *t, g = CANDIDATES[CAND_G];
+    s.variant = VNEC_VAR_S;
+    s.ipt = ipt;
+    s.grid = -1;
+    g.encoding = t->encoding;
+    double ts = INFINITY, tg = INFINITY;
+    if (rc == SpB_SUCCESS && (mask & 7u)) rc = time_tuning(a, &s, dx, dy, 20, &ts); /* slices apply */
+    if (rc == SpB_SUCCESS && ((mask >> CAND_G) & 1u)) rc = time_tuning(a, &g, dx, dy, 20, &tg);
+    free(hx);
+    vnec_free_dev(dx);
+    vnec_free_dev(dy);
+    if (rc != SpB_SUCCESS) return rc;
+    if (tg < ts) {
+        t->variant = VNEC_VAR_G;
+        t->ipt = g.ipt;
+    } else if (ts < INFINITY) {
+        t->variant = VNEC_VAR_S;
+        t->ipt = ipt;
+    }
+    t->grid = -1;
+    a->refined_slice_ipt = ipt;
+    a->refined_slice_variant = t->variant;
+    a->refined_slice_ipt_out = t->ipt;
+    return SpB_SUCCESS;
+}
+
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
 {
     if (a->refined_ipt == t->ipt) {
@@ -685,6 +771,15 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     a->refined_ipt = t->ipt;
     a->refined_variant = t->variant = bv;
     a->refined_grid = t->grid = bg;
+    return SpB_SUCCESS;
+}
+
+SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t)
+{
+    int c = CAND_L7_PERSISTENT;
+    if (model_trained() && select_learned(a, ~(1u << CAND_G), &c) != SpB_SUCCESS) return SpB_FAILURE;
+    if (!model_trained() && a->nnz > 0 && vnec_slice_padding(a) <= RULE_MAX_PADDING) c = CAND_S3;
+    *t = CANDIDATES[c];
     return SpB_SUCCESS;
 }
 

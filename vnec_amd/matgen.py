@@ -43,6 +43,7 @@ def _load():
             "vg_skew": [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
                         C.c_int64, C.c_uint64],
             "vg_stencil27": [C.c_int, C.c_uint64],
+            "vg_stencil": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64],
             "vg_batch": [C.c_int, C.c_int64, C.c_int64, C.c_double, C.c_double, C.c_int64,
                          C.c_uint64],
             "vg_banded": [C.c_int64, C.c_int, C.c_uint64],
@@ -118,6 +119,14 @@ def batch(nmat: int = 512, rmin: int = 1000, rmax: int = 200_000, mu_lo=4.0, mu_
           nrhs: int = 1, seed: int = 1) -> Csr:
     """Block-diagonal batch; ``col`` holds LOCAL column ids of each member."""
     return _take(_load().vg_batch(nmat, rmin, rmax, mu_lo, mu_hi, nrhs, seed), f"batch{nmat}")
+
+
+def stencil(nx: int, ny: int, nz: int, pts: int, seed: int = 1) -> Csr:
+    """Grid stencil: pts = 5 or 9 on an nx x ny grid (nz = 1), 7 or 27 on
+    an nx x ny x nz grid; truncated at the boundary."""
+    if pts in (5, 9) and nz != 1 or pts in (7, 27) and nz < 2 or pts not in (5, 9, 7, 27):
+        raise ValueError(f"stencil: pts {pts} on a {nx} x {ny} x {nz} grid")
+    return _take(_load().vg_stencil(nx, ny, nz, pts, seed), f"stencil{pts}_{nx}x{ny}x{nz}")
 
 
 def banded(n: int, hb: int, seed: int = 1) -> Csr:

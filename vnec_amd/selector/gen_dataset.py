@@ -33,12 +33,12 @@ def _save(path: str, rec: dict):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--start", type=int, default=0)
-    ap.add_argument("--count", type=int, default=specs.N_TOTAL)
+    ap.add_argument("--count", type=int, default=specs.N_ALL)
     ap.add_argument("--out", required=True)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--threads", type=int, default=6, help="generator threads")
     args = ap.parse_args(argv)
-    idx = list(range(args.start, min(args.start + args.count, specs.N_TOTAL)))
+    idx = list(range(args.start, min(args.start + args.count, specs.N_ALL)))
     spb.init()
     rec: dict[str, list] = {k: [] for k in ("index", "spec", "ints", "feats", "mask", "times",
                                             "best", "rule", "reference", "gen_s", "tune_s")}
@@ -63,7 +63,7 @@ def main(argv=None) -> int:
         del m
         ints = spb.feature_ints(A)
         feats = spb.features_from_ints(ints)
-        mask = spb.mask_from_ints(ints)
+        mask = spb.candidate_mask(A)  # the statistics' rules and the row-pattern gate
         times, best = spb.autotune(A, args.reps)
         rule = spb.select(A, "rule")
         ref = spb.select(A, "reference")

@@ -14,6 +14,17 @@ family i % 5 with parameters drawn from numpy's PCG64 seeded with
 
 Every family caps its expected entry count at 2^24 by shrinking n.
 Values and x are FP64 uniform[-1, 1] from the generator's own seed.
+
+Beyond these 2200, matrices 2200..2319 (2200..2299 train, 2300..2319 held
+out) are grid stencils, a structure none of the five families has (rows
+holding a few short runs of columns at fixed offsets, the layout pattern
+slices serve):
+
+  stencil    2-D 5- or 9-point on nx x ny, or 3-D 7- or 27-point on
+             nx x ny x nz, grid points log-uniform in [2^13, 2^20] (at most
+             2^24 entries), sides within a factor 2 of each other
+
+Stencil values: diagonal (points - 1) + U[0, 1), off-diagonals U[-1, 0).
 """
 from __future__ import annotations
 
@@ -26,9 +37,31 @@ from .. import matgen
 N_TRAIN = 2000
 N_HELD = 200
 N_TOTAL = N_TRAIN + N_HELD
+N_EXTRA_TRAIN = 100  # stencils 2200..2299
+N_EXTRA_HELD = 20    # stencils 2300..2319
+N_ALL = N_TOTAL + N_EXTRA_TRAIN + N_EXTRA_HELD
 FAMILIES = ("rmat", "banded", "blockdiag", "uniform", "powerlaw")
+
 SEED_BASE = 20250
 MAX_NNZ = 1 << 24
+
+
+def is_train(i):
+    """Training matrices: the five families' first 2000 and the first 100
+    stencils (works elementwise on index arrays)."""
+    i = np.asarray(i)
+    return (i < N_TRAIN) | ((i >= N_TOTAL) & (i < N_TOTAL + N_EXTRA_TRAIN))
+
+
+def is_held(i):
+    """The five families' held-out matrices."""
+    i = np.asarray(i)
+    return (i >= N_TRAIN) & (i < N_TOTAL)
+
+
+def is_held_extra(i):
+    """Held-out stencils."""
+    return np.asarray(i) >= N_TOTAL + N_EXTRA_TRAIN
 
 
 def _logu(rng, lo: float, hi: float) -> float:
@@ -42,9 +75,21 @@ def _powerlaw_mean(alpha: float, xmin: int, cap: int) -> float:
 
 def spec(i: int) -> dict:
     rng = np.random.default_rng([SEED_BASE, i])
-    fam = FAMILIES[i % len(FAMILIES)]
+    fam = FAMILIES[i % len(FAMILIES)] if i < N_TOTAL else "stencil"
     s: dict = {"index": i, "family": fam, "seed": SEED_BASE * 10000 + i}
-    if fam == "rmat":
+    if fam == "stencil":
+        pts = int(rng.choice([5, 9, 7, 27]))
+        npts = _logu(rng, 1 << 13, 1 << 20)
+        npts = min(npts, MAX_NNZ / pts)
+        dims = 2 if pts in (5, 9) else 3
+        aspect = 2.0 ** rng.uniform(-1, 1, dims)
+        aspect /= np.prod(aspect) ** (1.0 / dims)
+        side = npts ** (1.0 / dims)
+        n = [max(4, int(round(side * a))) for a in aspect]
+        while int(np.prod(n)) * pts > MAX_NNZ:
+            n[int(np.argmax(n))] -= 1
+        s.update(pts=pts, nx=n[0], ny=n[1], nz=n[2] if dims == 3 else 1)
+    elif fam == "rmat":
         scale = int(rng.integers(14, 21))
         ef = int(round(_logu(rng, 4, 32)))
         ef = max(1, min(ef, MAX_NNZ >> scale))
@@ -74,6 +119,8 @@ def spec(i: int) -> dict:
 
 def generate(s: dict) -> matgen.Csr:
     fam = s["family"]
+    if fam == "stencil":
+        return matgen.stencil(s["nx"], s["ny"], s["nz"], s["pts"], seed=s["seed"])
     if fam == "rmat":
         return matgen.rmat(s["scale"], s["ef"], s["a"], s["b"], s["c"], seed=s["seed"])
     if fam == "banded":

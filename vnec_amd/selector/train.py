@@ -73,8 +73,9 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     with np.load(args.data, allow_pickle=False) as z:
         d = {k: z[k] for k in z.files}
-    held = d["index"] >= specs.N_TRAIN
-    tr = ~held
+    tr = specs.is_train(d["index"])
+    held = specs.is_held(d["index"])
+    held_extra = specs.is_held_extra(d["index"])
     F = d["feats"].astype(np.float32)
     mu = F[tr].mean(axis=0)
     sd = F[tr].std(axis=0)
@@ -92,7 +93,9 @@ def main(argv=None) -> int:
     print(f"epoch {args.epochs:4d} loss {hist[-1]:.5f}  ({train_s:.2f} s on the GPU)", flush=True)
     t = d["times"]
     out = {}
-    for name, m in (("train", tr), ("held_out", held)):
+    for name, m in (("train", tr), ("held_out", held), ("held_out_stencils", held_extra)):
+        if not m.any():
+            continue
         ch = spb.mlp_select(params, mu, sd, F[m], d["mask"][m].astype(np.uint32))
         fixed = int(np.argmin([np.exp(np.log(np.where(np.isfinite(t[tr][:, c]), t[tr][:, c], 1e30)
                                             / t[tr].min(1)).mean()) for c in range(L.NCAND)]))

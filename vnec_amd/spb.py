@@ -46,7 +46,9 @@ def device_count() -> int:
 
 
 def _p(a: np.ndarray):
-    return a.ctypes.data
+    """Pointer to a's data that keeps a alive for the call (a may be a
+    temporary copy)."""
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def _spb_type(dtype) -> int:
