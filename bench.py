@@ -55,7 +55,7 @@ KERNEL_NAME = {("S", False): "k_spmv_slice", ("L", True): "k_spmv_stream",
                ("C", False): "k_spmv_merge1"}
 PLAIN_CANDIDATE = 4  # L, 7 items per thread, persistent: the plain merge-path CSR kernel
 METRIC = "FP64 SpMV GB/s as % of HBM peak; learned-vs-plain speedup at equal error"
-R18_FLUSH = 1 << 29  # 512 MiB scribbled between SpMVs for an HBM (not cache) figure
+R18_FLUSH = 1 << 29  # 512 MiB read between SpMVs (> L2 + 256 MB Infinity Cache): an HBM figure
 
 
 def log(*a):

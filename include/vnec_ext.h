@@ -165,9 +165,11 @@ SpB_Info vnec_spmv_graph_ext(vnec_plan plan, const void *x_dev, void *y_dev);
 SpB_Info vnec_spmm_colmajor_dev_ext(vnec_plan plan, const void *X_dev, void *Y_dev);
 /* Times `iters` back-to-back SpMVs on the library stream with HIP events
  * (optionally replayed from one captured hipGraph); *ms = mean ms per SpMV.
- * If flush_bytes > 0 a buffer of that size is overwritten before each SpMV
- * (outside the timed kernels: *ms then excludes it, using per-SpMV event
- * pairs) so that the matrix cannot stay resident in the on-die caches. */
+ * If flush_bytes > 0 a buffer of that size (make it larger than the L2 and
+ * Infinity Cache together) is read with cache-allocating loads before each
+ * SpMV, so that the matrix cannot stay resident in the on-die caches; *ms
+ * is then the time of iters (flush, SpMV) pairs less that of iters flushes
+ * alone, each sequence back to back between one pair of events. */
 SpB_Info vnec_time_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev, int iters,
                            int use_graph, size_t flush_bytes, double *ms);
 

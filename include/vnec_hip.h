@@ -187,10 +187,14 @@ int vnec_hip_gather_probe(const double *val, const int32_t *col, int64_t nnz, co
 
 /* Streaming copy of bytes (multiple of 16) from src to dst. */
 int vnec_hip_stream_copy(const void *src, void *dst, size_t bytes, void *stream);
-/* Streaming read of bytes (a multiple of 16) from src; out (>= 4096 words)
- * is written only for an input whose words XOR-fold to 0x9E3779B9 per
- * thread (a probe: pass bytes of a memset pattern). */
+/* Streaming read of bytes (a multiple of 8) from src, one contiguous 64 KiB
+ * chunk per workgroup; out (>= 4096 words) is written only for an input
+ * whose words XOR-fold to 0x9E3779B9 per thread (a probe or a cache flush:
+ * pass bytes of a memset pattern). */
 int vnec_hip_stream_read(const void *src, size_t bytes, void *out, void *stream);
+/* The same read with cache-allocating loads: reading more than the L2 and
+ * Infinity Cache hold evicts their contents without dirtying them. */
+int vnec_hip_flush_read(const void *src, size_t bytes, void *out, void *stream);
 
 /* ---- variant S: 64-row slices, [l][lane] layout ---- */
 typedef struct {
@@ -279,8 +283,8 @@ typedef struct {
     int64_t nslices;       /* ceil(nrows / 64)                                */
     const double *x;
     double *y;
-    /* x window of a workgroup's 256 rows r0 .. r0 + 255: the union of the
-       column ranges r0 + pat[l] + [0, 256), as nwin runs (offset wst[k] from
+    /* x window of a workgroup's VNEC_PATTERN_ROWS rows r0 ..: the union of the
+       column ranges r0 + pat[l] + [0, VNEC_PATTERN_ROWS), as nwin runs (offset wst[k] from
        r0, length wlen[k], LDS start wbase[k]), staged in LDS once; position
        l of local row t reads LDS entry wpos[l] + t.  wsize = 0: direct x
        reads.  ncols bounds the staged columns. */
@@ -292,6 +296,8 @@ typedef struct {
 } vnec_pattern_args;
 /* Largest x window staged in LDS (doubles). */
 #define VNEC_PATTERN_WIN_MAX 4096
+/* Rows per workgroup of the pattern kernel (the window covers their runs). */
+#define VNEC_PATTERN_ROWS 512
 /* masks[i] = positions of the pattern row i holds; flags[i] = 1 when row i
  * is not a subset of the pattern (a column outside it, or repeated). */
 int vnec_hip_pattern_match(const int32_t *ptr, const int32_t *col, int64_t nrows, const int32_t *pat,

@@ -207,6 +207,7 @@ SIGNATURES = {
     "vnec_hip_spmv_strided": (C.c_int, [vp, C.c_int, vp]),
     "vnec_hip_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "vnec_hip_stream_read": (C.c_int, [vp, C.c_size_t, vp, vp]),
+    "vnec_hip_flush_read": (C.c_int, [vp, C.c_size_t, vp, vp]),
     "vnec_hip_slice_len": (C.c_int, [vp, i64, i64, vp, vp, vp]),
     "vnec_hip_slice_tile_rows": (C.c_int, [vp, i64, vp, i64, vp, vp]),
     "vnec_hip_slice_fill": (C.c_int, [vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),

@@ -98,31 +98,67 @@ __device__ __forceinline__ const float *pattern_x(const vnec_pattern_args &a, fl
 __device__ __forceinline__ double *pattern_y(const vnec_pattern_args &a, double) { return a.y; }
 __device__ __forceinline__ float *pattern_y(const vnec_pattern_args &a, float) { return a.y32; }
 
-// One wavefront per slice, four slices per workgroup, XCD-aware order.
-// WIN: the workgroup first stages its x window in LDS (a.wsize doubles,
-// coalesced runs), so the rows' x reads, each entry shared by up to L rows
+// Rows (and threads) per workgroup of the pattern kernel: eight slices.
+// A wider workgroup shares its x window's halo among more rows (27-point
+// stencil: 4.5 staged entries per row at 512 rows, 6 at 256).
+#define G_ROWS VNEC_PATTERN_ROWS
+#define G_WAVES (G_ROWS / VNEC_WAVE)
+
+// Stages the window's runs into LDS with 16-byte loads: run k holds
+// columns r0 + wst[k] + [0, wlen[k]); each lane loads the 16-byte-aligned
+// group of x entries that covers its share (groups are aligned to x's own
+// address, whatever its alignment) and stores the entries that fall in the
+// run; columns outside [0, ncols) stage as 0.
+template <typename F>
+__device__ __forceinline__ void stage_window(const vnec_pattern_args &a, const F *__restrict__ x, int64_t r0, F *s_xw)
+{
+    constexpr int E = 16 / sizeof(F);  // entries per 16-byte group
+    typedef F vec __attribute__((ext_vector_type(E)));
+    const int mis = (int)(((uintptr_t)x / sizeof(F)) % E);  // x + c is 16-byte aligned iff (c + mis) % E == 0
+    for (int k = 0; k < a.nwin; k++) {
+        const int64_t c0 = r0 + a.wst[k];
+        const int64_t g0 = c0 - ((c0 + mis) % E + E) % E;  // aligned group start <= c0
+        const int sh = (int)(c0 - g0);
+        const int ng = (a.wlen[k] + sh + E - 1) / E;
+        for (int i = threadIdx.x; i < ng; i += G_ROWS) {
+            const int64_t c = g0 + (int64_t)i * E;
+            F v[E];
+            if (c >= 0 && c + E <= a.ncols) {
+                const vec w = *(const vec *)(x + c);
+#pragma unroll
+                for (int e = 0; e < E; e++) v[e] = w[e];
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) v[e] = c + e >= 0 && c + e < a.ncols ? x[c + e] : (F)0;
+            }
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int p = i * E + e - sh;  // window position
+                if (p >= 0 && p < a.wlen[k]) s_xw[a.wbase[k] + p] = v[e];
+            }
+        }
+    }
+}
+
+// One wavefront per slice, G_WAVES slices per workgroup, XCD-aware order.
+// WIN: the workgroup first stages its x window in LDS (a.wsize entries,
+// 16-byte loads), so the rows' x reads, each entry shared by up to L rows
 // of neighbouring slices, are LDS reads instead of L2 requests.
 template <int U, int ENC, bool WIN>
-__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
+__global__ __launch_bounds__(G_ROWS) void k_spmv_pattern(vnec_pattern_args a, int64_t grid)
 {
     typedef typename enc_real<ENC>::type F;
     extern __shared__ double s_dyn[];
     F *s_xw = (F *)s_dyn;
     const F *__restrict__ x = pattern_x(a, F());
     const int64_t b = xcd_tile(blockIdx.x, grid);
-    const int64_t r0 = b * VNEC_BLOCK;  // first row of the workgroup
-    if (r0 >= a.nrows) return;           // whole workgroup
+    const int64_t r0 = b * G_ROWS;  // first row of the workgroup
+    if (r0 >= a.nrows) return;       // whole workgroup
     if (WIN) {
-        for (int k = 0; k < a.nwin; k++) {
-            const int64_t c0 = r0 + a.wst[k];
-            for (int i = threadIdx.x; i < a.wlen[k]; i += VNEC_BLOCK) {
-                const int64_t c = c0 + i;
-                s_xw[a.wbase[k] + i] = c >= 0 && c < a.ncols ? x[c] : (F)0;
-            }
-        }
+        stage_window(a, x, r0, s_xw);
         __syncthreads();
     }
-    const int64_t s = b * VNEC_NWAVES + wave_id();
+    const int64_t s = b * G_WAVES + wave_id();
     if (s >= a.nslices) return;
     const int t = (int)threadIdx.x;  // local row
     const int64_t row = r0 + t;
@@ -199,7 +235,7 @@ extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
     if (a->nslices <= 0) return 0;
     if (a->L < 1 || a->L > VNEC_PATTERN_MAX) return (int)hipErrorInvalidValue;
     if (a->encoding < 0 || a->encoding > VNEC_ENC_F32_K) return (int)hipErrorInvalidValue;
-    const int64_t blocks = (a->nslices + VNEC_NWAVES - 1) / VNEC_NWAVES;
+    const int64_t blocks = (a->nslices + G_WAVES - 1) / G_WAVES;
     const int64_t grid = xcd_grid(blocks);
     if (a->wsize < 0 || a->wsize > VNEC_PATTERN_WIN_MAX || a->nwin < 0 || a->nwin > a->L)
         return (int)hipErrorInvalidValue;
@@ -214,7 +250,7 @@ extern "C" int vnec_hip_spmv_pattern(const vnec_pattern_args *a, void *stream)
                : a->encoding == 2 ? k_spmv_pattern<8, 2, false>
                                   : k_spmv_pattern<8, VNEC_ENC_F32_K, false>);
     const size_t lds = win ? (a->encoding == VNEC_ENC_F32_K ? sizeof(float) : sizeof(double)) * (size_t)a->wsize : 0;
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(VNEC_BLOCK), lds, (hipStream_t)stream, *a, grid);
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(G_ROWS), lds, (hipStream_t)stream, *a, grid);
     return (int)hipGetLastError();
 }
 

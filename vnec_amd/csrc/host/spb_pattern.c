@@ -169,7 +169,7 @@ static SpB_Info irregular_submatrix(const spb_csr *a, const int32_t *irr, int64_
 
 void vnec_pattern_window(const int32_t *pat, int L, vnec_pattern_args *g)
 {
-    const int R = 256; /* rows per workgroup of the pattern kernel */
+    const int R = VNEC_PATTERN_ROWS; /* rows per workgroup of the pattern kernel */
     int nwin = 0, wsize = 0, win_of[VNEC_PATTERN_MAX];
     for (int l = 0; l < L; l++) { /* offsets ascend: merge overlapping or adjacent runs */
         const int64_t lo = pat[l], hi = (int64_t)pat[l] + R; /* [lo, hi) */
