@@ -251,9 +251,11 @@ def selection_label(spb, A, mode, info, k):
     if mode != "learned" or k > 1:
         return mode
     c = spb.candidates()[spb.select(A, "learned")]
-    if (c.variant, c.ipt) == (info.variant, info.ipt) and (c.grid == info.grid or c.grid >= 0 and info.grid >= 0):
+    same_grid = c.grid == info.grid or c.grid >= 0 and info.grid >= 0
+    if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
         return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
-    return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {info.variant}{info.ipt})"
+    form = f"{info.variant}{info.ipt} grid {info.grid}" + (" relabelled columns" if info.lanes == 1 else "")
+    return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {form})"
 
 
 def kernel_name(info, k):

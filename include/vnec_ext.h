@@ -114,7 +114,9 @@ typedef enum {
 typedef struct {
     int variant;  /* vnec_variant, or a vnec_select_mode                    */
     int ipt;      /* merge items per thread (tile = 256 * ipt items)       */
-    int lanes;    /* D block width / H hot x entries in LDS                */
+    int lanes;    /* D block width / H hot x entries in LDS; L and C:
+                     VNEC_LANES_RELABEL relabels the columns by descending
+                     use (x permuted before each SpMV), else 64          */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
     int grid;     /* 0: persistent, one workgroup per resident slot;
@@ -126,6 +128,11 @@ typedef struct {
                      workgroups sweep each XCD's tiles together (variants
                      L and C, one right-hand side, not VNEC_ENC_FP32)    */
 } vnec_tuning;
+
+/* vnec_tuning.lanes of an L or C plan whose columns are relabelled by
+ * descending use: the most used x entries share cache lines (the same sums
+ * in the same order as without it). */
+#define VNEC_LANES_RELABEL 1
 
 typedef struct vnec_plan_s *vnec_plan;
 

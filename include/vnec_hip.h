@@ -157,7 +157,16 @@ int vnec_hip_col_degree(const int32_t *col, int64_t nnz, int32_t *deg, void *str
 int vnec_hip_hot_order(const int32_t *deg, int64_t ncols, int32_t *order, void *stream);
 int vnec_hip_hot_encode(const int32_t *col, int64_t nnz, const int32_t *order, int64_t nhot,
                         int64_t ncols, int32_t *slot, int32_t *hcol, void *stream);
-/* xhot[h] = x[order[h]], h < nhot (before each variant-H SpMV). */
+/* order (ncols): the first k columns of order_full, then the others in
+ * ascending column order (rank, flags: ncols scratch each). */
+int vnec_hip_pack_order(const int32_t *order_full, int64_t ncols, int64_t k, int32_t *rank, int32_t *flags,
+                        int32_t *order, void *stream);
+/* Relabelled columns: rank[order[h]] = h (rank: ncols scratch) and
+ * rcol[j] = rank[col[j]] (nnz), order a permutation of the columns. */
+int vnec_hip_relabel_cols(const int32_t *col, int64_t nnz, const int32_t *order, int64_t ncols,
+                          int32_t *rank, int32_t *rcol, void *stream);
+/* xhot[h] = x[order[h]], h < nhot (before each variant-H SpMV, and with
+ * nhot = ncols the x permutation of a relabelled plan). */
 int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_t nhot, double *xhot,
                         void *stream);
 /* Variant C tile storage: per tile t (nonzeros coords[t].y .. coords[t+1].y)

@@ -721,3 +721,28 @@ def test_reference_d_entry_runs_slice_layouts_on_dense_column_runs(fp32):
             assert O.check_rows(y.to_array(), O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
         spb.free_vnec("D", plan, fp32=fp32)
         A.free()
+
+
+@pytest.mark.parametrize("variant,ipt,grid", [("L", 7, 0), ("L", 11, -1), ("C", 9, 0), ("C", 11, -2),
+                                               ("C", 7, -1)])
+def test_relabelled_columns_give_the_plain_plans_bits(variant, ipt, grid):
+    """Columns relabelled by descending use (x permuted before the SpMV)
+    change which x entries share cache lines, not the sums: every row is
+    summed in its CSR order, so y equals the plain L plan's bit for bit and
+    passes the row gate; the plan refuses other variants."""
+    m = g.skew(14, 16, 2, 5000, seed=97)  # R-MAT with two long dense rows
+    A = _mat(m)
+    y_plain = _run_plan(A, m.x, "L", ipt, grid=grid)
+    y = _run_plan(A, m.x, variant, ipt, grid=grid, lanes=1)
+    assert np.array_equal(y, y_plain)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    assert O.check_rows(y, O.spmv_serial(m.ptr, m.col, m.val, m.x), ax, TOL)[0]
+    p = spb.Plan(A, variant, ipt, lanes=1, grid=grid)
+    bm, bs = p.format_bytes()
+    assert bs == bm + 20 * m.ncols  # the x permutation's bytes
+    p.free()
+    with pytest.raises(RuntimeError, match="relabelled columns"):
+        spb.Plan(A, "S", 3, lanes=1)
+    with pytest.raises(RuntimeError, match="relabelled columns"):
+        spb.Plan(A, "L", 7, lanes=1, encoding="hilo")
+    A.free()

@@ -229,6 +229,8 @@ SIGNATURES = {
     "vnec_hip_col_degree": (C.c_int, [vp, i64, vp, vp]),
     "vnec_hip_hot_order": (C.c_int, [vp, i64, vp, vp]),
     "vnec_hip_hot_encode": (C.c_int, [vp, i64, vp, i64, i64, vp, vp, vp]),
+    "vnec_hip_pack_order": (C.c_int, [vp, i64, i64, vp, vp, vp, vp]),
+    "vnec_hip_relabel_cols": (C.c_int, [vp, i64, vp, i64, vp, vp, vp]),
     "vnec_hip_gather_hot": (C.c_int, [vp, vp, i64, vp, vp]),
     "vnec_hip_sort_pairs_u64": (C.c_int, [vp, vp, i64, C.c_int, vp]),
     "vnec_hip_s_order": (C.c_int, [vp, i64, vp, i64, C.c_int, C.c_int, vp, vp, vp]),

@@ -118,3 +118,79 @@ extern "C" int vnec_hip_gather_hot(const double *x, const int32_t *order, int64_
                        (hipStream_t)stream, x, order, nhot, xhot);
     return (int)hipGetLastError();
 }
+
+// ------------------------------------------------- column relabelling
+// Relabelled columns: label h is column order[h] (order = columns by
+// descending use, vnec_hip_hot_order), so rank[order[h]] = h and each
+// nonzero's column c becomes rank[c]; x is permuted to xr[h] = x[order[h]]
+// before each SpMV (vnec_hip_gather_hot with nhot = ncols).
+__global__ void k_rank_of(const int32_t *__restrict__ order, int64_t ncols, int32_t *__restrict__ rank)
+{
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < ncols;
+         h += (int64_t)gridDim.x * blockDim.x)
+        rank[order[h]] = (int32_t)h;
+}
+
+__global__ void k_relabel(const int32_t *__restrict__ col, int64_t nnz, const int32_t *__restrict__ rank,
+                          int32_t *__restrict__ rcol)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x)
+        rcol[j] = rank[col[j]];
+}
+
+extern "C" int vnec_hip_relabel_cols(const int32_t *col, int64_t nnz, const int32_t *order, int64_t ncols,
+                                     int32_t *rank, int32_t *rcol, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (ncols > 0) {
+        hipLaunchKernelGGL(k_rank_of, dim3(grid_for(ncols)), dim3(256), 0, s, order, ncols, rank);
+        int rc = (int)hipGetLastError();
+        if (rc) return rc;
+    }
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_relabel, dim3(grid_for(nnz)), dim3(256), 0, s, col, nnz, rank, rcol);
+        return (int)hipGetLastError();
+    }
+    return 0;
+}
+
+// Hot-first column order: the k most used columns (the first k of
+// order_full, by descending use) keep their places, the others follow in
+// ascending column order, so permuting x reads k scattered entries and
+// the rest in address order.
+__global__ void k_cold_flags(const int32_t *__restrict__ rank, int64_t ncols, int64_t k, int32_t *__restrict__ flags)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncols;
+         c += (int64_t)gridDim.x * blockDim.x)
+        flags[c] = rank[c] >= k;
+}
+
+__global__ void k_pack_order(const int32_t *__restrict__ rank, const int32_t *__restrict__ pos, int64_t ncols,
+                             int64_t k, int32_t *__restrict__ order)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncols;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = rank[c];
+        order[r < k ? r : k + pos[c]] = (int32_t)c;
+    }
+}
+
+extern "C" int vnec_hip_pack_order(const int32_t *order_full, int64_t ncols, int64_t k, int32_t *rank,
+                                   int32_t *flags, int32_t *order, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (ncols <= 0) return 0;
+    if (k < 0 || k > ncols) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_rank_of, dim3(grid_for(ncols)), dim3(256), 0, s, order_full, ncols, rank);
+    int rc = (int)hipGetLastError();
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_cold_flags, dim3(grid_for(ncols)), dim3(256), 0, s, rank, ncols, k, flags);
+    rc = (int)hipGetLastError();
+    if (rc) return rc;
+    int64_t ncold = 0;
+    rc = vnec_hip_scan_excl_i32(flags, flags, ncols, &ncold, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_order, dim3(grid_for(ncols)), dim3(256), 0, s, rank, flags, ncols, k, order);
+    return (int)hipGetLastError();
+}

@@ -56,7 +56,7 @@ typedef struct spb_csr {
     int refined_ipt;    /* merge-path neighbourhood measured at this ipt
                            (0: not yet): the fastest of L / C under
                            both schedules                            */
-    int refined_variant, refined_grid;
+    int refined_variant, refined_grid, refined_lanes;
     int refined_slice_ipt; /* slice-family pick measured at this ipt (0: not
                               yet): the fastest of S at it and variant G  */
     int refined_slice_variant, refined_slice_ipt_out;
@@ -141,6 +141,12 @@ struct vnec_plan_s {
     int cs_shift;       /* > 0: cs_col = column | offset << cs_shift  */
     int32_t *cs_base;   /* ntiles, or NULL: with cs_shift > 0, the packed
                            column is relative to its tile's smallest one  */
+    /* relabelled columns (variants L and C, lanes = VNEC_LANES_RELABEL):
+       label h is column cperm[h] (columns by descending use); rcol holds the
+       relabelled columns, xperm the permuted x of the current SpMV */
+    int32_t *cperm;     /* ncols */
+    int32_t *rcol;      /* nnz   */
+    double *xperm;      /* ncols */
     /* borrowed from the matrix mirror */
     const int32_t *ptr, *col;
     const double *val;
@@ -208,8 +214,9 @@ SpB_Info vnec_select_tuning(spb_csr *a, int mode, int nrhs, vnec_tuning *t);
 SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t);
 /* Times the FP64 merge-path plan t (variant L or C) in its four forms —
  * CSR-ordered (L) or column-sorted (C) tiles, persistent (grid 0) or
- * per-tile carries (grid -1) — at t's granularity, and sets t's variant
- * and grid to the fastest.  Cached on the matrix. */
+ * per-tile carries (grid -1) — at t's granularity, and, when x takes at
+ * least 2 MiB, the same four with columns relabelled by descending use;
+ * sets t's variant, grid and lanes to the fastest.  Cached on the matrix. */
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
 /* The structural rules of the learned and rule modes: most rows fit one
  * column-offset pattern (variant G applies), and tiles reuse columns

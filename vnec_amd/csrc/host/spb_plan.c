@@ -34,6 +34,9 @@ static void plan_release(struct vnec_plan_s *p)
     vnec_free_dev(p->g_lo);
     vnec_free_dev(p->xt);
     vnec_free_dev(p->yt);
+    vnec_free_dev(p->cperm);
+    vnec_free_dev(p->rcol);
+    vnec_free_dev(p->xperm);
     vnec_free_dev(p->cs_val);
     vnec_free_dev(p->cs_col);
     vnec_free_dev(p->cs_pos);
@@ -346,6 +349,43 @@ static SpB_Info build_hot(struct vnec_plan_s *p, const spb_csr *a)
     return rc;
 }
 
+/* Hot columns a relabelled plan packs first: as many x entries as one
+ * XCD's 4 MiB L2 holds. */
+#define RELABEL_HOT ((int64_t)1 << 19)
+
+/* Relabelled columns: the RELABEL_HOT most used columns first, by
+ * descending use (ties by column), then the others in ascending column
+ * order (so the x permutation reads them in address order); the plan's own
+ * copy of the columns in the new labels, and the permuted-x scratch.  Rows
+ * and the order of each row's entries are unchanged, so the sums are L's,
+ * bit for bit. */
+static SpB_Info build_relabel(struct vnec_plan_s *p, const spb_csr *a)
+{
+    hipStream_t s = vnec_stream();
+    const int64_t nc = p->ncols, nnz = p->nnz;
+    int32_t *deg = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), NULL);
+    int32_t *full = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), NULL);
+    int32_t *rank = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), NULL);
+    p->cperm = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nc ? nc : 1), &p->dev_bytes);
+    p->rcol = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)(nnz ? nnz : 1), &p->dev_bytes);
+    p->xperm = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(nc ? nc : 1), &p->dev_bytes);
+    SpB_Info rc = (deg && full && rank && p->cperm && p->rcol && p->xperm) ? SpB_SUCCESS : SpB_FAILURE;
+    if (rc == SpB_SUCCESS && nc) rc = vnec_hip_ok(hipMemsetAsync(deg, 0, sizeof(int32_t) * (size_t)nc, s), "memset");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_col_degree(a->d_col, nnz, deg, s), "column uses");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_hot_order(deg, nc, full, s), "column order");
+    if (rc == SpB_SUCCESS) /* deg is free scratch from here */
+        rc = vnec_hip_ok(vnec_hip_pack_order(full, nc, nc < RELABEL_HOT ? nc : RELABEL_HOT, rank, deg, p->cperm, s),
+                         "hot-first order");
+    if (rc == SpB_SUCCESS)
+        rc = vnec_hip_ok(vnec_hip_relabel_cols(a->d_col, nnz, p->cperm, nc, rank, p->rcol, s), "relabel");
+    if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+    vnec_free_dev(deg);
+    vnec_free_dev(full);
+    vnec_free_dev(rank);
+    if (rc == SpB_SUCCESS) p->col = p->rcol; /* variant L reads them; C sorts them */
+    return rc;
+}
+
 /* Selection rule for variant G: at least 90 % of the rows fit the
  * dominant pattern, its slots hold at least 80 % nonzeros (little zero
  * padding), and the matrix is large enough (2^20 nonzeros) for the bytes
@@ -476,6 +516,11 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
         if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
     }
+    if (tune.lanes == VNEC_LANES_RELABEL &&
+        !((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.encoding == VNEC_ENC_FP64 &&
+          tune.nrhs == 1))
+        return vnec_fail("relabelled columns (lanes %d): variants L and C, FP64, one right-hand side",
+                         VNEC_LANES_RELABEL);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
@@ -568,6 +613,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (rc == SpB_SUCCESS)
             rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
                              "partition");
+        if (rc == SpB_SUCCESS && p->tune.lanes == VNEC_LANES_RELABEL) rc = build_relabel(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_D) rc = build_d(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_H) rc = build_hot(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_C) rc = build_csort(p, a);
@@ -650,6 +696,8 @@ SpB_Info vnec_plan_bytes_ext(vnec_plan plan, int64_t *main_bytes, int64_t *spmv_
         vnec_plan_bytes_ext(plan->g_subplan, NULL, &sub);
         tot += sub - xb * plan->g_subplan->ncols + (2 * xb + 4) * n;
     }
+    if (plan->cperm && plan->ntiles > 0) /* the x permutation: order and x read, permuted x written */
+        tot += (4 + 8 + 8) * plan->ncols;
     if (main_bytes) *main_bytes = mb;
     if (spmv_bytes) *spmv_bytes = tot;
     return SpB_SUCCESS;
@@ -682,7 +730,7 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
         p->cs_base = (int32_t *)vnec_malloc_dev(sizeof(int32_t) * (size_t)p->ntiles, &p->dev_bytes);
         int32_t span = 0;
         if (!p->cs_base ||
-            vnec_hip_ok(vnec_hip_tile_colspan(p->coords, p->ntiles, a->d_col, p->cs_base, &span, vnec_stream()),
+            vnec_hip_ok(vnec_hip_tile_colspan(p->coords, p->ntiles, p->col, p->cs_base, &span, vnec_stream()),
                         "tile column span"))
             return SpB_FAILURE;
         int rb = 1;
@@ -703,7 +751,7 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
     SpB_Info rc = (p->cs_val && p->cs_col && (p->cs_shift || p->cs_pos) && keys && idx) ? SpB_SUCCESS
                                                                                           : SpB_FAILURE;
     if (rc == SpB_SUCCESS)
-        rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, a->d_col, a->d_val, p->nnz, p->ncols, p->cs_shift,
+        rc = vnec_hip_ok(vnec_hip_csort(p->coords, p->ntiles, p->col, a->d_val, p->nnz, p->ncols, p->cs_shift,
                                         p->cs_base, keys, idx, p->cs_val, p->cs_col, p->cs_pos, vnec_stream()),
                          "column sort");
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(vnec_stream()), "sync");
@@ -899,6 +947,11 @@ SpB_Info vnec_plan_run(struct vnec_plan_s *p, const void *x_dev, void *y_dev)
         if (p->tune.encoding == VNEC_ENC_FP32)
             return vnec_hip_ok(vnec_hip_zero_f32((float *)y_dev, p->nrows, s), "zero y");
         return vnec_hip_ok(vnec_hip_zero_f64((double *)y_dev, p->nrows * p->tune.nrhs, s), "zero y");
+    }
+    if (p->cperm) { /* x in the relabelled order */
+        if (vnec_hip_ok(vnec_hip_gather_hot((const double *)x_dev, p->cperm, p->ncols, p->xperm, s), "x permute"))
+            return SpB_FAILURE;
+        x_dev = p->xperm;
     }
     vnec_merge_args m;
     fill_args(p, x_dev, y_dev, &m);
@@ -1115,7 +1168,9 @@ SpB_Info vnec_time_kernels_ext(vnec_plan plan, const void *x_dev, void *y_dev, i
     for (int i = 0; i < 3; i++)
         if (vnec_hip_ok(hipEventCreate(&ev[i]), "event")) return SpB_FAILURE;
     vnec_merge_args m;
-    fill_args(plan, x_dev, y_dev, &m);
+    /* a relabelled plan's kernels read the permuted x, which the SpMV just
+     * before the timed launches leaves in the plan's scratch */
+    fill_args(plan, plan->cperm ? plan->xperm : x_dev, y_dev, &m);
     SpB_Info rc = vnec_plan_run(plan, x_dev, y_dev);
     float a = 0.f, b = 0.f;
     if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipEventRecord(ev[0], s), "record");

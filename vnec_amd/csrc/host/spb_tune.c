@@ -724,11 +724,17 @@ SpB_Info vnec_refine_slice_plan(spb_csr *a, vnec_tuning *t)
     return SpB_SUCCESS;
 }
 
+/* Relabelled columns (descending use) are measured as further merge-path
+ * forms when x takes at least this many bytes: below it x stays in the
+ * caches closest to the kernels whatever its order. */
+#define RELABEL_MIN_X_BYTES ((int64_t)1 << 21)
+
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
 {
     if (a->refined_ipt == t->ipt) {
         t->variant = a->refined_variant;
         t->grid = a->refined_grid;
+        t->lanes = a->refined_lanes;
         return SpB_SUCCESS;
     }
     int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col;
@@ -740,28 +746,21 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     if (rc == SpB_SUCCESS && nc)
         rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
     static const int VARIANTS[2] = {VNEC_VAR_L, VNEC_VAR_C}, GRIDS[2] = {0, -1};
+    const int nforms = 8 * nc >= RELABEL_MIN_X_BYTES ? 8 : 4;
     double best = INFINITY;
-    int bv = t->variant, bg = t->grid;
-    for (int k = 0; rc == SpB_SUCCESS && k < 4; k++) {
+    int bv = t->variant, bg = t->grid, bl = 64;
+    for (int k = 0; rc == SpB_SUCCESS && k < nforms; k++) {
         vnec_tuning u = *t;
-        u.variant = VARIANTS[k / 2];
+        u.variant = VARIANTS[(k / 2) % 2];
         u.grid = GRIDS[k % 2];
-        vnec_plan p = NULL;
-        if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &u) != SpB_SUCCESS) {
-            vnec_clear_error(); /* not a contender */
-            continue;
-        }
+        u.lanes = k >= 4 ? VNEC_LANES_RELABEL : 64;
         double tb = INFINITY;
-        for (int r = 0; rc == SpB_SUCCESS && r < 3; r++) {
-            double ms = 0.0;
-            rc = vnec_time_dev_ext(p, dx, dy, 20, 1, 0, &ms);
-            if (rc == SpB_SUCCESS && ms < tb) tb = ms;
-        }
-        vnec_plan_free_ext(&p);
+        rc = time_tuning(a, &u, dx, dy, 20, &tb);
         if (tb < best) {
             best = tb;
             bv = u.variant;
             bg = u.grid;
+            bl = u.lanes;
         }
     }
     free(hx);
@@ -771,6 +770,7 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     a->refined_ipt = t->ipt;
     a->refined_variant = t->variant = bv;
     a->refined_grid = t->grid = bg;
+    a->refined_lanes = t->lanes = bl;
     return SpB_SUCCESS;
 }
 
