@@ -254,7 +254,9 @@ def selection_label(spb, A, mode, info, k):
     same_grid = c.grid == info.grid or c.grid >= 0 and info.grid >= 0
     if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
         return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
-    form = f"{info.variant}{info.ipt} grid {info.grid}" + (" relabelled columns" if info.lanes == 1 else "")
+    flags = info.lanes if info.variant in ("L", "C") and info.lanes != 64 else 0
+    form = (f"{info.variant}{info.ipt} grid {info.grid}" + (" 1024-thread tiles" if flags & 1024 else "")
+            + (" relabelled columns" if flags & 1 else ""))
     return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {form})"
 
 
@@ -274,7 +276,8 @@ def stored_traffic(name, info):
         return None
     with open(tf) as f:
         tj = json.load(f)
-    if (tj.get("variant"), tj.get("ipt"), tj.get("grid")) == (info.variant, info.ipt, info.grid):
+    if (tj.get("variant"), tj.get("ipt"), tj.get("grid"), tj.get("lanes", info.lanes)) == \
+            (info.variant, info.ipt, info.grid, info.lanes):
         return tj.get("hbm_bytes_per_launch")
     return None
 

@@ -115,8 +115,10 @@ typedef struct {
     int variant;  /* vnec_variant, or a vnec_select_mode                    */
     int ipt;      /* merge items per thread (tile = 256 * ipt items)       */
     int lanes;    /* D block width / H hot x entries in LDS; L and C:
-                     VNEC_LANES_RELABEL relabels the columns by descending
-                     use (x permuted before each SpMV), else 64          */
+                     64, or the flags VNEC_LANES_RELABEL (columns
+                     relabelled by descending use, x permuted before each
+                     SpMV) and, for C, VNEC_LANES_WIDE (1024-thread
+                     tiles), alone or or-ed together                     */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
     int grid;     /* 0: persistent, one workgroup per resident slot;
@@ -133,6 +135,11 @@ typedef struct {
  * descending use: the most used x entries share cache lines (the same sums
  * in the same order as without it). */
 #define VNEC_LANES_RELABEL 1
+/* vnec_tuning.lanes of a C plan whose tiles hold 1024 * ipt items (ipt 3,
+ * 5, 7 or 9; persistent launch, grid >= 0) instead of 256 * ipt: each
+ * tile's nonzeros are sorted by column over the larger window, so the x
+ * gathers of a wavefront share more cache lines. */
+#define VNEC_LANES_WIDE 1024
 
 typedef struct vnec_plan_s *vnec_plan;
 

@@ -108,7 +108,15 @@ typedef struct {
     const uint16_t *spos;     /* nnz                               */
     int cpos_shift;
     const int32_t *cbase;     /* ntiles, or NULL                   */
+    int block;                /* threads per tile of the persistent
+                                 kernel: 0 (256) or VNEC_WIDE_BLOCK */
 } vnec_merge_args;
+
+/* Threads per tile of the wide column-sorted tiles (variant C, FP64, one
+ * right-hand side, persistent launch; ipt 3, 5, 7 or 9): a tile holds
+ * VNEC_WIDE_BLOCK * ipt items and its nonzeros are sorted by column over
+ * that whole window. */
+#define VNEC_WIDE_BLOCK 1024
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
 /* Adds each tile's carry to y[row] in tile order (one add per row; rows
@@ -123,7 +131,10 @@ int vnec_hip_carry_fixup_f32(const int32_t *coords, const float *carry, int64_t 
  * partial row at its end in wg_row / wg_carry (nwg entries) for
  * vnec_hip_wg_fixup.  vnec_hip_stream_grid gives the resident workgroup
  * count (occupancy x CUs, rounded down to a multiple of 8). */
-int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg);
+int vnec_hip_stream_grid(int variant, int ipt, int encoding, int block, int *nwg);
+/* Whether the persistent kernel exists for (variant, ipt, encoding, block);
+ * block 0 is 256 threads per tile, VNEC_WIDE_BLOCK the wide tiles. */
+int vnec_hip_stream_supported(int variant, int ipt, int encoding, int block);
 /* The pipelined variant-E kernel holds at most this many window runs per
  * tile (one per thread); plans with a larger tile fall back to one
  * workgroup per tile. */

@@ -746,3 +746,40 @@ def test_relabelled_columns_give_the_plain_plans_bits(variant, ipt, grid):
     with pytest.raises(RuntimeError, match="relabelled columns"):
         spb.Plan(A, "L", 7, lanes=1, encoding="hilo")
     A.free()
+
+
+@pytest.mark.parametrize("name", ["rmat14", "skew14", "holes"])
+def test_wide_column_sorted_tiles(name):
+    """Variant C with 1024-thread tiles (1024 ipt items, each sorted by column
+    over the whole tile): every ipt passes the row gate, long rows spanning
+    many tiles and empty rows included; relabelled columns on top give the
+    same bits (rows are still summed in CSR order within the same tiles);
+    the plan refuses the combinations it has no kernel for."""
+    if name == "rmat14":
+        m = g.rmat(14, 16, seed=98)
+    elif name == "skew14":
+        m = g.skew(14, 16, 2, 12000, seed=99)  # two rows spanning several wide tiles
+    else:
+        m = g.uniform(30000, 7, seed=100)
+        lens = np.diff(m.ptr)
+        keep = np.repeat(np.arange(m.nrows), lens) % 3 != 0  # every third row empty
+        lens[::3] = 0
+        ptr = np.concatenate([[0], np.cumsum(lens)])
+        m = g.from_pattern(m.nrows, m.ncols, ptr, m.col[keep], seed=101)
+    A = _mat(m)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ref = O.spmv_serial(m.ptr, m.col, m.val, m.x)
+    for ipt in (3, 5, 7, 9):
+        y = _run_plan(A, m.x, "C", ipt, lanes=1024)
+        assert O.check_rows(y, ref, ax, TOL)[0], ipt
+        if ipt == 9:
+            assert np.array_equal(_run_plan(A, m.x, "C", ipt, lanes=1024 | 1), y)
+    p = spb.Plan(A, "C", 5, lanes=1024)
+    info = p.info()
+    assert info.ntiles == -(-(m.nrows + m.nnz) // (1024 * 5))
+    p.free()
+    for bad in [dict(variant="L", ipt=7, lanes=1024), dict(variant="C", ipt=11, lanes=1024),
+                dict(variant="C", ipt=9, lanes=1024, grid=-1), dict(variant="C", ipt=9, lanes=3)]:
+        with pytest.raises(RuntimeError, match="wide tiles|lanes"):
+            spb.Plan(A, bad.pop("variant"), bad.pop("ipt"), **bad)
+    A.free()

@@ -200,7 +200,8 @@ SIGNATURES = {
     "vnec_hip_carry_fixup": (C.c_int, [vp, vp, i64, vp, i64, C.c_int, vp]),
     "vnec_hip_carry_fixup_f32": (C.c_int, [vp, vp, i64, vp, i64, vp]),
     "vnec_hip_merge_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
-    "vnec_hip_stream_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "vnec_hip_stream_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]),
+    "vnec_hip_stream_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "vnec_hip_spmv_stream": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "vnec_hip_wg_fixup": (C.c_int, [vp, vp, C.c_int, vp, i64, vp]),
     "vnec_hip_strided_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, vp]),

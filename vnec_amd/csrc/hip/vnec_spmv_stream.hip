@@ -52,8 +52,8 @@ struct TileRegs {
     int cb;      // variant C: the tile's column base (packed relative columns)
 };
 
-// tid: the thread's index within its 256-thread tile pipeline.
-template <int IPT, bool WIN, int ENC, bool SORT = false>
+// tid: the thread's index within its BS-thread tile pipeline.
+template <int IPT, bool WIN, int ENC, bool SORT = false, int BS = VNEC_BLOCK>
 __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int64_t t, int2 c0, int2 c1,
                                           TileRegs<IPT, WIN, ENC> &R, int tid)
 {
@@ -61,7 +61,7 @@ __device__ __forceinline__ void load_tile(const vnec_merge_args &a, int64_t t, i
     if (SORT) R.cb = a.cbase ? a.cbase[t] : 0;
 #pragma unroll
     for (int u = 0; u < IPT; u++) {
-        const int k = u * VNEC_BLOCK + tid;
+        const int k = u * BS + tid;
         if (k < nz) {
             const int64_t j = (int64_t)c0.y + k;
             R.v[u] = ld_value<ENC>(a, j);
@@ -89,18 +89,21 @@ __device__ __forceinline__ double gather_x(const vnec_merge_args &a, const doubl
 // SORT (variant C): the tile's nonzeros arrive in column order, so each
 // wavefront's x gathers are ascending and share cache lines where columns
 // are dense; each product is written to its CSR position in LDS.
-template <int IPT, bool WIN, int ENC, int HOT = 0, bool SORT = false>
-__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, int nwg,
-                                                            int32_t *__restrict__ wg_row,
-                                                            double *__restrict__ wg_carry)
+// BS: threads per tile (256, or 1024 for the wide column-sorted tiles,
+// whose larger column-order window shares more cache lines per gather).
+template <int IPT, bool WIN, int ENC, int HOT = 0, bool SORT = false, int BS = VNEC_BLOCK>
+__global__ __launch_bounds__(BS) void k_spmv_stream(vnec_merge_args a, int nwg,
+                                                    int32_t *__restrict__ wg_row,
+                                                    double *__restrict__ wg_carry)
 {
-    constexpr int T = VNEC_BLOCK * IPT;
+    constexpr int T = BS * IPT;
+    constexpr int NW = BS / VNEC_WAVE;
     __shared__ double s_prod[T];
     __shared__ int s_rowend[T];
     __shared__ double s_win[WIN ? T : 1];
     __shared__ double s_hot[HOT > 0 ? HOT : 1];
-    __shared__ int s_f[VNEC_NWAVES + 1];
-    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ int s_f[NW + 1];
+    __shared__ double s_v[NW + 1];
     __shared__ double s_carry;
 
     const int tid = threadIdx.x;
@@ -139,9 +142,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         }
     }
     TileRegs<IPT, WIN, ENC> cur, nxt;
-    load_tile<IPT, WIN, ENC, SORT>(a, tb, A0, A1, cur, tid);
+    load_tile<IPT, WIN, ENC, SORT, BS>(a, tb, A0, A1, cur, tid);
     if (HOT > 0)
-        for (int i = tid; i < a.nhot; i += VNEC_BLOCK) s_hot[i] = a.xhot[i];
+        for (int i = tid; i < a.nhot; i += BS) s_hot[i] = a.xhot[i];
     __syncthreads();
 
     for (int64_t t = tb; t < te; t++) {
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
                 nec_c = a.nec[t + 2];
             }
         }
-        if (t + 1 < te) load_tile<IPT, WIN, ENC, SORT>(a, t + 1, A1, B1, nxt, tid);
+        if (t + 1 < te) load_tile<IPT, WIN, ENC, SORT, BS>(a, t + 1, A1, B1, nxt, tid);
 
         const int x0 = A0.x, y0 = A0.y;
         const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         // --- row ends and products into LDS
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
-            const int k = u * VNEC_BLOCK + tid;
+            const int k = u * BS + tid;
             if (k < nrow) s_rowend[k] = cur.re[u];
             if (k < nz) {
                 int c = cur.c[u], pos = k;
@@ -225,20 +228,20 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_stream(vnec_merge_args a, i
         __syncthreads();
         if (tid == 0) {
             double c = s_carry;
-            for (int k = 0; k < VNEC_NWAVES; k++) {
+            for (int k = 0; k < NW; k++) {
                 double tw = s_v[k];
                 int fw = s_f[k];
                 s_v[k] = c;
                 c = fw ? tw : c + tw;
             }
-            s_v[VNEC_NWAVES] = c;  // this tile's carry-out
+            s_v[NW] = c;  // this tile's carry-out
         }
         __syncthreads();
         const double cw = s_v[w];
         const double blk = f ? v : v + cw;
         const double cin = wave_shift_up1_f64(blk, cw);
         if (flag) a.y[head_row] = head_val + cin;
-        if (tid == 0) s_carry = s_v[VNEC_NWAVES];
+        if (tid == 0) s_carry = s_v[NW];
         __syncthreads();
 
         // --- rotate the pipeline
@@ -769,8 +772,17 @@ __global__ void k_wg_fixup(const int32_t *__restrict__ wg_row, const double *__r
 
 typedef void (*stream_kernel_t)(vnec_merge_args, int, int32_t *, double *);
 
-static stream_kernel_t pick_stream(int variant, int ipt, int enc)
+static stream_kernel_t pick_stream(int variant, int ipt, int enc, int block)
 {
+    if (block == VNEC_WIDE_BLOCK) {  // wide column-sorted tiles: FP64, C
+        if (variant != 6 || enc != 0) return nullptr;
+#define WCASE(I) \
+    if (ipt == I) return k_spmv_stream<I, false, 0, 0, true, VNEC_WIDE_BLOCK>;
+        WCASE(3) WCASE(5) WCASE(7) WCASE(9)
+#undef WCASE
+        return nullptr;
+    }
+    if (block != 0 && block != VNEC_BLOCK) return nullptr;
 #define SCASE(I, E)                                                  \
     if (ipt == I && enc == E) {                                      \
         if (variant == 0) return k_spmv_stream<I, false, E>;         \
@@ -784,13 +796,21 @@ static stream_kernel_t pick_stream(int variant, int ipt, int enc)
 #undef SCASE
 }
 
-extern "C" int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg)
+static int block_threads(int block) { return block == VNEC_WIDE_BLOCK ? VNEC_WIDE_BLOCK : VNEC_BLOCK; }
+
+extern "C" int vnec_hip_stream_supported(int variant, int ipt, int encoding, int block)
 {
-    stream_kernel_t k = (variant == 0 || variant == 1 || variant == 6) ? pick_stream(variant, ipt, encoding)
+    return (variant == 0 || variant == 1 || variant == 6) && pick_stream(variant, ipt, encoding, block) != nullptr;
+}
+
+extern "C" int vnec_hip_stream_grid(int variant, int ipt, int encoding, int block, int *nwg)
+{
+    stream_kernel_t k = (variant == 0 || variant == 1 || variant == 6) ? pick_stream(variant, ipt, encoding, block)
                                                                         : nullptr;
     if (!k) return (int)hipErrorInvalidValue;
     int per_cu = 0, dev = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, VNEC_BLOCK, 0);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, block_threads(block), 0);
     if (e != hipSuccess) return (int)e;
     e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
@@ -805,11 +825,11 @@ extern "C" int vnec_hip_stream_grid(int variant, int ipt, int encoding, int *nwg
 extern "C" int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *wg_row,
                                     double *wg_carry, void *stream)
 {
-    stream_kernel_t k = pick_stream(a->variant, a->ipt, a->encoding);
+    stream_kernel_t k = pick_stream(a->variant, a->ipt, a->encoding, a->block);
     if (!k || a->nrhs != 1 || nwg < VNEC_XCDS || nwg % VNEC_XCDS) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, nwg,
-                       wg_row, wg_carry);
+    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3((unsigned)block_threads(a->block)), 0, (hipStream_t)stream,
+                       *a, nwg, wg_row, wg_carry);
     return (int)hipGetLastError();
 }
 

@@ -11,6 +11,18 @@
 
 #define TILE_THREADS 256
 
+/* L and C plans: relabelled columns, and C's wide (1024-thread) tiles. */
+static int plan_relabelled(const vnec_tuning *t)
+{
+    return (t->variant == VNEC_VAR_L || t->variant == VNEC_VAR_C) && t->lanes != 64 &&
+           (t->lanes & VNEC_LANES_RELABEL);
+}
+static int plan_block(const vnec_tuning *t)
+{
+    return t->variant == VNEC_VAR_C && t->lanes != 64 && (t->lanes & VNEC_LANES_WIDE) ? VNEC_WIDE_BLOCK : 0;
+}
+static int plan_threads(const vnec_tuning *t) { return plan_block(t) ? VNEC_WIDE_BLOCK : TILE_THREADS; }
+
 static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
 /* Tiles per resident workgroup from which per-tile-carry plans use the
  * strided persistent launch (its pipelining pays once a workgroup sees
@@ -516,11 +528,19 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
         if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
     }
-    if (tune.lanes == VNEC_LANES_RELABEL &&
-        !((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.encoding == VNEC_ENC_FP64 &&
-          tune.nrhs == 1))
+    const int merge_form = tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C;
+    if (merge_form && tune.lanes != 64 && (tune.lanes & ~(VNEC_LANES_RELABEL | VNEC_LANES_WIDE)))
+        return vnec_fail("lanes %d: variants L and C take 64 or the flags VNEC_LANES_RELABEL | VNEC_LANES_WIDE",
+                         tune.lanes);
+    if ((tune.lanes == VNEC_LANES_RELABEL || (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_RELABEL))) &&
+        !(merge_form && tune.encoding == VNEC_ENC_FP64 && tune.nrhs == 1))
         return vnec_fail("relabelled columns (lanes %d): variants L and C, FP64, one right-hand side",
                          VNEC_LANES_RELABEL);
+    if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_WIDE) &&
+        !(tune.variant == VNEC_VAR_C && tune.grid >= 0 &&
+          vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)))
+        return vnec_fail("wide tiles (lanes %d): variant C, ipt 3, 5, 7 or 9, persistent launch (grid >= 0)",
+                         VNEC_LANES_WIDE);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
@@ -564,7 +584,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     p->val = a->d_val;
     p->A = a;
     int64_t M = p->nrows + p->nnz;
-    int64_t T = (int64_t)TILE_THREADS * tune.ipt;
+    int64_t T = (int64_t)plan_threads(&tune) * tune.ipt;
     p->ntiles = M > 0 ? (M + T - 1) / T : 0;
     hipStream_t s = vnec_stream();
     SpB_Info rc = SpB_SUCCESS;
@@ -582,7 +602,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             int nwg = tune.grid;
             if (nwg == 0 && tune.variant == VNEC_VAR_H) {
                 if (vnec_hip_ok(vnec_hip_hot_grid(tune.ipt, tune.lanes, &nwg), "occupancy")) rc = SpB_FAILURE;
-            } else if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding, &nwg),
+            } else if (nwg == 0 && vnec_hip_ok(vnec_hip_stream_grid(tune.variant, tune.ipt, tune.encoding,
+                                                                    plan_block(&tune), &nwg),
                                                "occupancy")) {
                 rc = SpB_FAILURE;
             }
@@ -613,7 +634,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (rc == SpB_SUCCESS)
             rc = vnec_hip_ok(vnec_hip_partition(a->d_ptr, p->nrows, p->nnz, p->ntiles, 0, p->coords, s),
                              "partition");
-        if (rc == SpB_SUCCESS && p->tune.lanes == VNEC_LANES_RELABEL) rc = build_relabel(p, a);
+        if (rc == SpB_SUCCESS && plan_relabelled(&p->tune)) rc = build_relabel(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_D) rc = build_d(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_H) rc = build_hot(p, a);
         if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_C) rc = build_csort(p, a);
@@ -721,7 +742,7 @@ static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a)
      * bits (12 bytes per nonzero, as CSR) */
     int cb = 1, ob = 1;
     while (((int64_t)1 << cb) < p->ncols) cb++;
-    while ((1 << ob) < TILE_THREADS * p->tune.ipt) ob++;
+    while ((1 << ob) < plan_threads(&p->tune) * p->tune.ipt) ob++;
     p->cs_shift = cb + ob <= 32 ? cb : 0;
     if (!p->cs_shift && p->nnz > 0) {
         /* too many columns to pack: columns relative to each tile's smallest
@@ -797,6 +818,7 @@ static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_de
         m.spos = p->cs_pos;
         m.cpos_shift = p->cs_shift;
         m.cbase = p->cs_base;
+        m.block = plan_block(&p->tune);
     }
     if (p->tune.variant == VNEC_VAR_H) {
         m.col = p->hcol;
