@@ -312,14 +312,22 @@ def test_d_layout_and_ird_bit_exact_with_reference(name):
 @pytest.mark.parametrize("name", golden_names())
 def test_s_order_bit_exact_with_reference(name):
     """GPU S task order and reordered matrix at the reference's P = 36,
-    L = 8, cut 32 against the reference build's own structures (fixtures
-    whose reference S build is well defined carry them)."""
+    L = 8, cut 32 against the reference build's own structures where the
+    fixture carries them; on the fixtures where the reference's length
+    buckets overflow (equal-length rows, spmv_vnec_S_for_mxv.c:442-524, so
+    its structures are undefined and were not exported) against the
+    oracle's restatement at the same parameters instead."""
     d = load_golden(name)
-    if "s_tasks" not in d:
-        pytest.skip("no reference S structures in this fixture")
     n = int(d["nrows"])
     A = spb.Matrix.from_csr(n, int(d["ncols"]), d["ptr"], d["col"], d["val"])
     r = spb.s_order(A, 36, 8, 32)
+    if "s_tasks" not in d:
+        o = O.s_build(d["ptr"], d["col"], d["val"], 36, 8, 32)
+        assert np.array_equal(r["tasks"], o.tasks) and np.array_equal(r["spvv_len"], o.spvv_len)
+        assert np.array_equal(r["row_begin"], o.row_begin) and np.array_equal(r["row_end"], o.row_end)
+        assert np.array_equal(r["s_col"], o.s_col) and np.array_equal(r["s_val"], o.s_val)
+        assert np.array_equal(r["task_start"], o.task_start) and np.array_equal(r["task_end"], o.task_end)
+        return
     assert np.array_equal(r["tasks"], d["s_tasks"])
     assert np.array_equal(r["spvv_len"], d["s_spvv_len"])
     assert np.array_equal(r["row_begin"], d["s_row_begin"])

@@ -231,3 +231,39 @@ def test_g_encoded_sub_plan_for_every_selector_pick(enc):
     ax, yl = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     assert O.check_rows(y.to_array(), yl, ax, TOL if enc == "hilo" else 2.0 ** -23)[0]
     p.free()
+
+
+@pytest.mark.parametrize("name,m", [CASES[0], CASES[2]], ids=[CASES[0][0], CASES[2][0]])
+def test_g_x_at_any_alignment(name, m):
+    """The LDS window is staged with 16-byte loads aligned to x's own
+    address: x and y at every 4- or 8-byte offset from a 16-byte boundary
+    (device pointers from the caller) give the aligned call's bits, FP64 and
+    the FP32 API alike, and meet the gate."""
+    L = spb.L.load()
+    for dtype, offs in ((np.float64, (0, 8)), (np.float32, (0, 4, 8, 12))):
+        val, x = m.val.astype(dtype), m.x.astype(dtype)
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, val, dtype=dtype)
+        p = spb.Plan(A, "G")
+        es = np.dtype(dtype).itemsize
+        dx = spb.DeviceArray(es * m.ncols + 16)
+        dy = spb.DeviceArray(es * m.nrows + 16)
+        outs = []
+        for off in offs:
+            print(f"case {name} variant G {np.dtype(dtype).name} offset {off}", flush=True)
+            spb.L.check(L.vnec_memcpy_h2d_ext(dx.ptr + off, x.ctypes.data, x.nbytes), "h2d")
+            p.spmv_dev(dx.ptr + off, dy.ptr + off)
+            y = np.empty(m.nrows, dtype)
+            spb.L.check(L.vnec_memcpy_d2h_ext(y.ctypes.data, dy.ptr + off, y.nbytes), "d2h")
+            outs.append(y)
+        for y in outs[1:]:
+            assert np.array_equal(y, outs[0])
+        v64, x64 = val.astype(np.float64), x.astype(np.float64)
+        ax, _ = O.spmv_abs(m.ptr, m.col, v64, x64)
+        if dtype == np.float64:
+            assert O.check_rows(outs[0], O.spmv_serial(m.ptr, m.col, v64, x64), ax, TOL)[0]
+        else:
+            assert O.check_rows_f32(outs[0], O.spmv_serial_f32(m.ptr, m.col, val, x), m.ptr, ax)[0]
+        dx.free()
+        dy.free()
+        p.free()
+        A.free()
