@@ -168,8 +168,9 @@ SpB_Info vnec_spmv_dev_ext(vnec_plan plan, const void *x_dev, void *y_dev);
  * for this (x_dev, y_dev) pair on the first call and replays it after
  * (bit-identical to vnec_spmv_dev_ext; x_dev's contents may change between
  * calls, its address may not).  A plan keeps graphs for 4 pairs; a fifth
- * recaptures over the least recently used.  vnec_spmv_ext and SpB_mxv use
- * it when x and y are device-resident (SpB_Vector_device_resident_ext). */
+ * recaptures over the least recently used.  One replay costs more host and
+ * launch time than the plan's one to three direct kernel launches, so
+ * vnec_spmv_dev_ext, vnec_spmv_ext and SpB_mxv launch directly. */
 SpB_Info vnec_spmv_graph_ext(vnec_plan plan, const void *x_dev, void *y_dev);
 /* Same with the right-hand sides column-major: X holds nrhs vectors of
  * ncols doubles one after another, Y nrhs vectors of nrows.  The plan

@@ -1,5 +1,5 @@
-"""Captured-graph SpMV (vnec_spmv_graph_ext, and vnec_spmv_ext / SpB_mxv on
-device-resident vectors): every replay gives the bits of the direct launch
+"""Captured-graph SpMV (vnec_spmv_graph_ext) and SpB_mxv on device-resident
+vectors: every replay gives the bits of the direct launch
 (vnec_spmv_dev_ext), with eager work and synchronisations between the
 replays, for every plan shape; x's contents may change between replays;
 a plan keeps graphs for four pointer pairs and recaptures a fifth."""
@@ -14,22 +14,22 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 
-SHAPES = [("L", 7, 0, 1, "fp64"), ("L", 11, -1, 1, "fp64"), ("C", 9, 0, 1, "fp64"),
-          ("C", 7, -2, 1, "fp64"), ("E", 5, 0, 1, "fp64"), ("S", 3, -1, 1, "fp64"),
-          ("S", 3, -1, 1, "hilo"), ("D", 7, -1, 1, "fp64"), ("H", 5, 0, 1, "fp64"),
-          ("L", 5, -1, 8, "fp64"), ("L", 7, 0, 1, "hi")]
+SHAPES = [("L", 7, 0, 1, "fp64", 64), ("L", 11, -1, 1, "fp64", 64), ("C", 9, 0, 1, "fp64", 64),
+          ("C", 7, -2, 1, "fp64", 64), ("E", 5, 0, 1, "fp64", 64), ("S", 3, -1, 1, "fp64", 64),
+          ("S", 3, -1, 1, "hilo", 64), ("D", 7, -1, 1, "fp64", 8), ("H", 5, 0, 1, "fp64", 4096),
+          ("L", 5, -1, 8, "fp64", 64), ("L", 7, 0, 1, "hi", 64),
+          ("C", 9, -2, 1, "fp64", 1), ("C", 7, 0, 1, "fp64", 1024)]  # relabelled; wide tiles
 
 
 def _sync():
     spb.L.check(spb.L.load().vnec_sync_ext(), "sync")
 
 
-@pytest.mark.parametrize("variant,ipt,grid,k,enc", SHAPES,
-                         ids=[f"{v}{i}_g{gr}_k{k}_{e}" for v, i, gr, k, e in SHAPES])
-def test_graph_replay_is_bit_identical_to_direct_launch(variant, ipt, grid, k, enc):
+@pytest.mark.parametrize("variant,ipt,grid,k,enc,lanes", SHAPES,
+                         ids=[f"{v}{i}_g{gr}_k{k}_{e}_l{ln}" for v, i, gr, k, e, ln in SHAPES])
+def test_graph_replay_is_bit_identical_to_direct_launch(variant, ipt, grid, k, enc, lanes):
     m = g.rmat(14, 16, seed=91)
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
-    lanes = 8 if variant == "D" else 4096 if variant == "H" else 64
     p = spb.Plan(A, variant, ipt, lanes=lanes, encoding=enc, nrhs=k, grid=grid)
     rng = np.random.default_rng(7)
     X = rng.uniform(-1, 1, m.ncols * k)
@@ -95,8 +95,8 @@ def test_graph_pattern_slices_and_pointer_pairs():
 
 def test_mxv_on_resident_vectors_replays_the_same_bits():
     """SpB_mxv with device-resident x and y (the reference harness's repeated
-    SpMV, test_spmv_vnec.cpp:205-214) replays a captured launch: the same
-    bits call after call, and new x values are picked up."""
+    SpMV, test_spmv_vnec.cpp:205-214): the same bits call after call, the
+    bits of host vectors, and new x values are picked up."""
     m = g.rmat(13, 16, seed=93)
     A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
     x = spb.Vector.from_array(m.x)

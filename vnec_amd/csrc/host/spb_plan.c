@@ -1066,11 +1066,10 @@ SpB_Info vnec_spmv_ext(vnec_plan plan, SpB_Vector y, const SpB_Vector x)
                          xv->base.n, (long)(plan->ncols * k), yv->base.n, (long)(plan->nrows * k));
     if (spb_vector_to_device(xv) != SpB_SUCCESS) return SpB_FAILURE;
     if (spb_vector_device_alloc(yv) != SpB_SUCCESS) return SpB_FAILURE;
-    /* device-resident vectors keep their device arrays between calls, so a
-     * repeated SpMV replays one captured launch sequence */
-    if ((xv->resident && yv->resident ? plan_run_graph(plan, xv->d_values, yv->d_values)
-                                      : vnec_plan_run(plan, xv->d_values, yv->d_values)) != SpB_SUCCESS)
-        return SpB_FAILURE;
+    /* direct launches: replaying a captured graph per call measured slower
+     * (one hipGraphLaunch costs more than the plan's one to three kernel
+     * launches), so vnec_spmv_graph_ext stays an explicit choice */
+    if (vnec_plan_run(plan, xv->d_values, yv->d_values) != SpB_SUCCESS) return SpB_FAILURE;
     yv->dev_stale = 0;
     yv->host_stale = 1;
     if (!yv->resident) return spb_vector_from_device(yv);
