@@ -94,8 +94,9 @@ typedef enum {
     VNEC_ENC_HILO = 1, /* FP32 hi + FP32 lo, a = hi + lo, FP64 accumulate */
     VNEC_ENC_HI = 2,   /* FP32 hi only, FP64 accumulate                 */
     VNEC_ENC_FP32 = 3, /* the FP32 API: FP32 values, x, y and sums (FP32
-                          matrices only; variants L (one workgroup per
-                          tile), S, D and G, one right-hand side)       */
+                          matrices only; variants L (per-tile carries:
+                          grid -1 or -2), S, D and G, one right-hand
+                          side)                                         */
     VNEC_ENC_N
 } vnec_encoding;
 
@@ -128,7 +129,7 @@ typedef struct {
                      resident workgroup, the form of -2); -2: the same
                      per-tile carries from a persistent launch whose
                      workgroups sweep each XCD's tiles together (variants
-                     L and C, one right-hand side, not VNEC_ENC_FP32)    */
+                     L and C, one right-hand side)                       */
 } vnec_tuning;
 
 /* vnec_tuning.lanes of an L or C plan whose columns are relabelled by

@@ -181,3 +181,49 @@ def test_fp32_entry_points_refuse_mixed_types():
         spb.Plan(A64, "L", 7, encoding="fp32")
     with pytest.raises(RuntimeError, match="variants L, S, D and G"):
         spb.Plan(A32, "E", 7)
+
+
+@pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("ipt", [3, 7, 11])
+def test_fp32_strided_launch_is_bit_identical_to_one_workgroup_per_tile(name, m, ipt):
+    """The FP32 API's merge-path plan with per-tile carries from the strided
+    persistent launch (grid -2): the tiles, carries and fix-up of grid -1, so
+    the same bits, within the FP32 gate."""
+    ptr, col, val, x = _f32(m)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
+    out = []
+    for grid in (-1, -2):
+        print(f"case {name} FP32 L{ipt} grid {grid}", flush=True)
+        p = spb.Plan(A, "L", ipt, grid=grid)
+        assert p.info().grid == grid and p.info().encoding == "fp32"
+        yv = spb.Vector(m.nrows, np.float32)
+        p.spmv(yv, spb.Vector.from_array(x, np.float32))
+        out.append(yv.to_array())
+        p.free()
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
+    ok, worst = _gate(out[1], ptr, col, val, x)
+    assert ok, worst
+
+
+def test_fp32_selection_takes_pattern_slices_where_the_rule_holds():
+    """FP32 matrices in a selection mode (what SpB_mxv uses): pattern slices
+    for a stencil of more than 2^20 nonzeros, 64-row slices or merge-path
+    tiles elsewhere; results within the FP32 gate."""
+    for m, want in ((g.stencil27(40, seed=31), "G"), (g.rmat(14, 16, seed=32), None)):
+        ptr, col, val, x = _f32(m)
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
+        p = spb.Plan(A, "learned")
+        info = p.info()
+        assert info.encoding == "fp32"
+        if want:
+            assert info.variant == want
+        else:
+            assert info.variant in ("S", "L")
+        yv = spb.Vector(m.nrows, np.float32)
+        p.spmv(yv, spb.Vector.from_array(x, np.float32))
+        ok, worst = _gate(yv.to_array(), ptr, col, val, x)
+        assert ok, (info, worst)
+        y2 = spb.Vector(m.nrows, np.float32)
+        spb.mxv(y2, A, spb.Vector.from_array(x, np.float32))
+        assert np.array_equal(y2.to_array().view(np.uint32), yv.to_array().view(np.uint32))
+        p.free()

@@ -23,6 +23,7 @@
 template <int ENC> struct RawVal { double d; };
 template <> struct RawVal<1> { float h, l; };
 template <> struct RawVal<2> { float h; };
+template <> struct RawVal<3> { float h; };  // the FP32 API: FP32 values, x and sums
 
 template <int ENC>
 __device__ __forceinline__ RawVal<ENC> ld_value(const vnec_merge_args &a, int64_t j);
@@ -38,9 +39,23 @@ template <> __device__ __forceinline__ RawVal<2> ld_value<2>(const vnec_merge_ar
 {
     return {__builtin_nontemporal_load(a.hi + j)};
 }
+template <> __device__ __forceinline__ RawVal<3> ld_value<3>(const vnec_merge_args &a, int64_t j)
+{
+    return {__builtin_nontemporal_load(a.hi + j)};
+}
 __device__ __forceinline__ double widen(RawVal<0> v) { return v.d; }
 __device__ __forceinline__ double widen(RawVal<1> v) { return (double)v.h + (double)v.l; }
 __device__ __forceinline__ double widen(RawVal<2> v) { return (double)v.h; }
+__device__ __forceinline__ float widen(RawVal<3> v) { return v.h; }
+
+// x, y and the per-tile carries in the kernel's value type (FP32 for the
+// FP32 API, else FP64).
+__device__ __forceinline__ const double *x_in(const vnec_merge_args &a, double) { return a.x; }
+__device__ __forceinline__ const float *x_in(const vnec_merge_args &a, float) { return a.x32; }
+__device__ __forceinline__ double *y_in(const vnec_merge_args &a, double) { return a.y; }
+__device__ __forceinline__ float *y_in(const vnec_merge_args &a, float) { return a.y32; }
+__device__ __forceinline__ double *carry_in(const vnec_merge_args &a, double) { return a.carry; }
+__device__ __forceinline__ float *carry_in(const vnec_merge_args &a, float) { return a.carry32; }
 
 // Registers holding one tile's streamed data (striped: item u * 256 + tid).
 template <int IPT, bool WIN, int ENC = 0>
@@ -274,11 +289,14 @@ __global__ __launch_bounds__(BS) void k_spmv_stream(vnec_merge_args a, int nwg,
 template <int IPT, int ENC, bool SORT>
 __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, int nwg)
 {
+    typedef typename enc_real<ENC>::type F;  // FP32 for the FP32 API's arithmetic
     constexpr int T = VNEC_BLOCK * IPT;
-    __shared__ double s_prod[T];
+    __shared__ F s_prod[T];
     __shared__ int s_rowend[T];
     __shared__ int s_f[VNEC_NWAVES + 1];
-    __shared__ double s_v[VNEC_NWAVES + 1];
+    __shared__ F s_v[VNEC_NWAVES + 1];
+    const F *__restrict__ x = x_in(a, F());
+    F *__restrict__ y = y_in(a, F());
 
     const int tid = threadIdx.x;
     const int64_t per = nwg / VNEC_XCDS;  // workgroups per XCD
@@ -319,7 +337,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
                         c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
                     }
                 }
-                s_prod[pos] = widen(cur.v[u]) * a.x[c];
+                s_prod[pos] = widen(cur.v[u]) * x[c];
             }
         }
         __syncthreads();
@@ -330,7 +348,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
         const int d1 = d0 + IPT < items ? d0 + IPT : items;
         int rx = (int)merge_search(s_rowend, nrow, y0, nz, d0);
         int ny = d0 - rx;
-        double acc = 0.0, head_val = 0.0;
+        F acc = 0, head_val = 0;
         int flag = 0, head_row = 0;
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
@@ -341,9 +359,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
                         head_val = acc;
                         flag = 1;
                     } else {
-                        a.y[x0 + rx] = acc;
+                        y[x0 + rx] = acc;
                     }
-                    acc = 0.0;
+                    acc = 0;
                     rx++;
                 } else {
                     acc += s_prod[ny];
@@ -353,10 +371,10 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
         }
 
         // --- segmented join across the tile's threads; the tile's carry-out
-        double incl;
-        const double cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
-        if (flag) a.y[head_row] = head_val + cin;
-        if (tid == VNEC_BLOCK - 1) a.carry[t] = incl;
+        F incl;
+        const F cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
+        if (flag) y[head_row] = head_val + cin;
+        if (tid == VNEC_BLOCK - 1) carry_in(a, F())[t] = incl;
         __syncthreads();
 
         cur = nxt;
@@ -374,7 +392,7 @@ static strided_kernel_t pick_strided(int variant, int ipt, int enc)
         if (variant == 0) return k_spmv_strided<I, E, false>;                 \
         if (variant == 6 && E == 0) return k_spmv_strided<I, 0, true>;        \
     }
-#define TIPT(I) TCASE(I, 0) TCASE(I, 1) TCASE(I, 2)
+#define TIPT(I) TCASE(I, 0) TCASE(I, 1) TCASE(I, 2) TCASE(I, 3)
     TIPT(3) TIPT(5) TIPT(7) TIPT(9) TIPT(11) TIPT(13) TIPT(15)
     return nullptr;
 #undef TIPT

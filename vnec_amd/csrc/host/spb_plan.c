@@ -453,11 +453,15 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     vnec_tuning tune;
     const int fp32 = a->base.element_type == SpB_FP32;
     if (fp32 && (!t || t->variant < 0)) {
-        /* FP32 matrices: the 64-row slice layout when its padding is small,
-         * else merge-path tiles of 7 items per thread (the selector's
-         * measurements are of the FP64 kernels) */
+        /* FP32 matrices (the selector's measurements are of the FP64
+         * kernels): pattern slices where the pattern rule holds, else the
+         * 64-row slice layout when its padding is small, else merge-path
+         * tiles of 7 items per thread with per-tile carries */
         vnec_tuning f = {VNEC_VAR_L, 7, 64, VNEC_ENC_FP32, 1, -1};
-        if (vnec_slice_padding(a) <= 1.15) {
+        if (vnec_pattern_rule(a)) {
+            f.variant = VNEC_VAR_G;
+            f.ipt = 8;
+        } else if (vnec_slice_padding(a) <= 1.15) {
             f.variant = VNEC_VAR_S;
             f.ipt = 3;
         }
@@ -515,8 +519,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         return vnec_fail("variant %d is not available", tune.variant);
     if (tune.encoding < 0 || tune.encoding >= VNEC_ENC_N) return vnec_fail("bad encoding");
     if (tune.grid < -2) return vnec_fail("grid %d: use -2, -1, 0 or a positive multiple of 8", tune.grid);
-    if (tune.grid == -2 && !((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.nrhs <= 1 &&
-                             tune.encoding != VNEC_ENC_FP32))
+    if (tune.grid == -2 && !((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.nrhs <= 1))
         return vnec_fail("grid -2 (strided persistent): variants L and C with one right-hand side");
     if (fp32 != (tune.encoding == VNEC_ENC_FP32))
         return vnec_fail(fp32 ? "an FP32 matrix takes encoding VNEC_ENC_FP32"
@@ -526,7 +529,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             tune.variant != VNEC_VAR_G)
             return vnec_fail("encoding VNEC_ENC_FP32: variants L, S, D and G");
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
-        if (tune.variant == VNEC_VAR_L) tune.grid = -1; /* one workgroup per tile */
+        if (tune.variant == VNEC_VAR_L && tune.grid >= 0) tune.grid = -1; /* per-tile carries */
     }
     const int merge_form = tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C;
     if (merge_form && tune.lanes != 64 && (tune.lanes & ~(VNEC_LANES_RELABEL | VNEC_LANES_WIDE)))
@@ -622,7 +625,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             /* per-tile carries from a persistent launch when asked for, or
              * when each resident workgroup would see at least 16 tiles */
             int snwg = 0;
-            if (rc == SpB_SUCCESS && tune.nrhs == 1 && tune.encoding != VNEC_ENC_FP32 &&
+            if (rc == SpB_SUCCESS && tune.nrhs == 1 &&
                 (tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) &&
                 vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, &snwg) == 0 &&
                 (tune.grid == -2 || p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
