@@ -125,8 +125,9 @@ typedef struct {
     int grid;     /* 0: persistent, one workgroup per resident slot;
                      > 0: persistent with this many workgroups (multiple
                      of 8); -1: every tile leaves its own carry, one
-                     workgroup per tile (or, with at least 16 tiles per
-                     resident workgroup, the form of -2); -2: the same
+                     workgroup per tile (or, with at least as many
+                     tiles as resident workgroups, the form of -2); -2:
+                     the same
                      per-tile carries from a persistent launch whose
                      workgroups sweep each XCD's tiles together (variants
                      L and C, one right-hand side)                       */

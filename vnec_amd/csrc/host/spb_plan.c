@@ -25,9 +25,11 @@ static int plan_threads(const vnec_tuning *t) { return plan_block(t) ? VNEC_WIDE
 
 static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
 /* Tiles per resident workgroup from which per-tile-carry plans use the
- * strided persistent launch (its pipelining pays once a workgroup sees
- * many tiles; below, the one-workgroup-per-tile launch is as fast). */
-#define STRIDED_MIN_TILES_PER_WG 16
+ * strided persistent launch (the same tiles and carries, so the same bits):
+ * from one tile per resident workgroup on it was 1-8 % faster on R-MAT,
+ * uniform and power-law matrices; on a fifth of a tile per workgroup the
+ * one-workgroup-per-tile launch was up to 4 % faster. */
+#define STRIDED_MIN_TILES_PER_WG 1
 /* The largest matrices the compiled selector model was trained on. */
 #define SELECTOR_TRAINED_MAX_NNZ ((int64_t)1 << 24)
 static SpB_Info build_csort(struct vnec_plan_s *p, spb_csr *a);
@@ -623,7 +625,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             p->carry = (double *)vnec_malloc_dev(sizeof(double) * (size_t)(p->ntiles * tune.nrhs), &p->dev_bytes);
             if (!p->carry) rc = SpB_FAILURE;
             /* per-tile carries from a persistent launch when asked for, or
-             * when each resident workgroup would see at least 16 tiles */
+             * when each resident workgroup would see at least
+             * STRIDED_MIN_TILES_PER_WG tiles */
             int snwg = 0;
             if (rc == SpB_SUCCESS && tune.nrhs == 1 &&
                 (tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) &&
