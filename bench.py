@@ -251,7 +251,9 @@ def selection_label(spb, A, mode, info, k):
     if mode != "learned" or k > 1:
         return mode
     c = spb.candidates()[spb.select(A, "learned")]
-    same_grid = c.grid == info.grid or c.grid >= 0 and info.grid >= 0
+    # persistent picks report their workgroup count; per-tile-carry picks (-1)
+    # run the strided launch (-2) by themselves once the tiles fill the GPU
+    same_grid = c.grid == info.grid or (c.grid >= 0 and info.grid >= 0) or (c.grid < 0 and info.grid < 0)
     if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
         return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
     flags = info.lanes if info.variant in ("L", "C") and info.lanes != 64 else 0

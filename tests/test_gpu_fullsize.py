@@ -10,7 +10,9 @@ plan run twice with identical bits.
 * skew (R-MAT 22 + 8 dense rows of 500k columns): the learned plan, whose
   dense rows span hundreds of tiles;
 * the 512-member batch, k = 1 and k = 8 (the grouped multi-RHS kernel),
-  against the block-diagonal CSR with member columns made global.
+  against the block-diagonal CSR with member columns made global;
+* R-MAT 18 and the stencil once more, against the y of the reference
+  library compiled from its own sources (its IRD dispatch, VNEC-D or -S).
 """
 import numpy as np
 import pytest
@@ -80,6 +82,38 @@ def test_stencil128_variant_g(enc, tol):
     worst = _gate(m, _run_twice(p, m.x, m.nrows), tol)
     if enc == "hi":
         assert worst > 1e-12
+    p.free()
+
+
+def _avx512():
+    try:
+        return "avx512f" in open("/proc/cpuinfo").read()
+    except OSError:
+        return False
+
+
+@pytest.mark.parametrize("name", ["rmat18", "stencil128"])
+def test_full_size_against_the_reference_library(name):
+    """The learned plan's y at the benchmark sizes against the y of the
+    reference library itself (compiled from its sources; its dispatch: IRD,
+    then VNEC-D or VNEC-S at P = 36, L = 8), row by row:
+    |y - y_ref| <= 1e-12 (|A||x|)_i."""
+    if not (O.ref_available() and _avx512()):
+        pytest.skip("the reference library needs its build and an AVX-512 host")
+    m = g.rmat(18, 16, 0.57, 0.19, 0.19, seed=1) if name == "rmat18" else _stencil()
+    R = O.RefMatrix(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    kind = "D" if R.ird() == 0 else "S"
+    print(f"{name} nnz {m.nnz}: reference VNEC-{kind}", flush=True)
+    plan = R.build_d() if kind == "D" else R.build_s()
+    y_ref = R.spmv(kind, plan, m.x)
+    R.free(kind, plan)
+    R.close()
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, m.ptr, m.col, m.val)
+    p = spb.Plan(A, "learned")
+    y = _run_twice(p, m.x, m.nrows)
+    ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
+    ok, worst = O.check_rows(y, y_ref, ax, TOL)
+    assert ok, (name, kind, worst)
     p.free()
 
 
