@@ -430,10 +430,24 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_carry_fixup(const int2 *__restri
         longrun = head && t + FIX_SHORT < nb && co[t + FIX_SHORT + 1].x == row;
     }
     if (head && !longrun) {
-        int n = 1;  // tiles in the run, at most FIX_SHORT
-        while (t + n < nb && co[t + n + 1].x == row) n++;
+        // the run is tiles t .. t + n - 1, n <= FIX_SHORT: the end rows and
+        // carries of all FIX_SHORT candidates are loaded together, then the
+        // run's carries are summed in tile order
+        int er[FIX_SHORT];
+        F cv[FIX_SHORT];
+#pragma unroll
+        for (int i = 0; i < FIX_SHORT; i++) {
+            const bool ok = t + i < nb;
+            er[i] = i == 0 ? row : (ok ? co[t + i + 1].x : -1);
+            cv[i] = ok ? carry[(t + i) * K + c] : (F)0;
+        }
         F s = 0;
-        for (int i = 0; i < n; i++) s += carry[(t + i) * K + c];
+        bool in = true;
+#pragma unroll
+        for (int i = 0; i < FIX_SHORT; i++) {
+            in = in && er[i] == row;
+            if (in) s += cv[i];
+        }
         y[(int64_t)row * K + c] += s;
     }
     const int lane = lane_id();

@@ -783,8 +783,25 @@ __global__ void k_wg_fixup(const int32_t *__restrict__ wg_row, const double *__r
     int row = wg_row[w];
     if (row >= nrows) return;
     if (w > 0 && wg_row[w - 1] == row) return;
+    // the run's first four candidates are loaded together, then summed in
+    // workgroup order; longer runs continue one by one
+    int rr[4];
+    double cc[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const bool ok = w + i < nwg;
+        rr[i] = i == 0 ? row : (ok ? wg_row[w + i] : -1);
+        cc[i] = ok ? wg_carry[w + i] : 0.0;
+    }
     double s = 0.0;
-    for (int u = w; u < nwg && wg_row[u] == row; u++) s += wg_carry[u];
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        in = in && rr[i] == row;
+        if (in) s += cc[i];
+    }
+    if (in)
+        for (int u = w + 4; u < nwg && wg_row[u] == row; u++) s += wg_carry[u];
     y[row] += s;
 }
 
