@@ -231,8 +231,16 @@ class TuningInfo:
     dev_bytes: int
 
 
+# vnec_tuning.lanes flags of L and C plans (include/vnec_ext.h)
+LANES_RELABEL = 1  # columns relabelled hot-first, x permuted before each SpMV
+LANES_WIDE = 1024  # C only: 1024-thread tiles of 1024 * ipt items
+
+
 class Plan:
-    """GPU execution plan (vnec_plan_build_ext)."""
+    """GPU execution plan (vnec_plan_build_ext).  variant: a variant letter
+    or a selection mode ("learned", "autotune", "rule", "reference");
+    lanes: the D block width, H's hot entries, or for L and C 64 or the
+    flags LANES_RELABEL | LANES_WIDE."""
 
     def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
                  encoding: str | None = None, nrhs: int = 1, grid: int = 0):
