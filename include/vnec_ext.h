@@ -118,7 +118,7 @@ typedef struct {
     int lanes;    /* D block width / H hot x entries in LDS; L and C:
                      64, or the flags VNEC_LANES_RELABEL (columns
                      relabelled by descending use, x permuted before each
-                     SpMV) and, for C, VNEC_LANES_WIDE (1024-thread
+                     SpMV) and, for C, VNEC_LANES_WIDE (512-thread
                      tiles), alone or or-ed together                     */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
@@ -137,10 +137,10 @@ typedef struct {
  * descending use: the most used x entries share cache lines (the same sums
  * in the same order as without it). */
 #define VNEC_LANES_RELABEL 1
-/* vnec_tuning.lanes of a C plan whose tiles hold 1024 * ipt items (ipt 3,
- * 5, 7 or 9; persistent launch, grid >= 0) instead of 256 * ipt: each
- * tile's nonzeros are sorted by column over the larger window, so the x
- * gathers of a wavefront share more cache lines. */
+/* vnec_tuning.lanes of a C plan whose tiles hold 512 * ipt items (ipt 3,
+ * 5, 7, 9 or 11; persistent launch, grid >= 0; 512-thread workgroups)
+ * instead of 256 * ipt: each tile's nonzeros are sorted by column over the
+ * larger window, so the x gathers of a wavefront share more cache lines. */
 #define VNEC_LANES_WIDE 1024
 
 typedef struct vnec_plan_s *vnec_plan;
@@ -275,7 +275,7 @@ SpB_Info vnec_ird_ext(const SpB_Matrix A, int64_t P, int L, double thr, float *i
 
 /* Candidate configurations (single right-hand side, FP64).  Copies up to
  * max of them to out (may be NULL); returns VNEC_NCAND. */
-#define VNEC_NCAND 22
+#define VNEC_NCAND 23
 int vnec_candidates_ext(vnec_tuning *out, int max);
 /* Which candidates may run on A: bit c set when candidate c applies
  * (slice candidates need padding <= 3 and fewer than 2^31 slots; D

@@ -113,10 +113,10 @@ typedef struct {
 } vnec_merge_args;
 
 /* Threads per tile of the wide column-sorted tiles (variant C, FP64, one
- * right-hand side, persistent launch; ipt 3, 5, 7 or 9): a tile holds
+ * right-hand side, persistent launch; ipt 3, 5, 7, 9 or 11): a tile holds
  * VNEC_WIDE_BLOCK * ipt items and its nonzeros are sorted by column over
  * that whole window. */
-#define VNEC_WIDE_BLOCK 1024
+#define VNEC_WIDE_BLOCK 512
 
 int vnec_hip_spmv_merge(const vnec_merge_args *a, void *stream);
 /* Adds each tile's carry to y[row] in tile order (one add per row; rows

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-NF, NH, NC = 32, 64, 22
+NF, NH, NC = 32, 64, 23
 SHAPES = [("W1", (NF, NH)), ("b1", (NH,)), ("W2", (NH, NH)), ("b2", (NH,)),
           ("W3", (NH, NC)), ("b3", (NC,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)

@@ -758,7 +758,7 @@ def test_relabelled_columns_give_the_plain_plans_bits(variant, ipt, grid):
 
 @pytest.mark.parametrize("name", ["rmat14", "skew14", "holes"])
 def test_wide_column_sorted_tiles(name):
-    """Variant C with 1024-thread tiles (1024 ipt items, each sorted by column
+    """Variant C with 512-thread tiles (512 ipt items, each sorted by column
     over the whole tile): every ipt passes the row gate, long rows spanning
     many tiles and empty rows included; relabelled columns on top give the
     same bits (rows are still summed in CSR order within the same tiles);
@@ -777,16 +777,16 @@ def test_wide_column_sorted_tiles(name):
     A = _mat(m)
     ax, _ = O.spmv_abs(m.ptr, m.col, m.val, m.x)
     ref = O.spmv_serial(m.ptr, m.col, m.val, m.x)
-    for ipt in (3, 5, 7, 9):
+    for ipt in (3, 5, 7, 9, 11):
         y = _run_plan(A, m.x, "C", ipt, lanes=1024)
         assert O.check_rows(y, ref, ax, TOL)[0], ipt
         if ipt == 9:
             assert np.array_equal(_run_plan(A, m.x, "C", ipt, lanes=1024 | 1), y)
     p = spb.Plan(A, "C", 5, lanes=1024)
     info = p.info()
-    assert info.ntiles == -(-(m.nrows + m.nnz) // (1024 * 5))
+    assert info.ntiles == -(-(m.nrows + m.nnz) // (512 * 5))  # VNEC_WIDE_BLOCK threads per tile
     p.free()
-    for bad in [dict(variant="L", ipt=7, lanes=1024), dict(variant="C", ipt=11, lanes=1024),
+    for bad in [dict(variant="L", ipt=7, lanes=1024), dict(variant="C", ipt=13, lanes=1024),
                 dict(variant="C", ipt=9, lanes=1024, grid=-1), dict(variant="C", ipt=9, lanes=3)]:
         with pytest.raises(RuntimeError, match="wide tiles|lanes"):
             spb.Plan(A, bad.pop("variant"), bad.pop("ipt"), **bad)

@@ -251,7 +251,7 @@ SIGNATURES = {
 }
 
 # Constants of include/vnec_ext.h used by the selector bindings.
-NCAND = 22
+NCAND = 23
 NFEAT = 32
 NFEAT_INTS = 32
 MLP_HIDDEN = 64

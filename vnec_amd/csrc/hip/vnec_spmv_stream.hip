@@ -104,8 +104,9 @@ __device__ __forceinline__ double gather_x(const vnec_merge_args &a, const doubl
 // SORT (variant C): the tile's nonzeros arrive in column order, so each
 // wavefront's x gathers are ascending and share cache lines where columns
 // are dense; each product is written to its CSR position in LDS.
-// BS: threads per tile (256, or 1024 for the wide column-sorted tiles,
-// whose larger column-order window shares more cache lines per gather).
+// BS: threads per tile (256, or VNEC_WIDE_BLOCK for the wide column-sorted
+// tiles, whose larger column-order window shares more cache lines per
+// gather).
 template <int IPT, bool WIN, int ENC, int HOT = 0, bool SORT = false, int BS = VNEC_BLOCK>
 __global__ __launch_bounds__(BS) void k_spmv_stream(vnec_merge_args a, int nwg,
                                                     int32_t *__restrict__ wg_row,
@@ -813,7 +814,7 @@ static stream_kernel_t pick_stream(int variant, int ipt, int enc, int block)
         if (variant != 6 || enc != 0) return nullptr;
 #define WCASE(I) \
     if (ipt == I) return k_spmv_stream<I, false, 0, 0, true, VNEC_WIDE_BLOCK>;
-        WCASE(3) WCASE(5) WCASE(7) WCASE(9)
+        WCASE(3) WCASE(5) WCASE(7) WCASE(9) WCASE(11)
 #undef WCASE
         return nullptr;
     }

@@ -491,6 +491,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         if (!ok) {
             tune.variant = VNEC_VAR_L;
             tune.ipt = 7;
+            tune.lanes = 64;
         }
         const int mode = t ? t->variant : VNEC_SELECT_LEARNED;
         if ((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.nrhs == 1 &&
@@ -502,6 +503,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
              * on, measure the four forms */
             if (mode == VNEC_SELECT_RULE) {
                 tune.variant = vnec_csort_rule(a) ? VNEC_VAR_C : VNEC_VAR_L;
+                tune.lanes = 64;
             } else if (mode == VNEC_SELECT_AUTOTUNE ||
                        (mode == VNEC_SELECT_LEARNED && (int64_t)a->nnz > SELECTOR_TRAINED_MAX_NNZ)) {
                 if (vnec_refine_merge_plan(a, &tune) != SpB_SUCCESS) return SpB_FAILURE;
@@ -544,7 +546,7 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_WIDE) &&
         !(tune.variant == VNEC_VAR_C && tune.grid >= 0 &&
           vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)))
-        return vnec_fail("wide tiles (lanes %d): variant C, ipt 3, 5, 7 or 9, persistent launch (grid >= 0)",
+        return vnec_fail("wide tiles (lanes %d): variant C, ipt 3, 5, 7, 9 or 11, persistent launch (grid >= 0)",
                          VNEC_LANES_WIDE);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");

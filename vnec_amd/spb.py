@@ -233,7 +233,7 @@ class TuningInfo:
 
 # vnec_tuning.lanes flags of L and C plans (include/vnec_ext.h)
 LANES_RELABEL = 1  # columns relabelled hot-first, x permuted before each SpMV
-LANES_WIDE = 1024  # C only: 1024-thread tiles of 1024 * ipt items
+LANES_WIDE = 1024  # C only: 512-thread tiles of 512 * ipt items
 
 
 class Plan:
