@@ -257,7 +257,7 @@ def selection_label(spb, A, mode, info, k):
     if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
         return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
     flags = info.lanes if info.variant in ("L", "C") and info.lanes != 64 else 0
-    form = (f"{info.variant}{info.ipt} grid {info.grid}" + (" 1024-thread tiles" if flags & 1024 else "")
+    form = (f"{info.variant}{info.ipt} grid {info.grid}" + (" 512-thread tiles" if flags & 1024 else "")
             + (" relabelled columns" if flags & 1 else ""))
     return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {form})"
 

@@ -126,8 +126,8 @@ typedef struct {
                      > 0: persistent with this many workgroups (multiple
                      of 8); -1: every tile leaves its own carry, one
                      workgroup per tile (or, with at least as many
-                     tiles as resident workgroups, the form of -2); -2:
-                     the same
+                     tiles as resident workgroups, and always with
+                     VNEC_LANES_WIDE, the form of -2); -2: the same
                      per-tile carries from a persistent launch whose
                      workgroups sweep each XCD's tiles together (variants
                      L and C, one right-hand side)                       */
@@ -138,9 +138,10 @@ typedef struct {
  * in the same order as without it). */
 #define VNEC_LANES_RELABEL 1
 /* vnec_tuning.lanes of a C plan whose tiles hold 512 * ipt items (ipt 3,
- * 5, 7, 9 or 11; persistent launch, grid >= 0; 512-thread workgroups)
- * instead of 256 * ipt: each tile's nonzeros are sorted by column over the
- * larger window, so the x gathers of a wavefront share more cache lines. */
+ * 5, 7, 9 or 11; FP64, one right-hand side; 512-thread workgroups, from a
+ * persistent launch or with per-tile carries) instead of 256 * ipt: each
+ * tile's nonzeros are sorted by column over the larger window, so the x
+ * gathers of a wavefront share more cache lines. */
 #define VNEC_LANES_WIDE 1024
 
 typedef struct vnec_plan_s *vnec_plan;

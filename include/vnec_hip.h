@@ -148,7 +148,7 @@ int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, do
  * a.carry (for vnec_hip_carry_fixup) and the same results bit for bit;
  * nwg workgroups (a multiple of 8), those on XCD x sweeping x's share of
  * the tiles together.  vnec_hip_strided_grid gives the resident count. */
-int vnec_hip_strided_grid(int variant, int ipt, int encoding, int *nwg);
+int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg);
 int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream);
 /* Variant H: the persistent kernel with the hot x entries in LDS (hot =
  * 4096 or 8192 slots at 5 or 7 items per thread; FP64, one right-hand

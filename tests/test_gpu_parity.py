@@ -759,7 +759,8 @@ def test_relabelled_columns_give_the_plain_plans_bits(variant, ipt, grid):
 @pytest.mark.parametrize("name", ["rmat14", "skew14", "holes"])
 def test_wide_column_sorted_tiles(name):
     """Variant C with 512-thread tiles (512 ipt items, each sorted by column
-    over the whole tile): every ipt passes the row gate, long rows spanning
+    over the whole tile), persistent and with per-tile carries: every ipt
+    passes the row gate, long rows spanning
     many tiles and empty rows included; relabelled columns on top give the
     same bits (rows are still summed in CSR order within the same tiles);
     the plan refuses the combinations it has no kernel for."""
@@ -780,14 +781,19 @@ def test_wide_column_sorted_tiles(name):
     for ipt in (3, 5, 7, 9, 11):
         y = _run_plan(A, m.x, "C", ipt, lanes=1024)
         assert O.check_rows(y, ref, ax, TOL)[0], ipt
+        # per-tile carries (the strided launch of 512-thread tiles)
+        assert O.check_rows(_run_plan(A, m.x, "C", ipt, lanes=1024, grid=-1), ref, ax, TOL)[0], ipt
         if ipt == 9:
             assert np.array_equal(_run_plan(A, m.x, "C", ipt, lanes=1024 | 1), y)
     p = spb.Plan(A, "C", 5, lanes=1024)
     info = p.info()
     assert info.ntiles == -(-(m.nrows + m.nnz) // (512 * 5))  # VNEC_WIDE_BLOCK threads per tile
     p.free()
+    p = spb.Plan(A, "C", 7, lanes=1024, grid=-1)
+    assert p.info().grid == -2  # wide tiles with per-tile carries run the strided launch
+    p.free()
     for bad in [dict(variant="L", ipt=7, lanes=1024), dict(variant="C", ipt=13, lanes=1024),
-                dict(variant="C", ipt=9, lanes=1024, grid=-1), dict(variant="C", ipt=9, lanes=3)]:
+                dict(variant="C", ipt=9, lanes=1024, nrhs=4), dict(variant="C", ipt=9, lanes=3)]:
         with pytest.raises(RuntimeError, match="wide tiles|lanes"):
             spb.Plan(A, bad.pop("variant"), bad.pop("ipt"), **bad)
     A.free()

@@ -196,8 +196,8 @@ __device__ __forceinline__ void fill_window(const F *__restrict__ x,
 // Joins per-thread (flag, tail) pairs of a segmented sum across the
 // workgroup.  Returns the exclusive carry into this thread; *incl (if not
 // NULL) gets the inclusive value (the last thread's closes the tile).
-// s_f / s_v: VNEC_NWAVES + 1 entries.
-template <typename F>
+// s_f / s_v: NW + 1 entries (NW: wavefronts in the workgroup).
+template <typename F, int NW = VNEC_NWAVES>
 __device__ __forceinline__ F block_seg_carry(int flag, F tail, int *s_f, F *s_v, F *incl)
 {
     int f = flag;
@@ -211,7 +211,7 @@ __device__ __forceinline__ F block_seg_carry(int flag, F tail, int *s_f, F *s_v,
     __syncthreads();
     if (threadIdx.x == 0) {
         F c = 0;
-        for (int k = 0; k < VNEC_NWAVES; k++) {
+        for (int k = 0; k < NW; k++) {
             const F tw = s_v[k];
             const int fw = s_f[k];
             s_v[k] = c;  // carry into wave k

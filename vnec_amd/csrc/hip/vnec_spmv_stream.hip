@@ -287,15 +287,17 @@ __global__ __launch_bounds__(BS) void k_spmv_stream(vnec_merge_args a, int nwg,
 // tiles j, j + W, j + 2W, ... of it (W workgroups per XCD), so the tiles in
 // flight on one XCD are neighbours and share its L2, while each workgroup
 // issues its next tile's loads before reducing the current one.
-template <int IPT, int ENC, bool SORT>
-__global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, int nwg)
+// BS: threads per tile (256, or VNEC_WIDE_BLOCK for wide column-sorted tiles).
+template <int IPT, int ENC, bool SORT, int BS = VNEC_BLOCK>
+__global__ __launch_bounds__(BS) void k_spmv_strided(vnec_merge_args a, int nwg)
 {
     typedef typename enc_real<ENC>::type F;  // FP32 for the FP32 API's arithmetic
-    constexpr int T = VNEC_BLOCK * IPT;
+    constexpr int T = BS * IPT;
+    constexpr int NW = BS / VNEC_WAVE;
     __shared__ F s_prod[T];
     __shared__ int s_rowend[T];
-    __shared__ int s_f[VNEC_NWAVES + 1];
-    __shared__ F s_v[VNEC_NWAVES + 1];
+    __shared__ int s_f[NW + 1];
+    __shared__ F s_v[NW + 1];
     const F *__restrict__ x = x_in(a, F());
     F *__restrict__ y = y_in(a, F());
 
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
 
     int2 A0 = co[t], A1 = co[t + 1];
     TileRegs<IPT, false, ENC> cur, nxt;
-    load_tile<IPT, false, ENC, SORT>(a, t, A0, A1, cur, tid);
+    load_tile<IPT, false, ENC, SORT, BS>(a, t, A0, A1, cur, tid);
 
     for (; t < xhi; t += per) {
         // --- issue the next tile's loads
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
         if (tn < xhi) {
             B0 = co[tn];
             B1 = co[tn + 1];
-            load_tile<IPT, false, ENC, SORT>(a, tn, B0, B1, nxt, tid);
+            load_tile<IPT, false, ENC, SORT, BS>(a, tn, B0, B1, nxt, tid);
         }
 
         const int x0 = A0.x, y0 = A0.y;
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
         // --- row ends and products into LDS
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
-            const int k = u * VNEC_BLOCK + tid;
+            const int k = u * BS + tid;
             if (k < nrow) s_rowend[k] = cur.re[u];
             if (k < nz) {
                 int c = cur.c[u], pos = k;
@@ -373,9 +375,9 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
 
         // --- segmented join across the tile's threads; the tile's carry-out
         F incl;
-        const F cin = block_seg_carry(flag, acc, s_f, s_v, &incl);
+        const F cin = block_seg_carry<F, NW>(flag, acc, s_f, s_v, &incl);
         if (flag) y[head_row] = head_val + cin;
-        if (tid == VNEC_BLOCK - 1) carry_in(a, F())[t] = incl;
+        if (tid == BS - 1) carry_in(a, F())[t] = incl;
         __syncthreads();
 
         cur = nxt;
@@ -384,10 +386,22 @@ __global__ __launch_bounds__(VNEC_BLOCK) void k_spmv_strided(vnec_merge_args a, 
     }
 }
 
+// Threads per workgroup of a launch: 256, or VNEC_WIDE_BLOCK for wide tiles.
+static int block_threads(int block) { return block == VNEC_WIDE_BLOCK ? VNEC_WIDE_BLOCK : VNEC_BLOCK; }
+
 typedef void (*strided_kernel_t)(vnec_merge_args, int);
 
-static strided_kernel_t pick_strided(int variant, int ipt, int enc)
+static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
 {
+    if (block == VNEC_WIDE_BLOCK) {  // wide column-sorted tiles: FP64, C
+        if (variant != 6 || enc != 0) return nullptr;
+#define WTCASE(I) \
+    if (ipt == I) return k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK>;
+        WTCASE(3) WTCASE(5) WTCASE(7) WTCASE(9) WTCASE(11)
+#undef WTCASE
+        return nullptr;
+    }
+    if (block != 0 && block != VNEC_BLOCK) return nullptr;
 #define TCASE(I, E)                                                           \
     if (ipt == I && enc == E) {                                               \
         if (variant == 0) return k_spmv_strided<I, E, false>;                 \
@@ -400,12 +414,13 @@ static strided_kernel_t pick_strided(int variant, int ipt, int enc)
 #undef TCASE
 }
 
-extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int *nwg)
+extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg)
 {
-    strided_kernel_t k = pick_strided(variant, ipt, encoding);
+    strided_kernel_t k = pick_strided(variant, ipt, encoding, block);
     if (!k) return (int)hipErrorInvalidValue;
     int per_cu = 0, dev = 0, cus = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, VNEC_BLOCK, 0);
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, block_threads(block), 0);
     if (e != hipSuccess) return (int)e;
     e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
@@ -419,10 +434,11 @@ extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int *nw
 
 extern "C" int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream)
 {
-    strided_kernel_t k = pick_strided(a->variant, a->ipt, a->encoding);
+    strided_kernel_t k = pick_strided(a->variant, a->ipt, a->encoding, a->block);
     if (!k || a->nrhs != 1 || nwg < VNEC_XCDS || nwg % VNEC_XCDS) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3(VNEC_BLOCK), 0, (hipStream_t)stream, *a, nwg);
+    hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3((unsigned)block_threads(a->block)), 0, (hipStream_t)stream,
+                       *a, nwg);
     return (int)hipGetLastError();
 }
 
@@ -832,7 +848,6 @@ static stream_kernel_t pick_stream(int variant, int ipt, int enc, int block)
 #undef SCASE
 }
 
-static int block_threads(int block) { return block == VNEC_WIDE_BLOCK ? VNEC_WIDE_BLOCK : VNEC_BLOCK; }
 
 extern "C" int vnec_hip_stream_supported(int variant, int ipt, int encoding, int block)
 {

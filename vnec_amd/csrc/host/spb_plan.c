@@ -544,9 +544,9 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         return vnec_fail("relabelled columns (lanes %d): variants L and C, FP64, one right-hand side",
                          VNEC_LANES_RELABEL);
     if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_WIDE) &&
-        !(tune.variant == VNEC_VAR_C && tune.grid >= 0 &&
+        !(tune.variant == VNEC_VAR_C && tune.nrhs == 1 &&
           vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)))
-        return vnec_fail("wide tiles (lanes %d): variant C, ipt 3, 5, 7, 9 or 11, persistent launch (grid >= 0)",
+        return vnec_fail("wide tiles (lanes %d): variant C, FP64, one right-hand side, ipt 3, 5, 7, 9 or 11",
                          VNEC_LANES_WIDE);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
@@ -632,8 +632,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             int snwg = 0;
             if (rc == SpB_SUCCESS && tune.nrhs == 1 &&
                 (tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) &&
-                vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, &snwg) == 0 &&
-                (tune.grid == -2 || p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
+                vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, plan_block(&tune), &snwg) == 0 &&
+                (tune.grid == -2 || plan_block(&tune) || p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
                 p->snwg = snwg;
                 p->tune.grid = -2;
             }

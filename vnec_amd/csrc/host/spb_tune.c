@@ -28,7 +28,7 @@ static const vnec_tuning CANDIDATES[VNEC_NCAND] = {
     {VNEC_VAR_E, 5, 64, 0, 1, 0},   {VNEC_VAR_E, 7, 64, 0, 1, 0},
     {VNEC_VAR_D, 7, 4, 0, 1, -1},   {VNEC_VAR_D, 7, 8, 0, 1, -1},   {VNEC_VAR_D, 7, 16, 0, 1, -1},
     {VNEC_VAR_C, 9, 64, 0, 1, 0},   {VNEC_VAR_C, 11, 64, 0, 1, 0},  {VNEC_VAR_C, 7, 64, 0, 1, -1},
-    {VNEC_VAR_C, 11, 64, 0, 1, -1}, {VNEC_VAR_G, 8, 64, 0, 1, -1},  {VNEC_VAR_C, 9, VNEC_LANES_WIDE, 0, 1, 0},
+    {VNEC_VAR_C, 11, 64, 0, 1, -1}, {VNEC_VAR_G, 8, 64, 0, 1, -1},  {VNEC_VAR_C, 9, VNEC_LANES_WIDE, 0, 1, -1},
 };
 /* Candidates named by the fixed selection modes. */
 #define CAND_S3 1
@@ -747,16 +747,21 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
         rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
     static const int VARIANTS[2] = {VNEC_VAR_L, VNEC_VAR_C}, GRIDS[2] = {0, -1};
     const int nforms = 8 * nc >= RELABEL_MIN_X_BYTES ? 8 : 4;
-    /* a pick of wide column-sorted tiles is measured as itself too */
-    const int wide = t->variant == VNEC_VAR_C && t->lanes != 64 && (t->lanes & VNEC_LANES_WIDE);
+    /* and column-sorted tiles of 512 threads, persistent or with per-tile
+     * carries, where the granularity has them */
+    const int nwide = vnec_hip_stream_supported(VNEC_VAR_C, t->ipt, VNEC_ENC_FP64, VNEC_WIDE_BLOCK) ? 2 : 0;
     double best = INFINITY;
     int bv = t->variant, bg = t->grid, bl = 64;
-    for (int k = 0; rc == SpB_SUCCESS && k < nforms + wide; k++) {
+    for (int k = 0; rc == SpB_SUCCESS && k < nforms + nwide; k++) {
         vnec_tuning u = *t;
         if (k < nforms) {
             u.variant = VARIANTS[(k / 2) % 2];
             u.grid = GRIDS[k % 2];
             u.lanes = k >= 4 ? VNEC_LANES_RELABEL : 64;
+        } else {
+            u.variant = VNEC_VAR_C;
+            u.grid = GRIDS[k - nforms];
+            u.lanes = VNEC_LANES_WIDE;
         }
         double tb = INFINITY;
         rc = time_tuning(a, &u, dx, dy, 20, &tb);
