@@ -159,7 +159,12 @@ def test_selection_modes_parity(mode):
         p = spb.Plan(A, mode)
         info = p.info()
         want = spb.candidates()[c]
-        if info.variant != want.variant:
+        if mode == "autotune" and want.variant in ("L", "C") and info.ipt != want.ipt:
+            # the autotune mode also measures column-sorted 512-thread tiles
+            # beside a merge-path pick, at 9 items per thread above 9
+            assert info.variant == "C" and info.lanes & spb.LANES_WIDE
+            assert info.ipt == min(want.ipt, 9)
+        elif info.variant != want.variant:
             # the same merge-path plan, CSR-ordered (L) or column-sorted (C):
             # the rule mode follows the column-reuse rule, the autotune mode
             # measures both (the learned mode does so only beyond its

@@ -56,7 +56,7 @@ typedef struct spb_csr {
     int refined_ipt;    /* merge-path neighbourhood measured at this ipt
                            (0: not yet): the fastest of L / C under
                            both schedules                            */
-    int refined_variant, refined_grid, refined_lanes;
+    int refined_variant, refined_grid, refined_lanes, refined_ipt_out;
     int refined_slice_ipt; /* slice-family pick measured at this ipt (0: not
                               yet): the fastest of S at it and variant G  */
     int refined_slice_variant, refined_slice_ipt_out;

@@ -729,10 +729,21 @@ SpB_Info vnec_refine_slice_plan(spb_csr *a, vnec_tuning *t)
  * caches closest to the kernels whatever its order. */
 #define RELABEL_MIN_X_BYTES ((int64_t)1 << 21)
 
+/* Granularity of the 512-thread column-sorted forms measured beside a pick:
+ * the pick's own where those tiles exist at it, 9 above it (at 11 items per
+ * thread a 512-thread workgroup holds 156 registers per lane, 3 wavefronts
+ * per SIMD), 0 where none applies. */
+static int wide_ipt(int ipt)
+{
+    if (ipt > 9) ipt = 9;
+    return vnec_hip_stream_supported(VNEC_VAR_C, ipt, VNEC_ENC_FP64, VNEC_WIDE_BLOCK) ? ipt : 0;
+}
+
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
 {
     if (a->refined_ipt == t->ipt) {
         t->variant = a->refined_variant;
+        t->ipt = a->refined_ipt_out;
         t->grid = a->refined_grid;
         t->lanes = a->refined_lanes;
         return SpB_SUCCESS;
@@ -745,31 +756,37 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     for (int64_t i = 0; rc == SpB_SUCCESS && i < nc; i++) hx[i] = 1.0 / (double)(1 + (i % 7));
     if (rc == SpB_SUCCESS && nc)
         rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(double) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
-    static const int VARIANTS[2] = {VNEC_VAR_L, VNEC_VAR_C}, GRIDS[2] = {0, -1};
-    const int nforms = 8 * nc >= RELABEL_MIN_X_BYTES ? 8 : 4;
-    /* and column-sorted tiles of 512 threads, persistent or with per-tile
-     * carries, where the granularity has them */
-    const int nwide = vnec_hip_stream_supported(VNEC_VAR_C, t->ipt, VNEC_ENC_FP64, VNEC_WIDE_BLOCK) ? 2 : 0;
-    double best = INFINITY;
-    int bv = t->variant, bg = t->grid, bl = 64;
-    for (int k = 0; rc == SpB_SUCCESS && k < nforms + nwide; k++) {
+    /* L or C tiles, persistent or with per-tile carries, at the pick's
+     * granularity; the same over hot-first relabelled columns when x is
+     * large; column-sorted 512-thread tiles, persistent or with per-tile
+     * carries, and over relabelled columns with per-tile carries */
+    const int relabel = 8 * nc >= RELABEL_MIN_X_BYTES, wipt = wide_ipt(t->ipt);
+    vnec_tuning forms[11];
+    int nf = 0;
+    for (int k = 0; k < (relabel ? 8 : 4); k++) {
         vnec_tuning u = *t;
-        if (k < nforms) {
-            u.variant = VARIANTS[(k / 2) % 2];
-            u.grid = GRIDS[k % 2];
-            u.lanes = k >= 4 ? VNEC_LANES_RELABEL : 64;
-        } else {
-            u.variant = VNEC_VAR_C;
-            u.grid = GRIDS[k - nforms];
-            u.lanes = VNEC_LANES_WIDE;
-        }
+        u.variant = (k / 2) % 2 ? VNEC_VAR_C : VNEC_VAR_L;
+        u.grid = k % 2 ? -1 : 0;
+        u.lanes = k >= 4 ? VNEC_LANES_RELABEL : 64;
+        forms[nf++] = u;
+    }
+    for (int k = 0; wipt && k < (relabel ? 3 : 2); k++) {
+        vnec_tuning u = *t;
+        u.variant = VNEC_VAR_C;
+        u.ipt = wipt;
+        u.grid = k == 0 ? 0 : -1;
+        u.lanes = k == 2 ? (VNEC_LANES_WIDE | VNEC_LANES_RELABEL) : VNEC_LANES_WIDE;
+        forms[nf++] = u;
+    }
+    double best = INFINITY;
+    vnec_tuning b = *t;
+    b.lanes = 64;
+    for (int k = 0; rc == SpB_SUCCESS && k < nf; k++) {
         double tb = INFINITY;
-        rc = time_tuning(a, &u, dx, dy, 20, &tb);
+        rc = time_tuning(a, &forms[k], dx, dy, 20, &tb);
         if (tb < best) {
             best = tb;
-            bv = u.variant;
-            bg = u.grid;
-            bl = u.lanes;
+            b = forms[k];
         }
     }
     free(hx);
@@ -777,9 +794,10 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     vnec_free_dev(dy);
     if (rc != SpB_SUCCESS) return rc;
     a->refined_ipt = t->ipt;
-    a->refined_variant = t->variant = bv;
-    a->refined_grid = t->grid = bg;
-    a->refined_lanes = t->lanes = bl;
+    a->refined_variant = t->variant = b.variant;
+    a->refined_ipt_out = t->ipt = b.ipt;
+    a->refined_grid = t->grid = b.grid;
+    a->refined_lanes = t->lanes = b.lanes;
     return SpB_SUCCESS;
 }
 
