@@ -94,9 +94,9 @@ typedef enum {
     VNEC_ENC_HILO = 1, /* FP32 hi + FP32 lo, a = hi + lo, FP64 accumulate */
     VNEC_ENC_HI = 2,   /* FP32 hi only, FP64 accumulate                 */
     VNEC_ENC_FP32 = 3, /* the FP32 API: FP32 values, x, y and sums (FP32
-                          matrices only; variants L (per-tile carries:
-                          grid -1 or -2), S, D and G, one right-hand
-                          side)                                         */
+                          matrices only; variants L and C (per-tile
+                          carries: grid -1 or -2), S, D and G, one
+                          right-hand side)                              */
     VNEC_ENC_N
 } vnec_encoding;
 

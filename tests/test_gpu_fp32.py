@@ -61,6 +61,7 @@ CASES = _cases()
 
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("variant,ipt,lanes", [("L", 3, 64), ("L", 7, 64), ("L", 15, 64),
+                                                ("C", 3, 64), ("C", 7, 64), ("C", 11, 64),
                                                 ("S", 3, 64), ("D", 5, 16), ("D", 7, 8),
                                                 ("D", 9, 4)])
 def test_fp32_plans_match_the_fp32_serial_oracle(name, m, variant, ipt, lanes):
@@ -179,22 +180,27 @@ def test_fp32_entry_points_refuse_mixed_types():
         spb.Plan(A32, "L", 7, encoding="fp64")
     with pytest.raises(RuntimeError, match="SpB_FP32"):
         spb.Plan(A64, "L", 7, encoding="fp32")
-    with pytest.raises(RuntimeError, match="variants L, S, D and G"):
+    with pytest.raises(RuntimeError, match="variants L, C, S, D and G"):
         spb.Plan(A32, "E", 7)
+    with pytest.raises(RuntimeError, match="relabelled columns"):
+        spb.Plan(A32, "C", 7, lanes=1)
+    with pytest.raises(RuntimeError, match="wide tiles"):
+        spb.Plan(A32, "C", 9, lanes=1024)
 
 
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("ipt", [3, 7, 11])
-def test_fp32_strided_launch_is_bit_identical_to_one_workgroup_per_tile(name, m, ipt):
-    """The FP32 API's merge-path plan with per-tile carries from the strided
-    persistent launch (grid -2): the tiles, carries and fix-up of grid -1, so
-    the same bits, within the FP32 gate."""
+@pytest.mark.parametrize("variant,ipt", [("L", 3), ("L", 7), ("L", 11), ("C", 7), ("C", 9)])
+def test_fp32_strided_launch_is_bit_identical_to_one_workgroup_per_tile(name, m, variant, ipt):
+    """The FP32 API's merge-path plans (CSR-ordered or column-sorted tiles)
+    with per-tile carries from the strided persistent launch (grid -2): the
+    tiles, carries and fix-up of grid -1, so the same bits, within the FP32
+    gate."""
     ptr, col, val, x = _f32(m)
     A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
     out = []
     for grid in (-1, -2):
-        print(f"case {name} FP32 L{ipt} grid {grid}", flush=True)
-        p = spb.Plan(A, "L", ipt, grid=grid)
+        print(f"case {name} FP32 {variant}{ipt} grid {grid}", flush=True)
+        p = spb.Plan(A, variant, ipt, grid=grid)
         assert p.info().grid == grid and p.info().encoding == "fp32"
         yv = spb.Vector(m.nrows, np.float32)
         p.spmv(yv, spb.Vector.from_array(x, np.float32))
@@ -207,9 +213,11 @@ def test_fp32_strided_launch_is_bit_identical_to_one_workgroup_per_tile(name, m,
 
 def test_fp32_selection_takes_pattern_slices_where_the_rule_holds():
     """FP32 matrices in a selection mode (what SpB_mxv uses): pattern slices
-    for a stencil of more than 2^20 nonzeros, 64-row slices or merge-path
-    tiles elsewhere; results within the FP32 gate."""
-    for m, want in ((g.stencil27(40, seed=31), "G"), (g.rmat(14, 16, seed=32), None)):
+    for a stencil of more than 2^20 nonzeros, column-sorted tiles where the
+    column-reuse rule holds (R-MAT 17), 64-row slices or merge-path tiles
+    elsewhere; results within the FP32 gate."""
+    for m, want in ((g.stencil27(40, seed=31), "G"), (g.rmat(17, 16, seed=71), "C"),
+                    (g.rmat(14, 16, seed=32), None)):
         ptr, col, val, x = _f32(m)
         A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
         p = spb.Plan(A, "learned")
@@ -227,3 +235,24 @@ def test_fp32_selection_takes_pattern_slices_where_the_rule_holds():
         spb.mxv(y2, A, spb.Vector.from_array(x, np.float32))
         assert np.array_equal(y2.to_array().view(np.uint32), yv.to_array().view(np.uint32))
         p.free()
+
+
+def test_fp32_reference_l_entry_takes_column_sorted_tiles_on_column_reuse():
+    """Build_VNEC_L on an FP32 matrix whose columns are reused (R-MAT 17,
+    the column-reuse rule) runs column-sorted tiles with per-tile carries;
+    SpMV_VNEC_L_FP32 stays within the FP32 gate and repeats its bits."""
+    m = g.rmat(17, 16, seed=71)
+    ptr, col, val, x = _f32(m)
+    A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
+    plan = spb.build_vnec("L", A)
+    info = spb.reference_plan_info(plan)
+    assert info.variant == "C" and info.encoding == "fp32" and info.grid in (-1, -2)
+    xv = spb.Vector.from_array(x, np.float32)
+    y = spb.Vector(m.nrows, np.float32)
+    assert spb.spmv_vnec("L", y, A, xv, plan) == ""
+    y1 = y.to_array()
+    ok, worst = _gate(y1, ptr, col, val, x)
+    assert ok, worst
+    assert spb.spmv_vnec("L", y, A, xv, plan) == ""
+    assert np.array_equal(y.to_array().view(np.uint32), y1.view(np.uint32))
+    spb.free_vnec("L", plan, fp32=True)

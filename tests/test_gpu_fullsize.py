@@ -127,12 +127,15 @@ def test_skew_full_size():
     p.free()
 
 
-@pytest.mark.parametrize("k,ipt", [(1, 11), (8, 5)])
-def test_batch512_full_size(k, ipt):
+@pytest.mark.parametrize("k,variant,ipt,lanes", [(1, "L", 11, 64), (1, "C", 9, 1024), (8, "L", 5, 64)])
+def test_batch512_full_size(k, variant, ipt, lanes):
+    """The 512-matrix batch at full size: L11 and the autotuner's pick for
+    it (C9 in 512-thread tiles, column-sorted) with per-tile carries, and
+    the grouped multi-RHS kernel at k = 8, against the oracle per member."""
     m = g.batch(512, 1000, 200_000, 4.0, 64.0, nrhs=k, seed=1)
     A = spb.Matrix.from_batch(m.row_off, m.col_off, m.ptr, m.col, m.val)
-    print(f"batch512 nnz {m.nnz} k {k} plan L{ipt}", flush=True)
-    p = spb.Plan(A, "L", ipt, nrhs=k, grid=-1)
+    print(f"batch512 nnz {m.nnz} k {k} plan {variant}{ipt} lanes {lanes}", flush=True)
+    p = spb.Plan(A, variant, ipt, nrhs=k, grid=-1, lanes=lanes)
     if k == 1:  # hundreds of tiles per workgroup: the strided persistent launch
         assert p.info().grid == -2
     y = _run_twice(p, m.x, m.nrows * k)

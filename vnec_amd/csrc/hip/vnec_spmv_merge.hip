@@ -479,7 +479,9 @@ static merge_kernel_t pick1(int ipt, bool win, int enc, bool sort)
 {
 #define CASE1(I, W, E) \
     if (ipt == I && win == W && enc == E && !sort) return k_spmv_merge1<I, W, E>; \
-    if (ipt == I && !W && E == 0 && !win && enc == 0 && sort) return k_spmv_merge1<I, false, 0, true>;
+    if (ipt == I && !W && E == 0 && !win && enc == 0 && sort) return k_spmv_merge1<I, false, 0, true>; \
+    if (ipt == I && !W && E == VNEC_ENC_F32_K && !win && enc == E && sort) \
+        return k_spmv_merge1<I, false, VNEC_ENC_F32_K, true>;
 #define ENC3(I, W) CASE1(I, W, 0) CASE1(I, W, 1) CASE1(I, W, 2)
 #define WIN2(I, _) ENC3(I, false) ENC3(I, true) CASE1(I, false, VNEC_ENC_F32_K)
     IPT_LIST(WIN2, 0)

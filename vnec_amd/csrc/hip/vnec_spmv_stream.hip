@@ -405,7 +405,7 @@ static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
 #define TCASE(I, E)                                                           \
     if (ipt == I && enc == E) {                                               \
         if (variant == 0) return k_spmv_strided<I, E, false>;                 \
-        if (variant == 6 && E == 0) return k_spmv_strided<I, 0, true>;        \
+        if (variant == 6 && (E == 0 || E == 3)) return k_spmv_strided<I, E, true>; \
     }
 #define TIPT(I) TCASE(I, 0) TCASE(I, 1) TCASE(I, 2) TCASE(I, 3)
     TIPT(3) TIPT(5) TIPT(7) TIPT(9) TIPT(11) TIPT(13) TIPT(15)

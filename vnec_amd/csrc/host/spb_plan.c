@@ -458,7 +458,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         /* FP32 matrices (the selector's measurements are of the FP64
          * kernels): pattern slices where the pattern rule holds, else the
          * 64-row slice layout when its padding is small, else merge-path
-         * tiles of 7 items per thread with per-tile carries */
+         * tiles of 7 items per thread with per-tile carries, stored in
+         * column order where the column-reuse rule holds */
         vnec_tuning f = {VNEC_VAR_L, 7, 64, VNEC_ENC_FP32, 1, -1};
         if (vnec_pattern_rule(a)) {
             f.variant = VNEC_VAR_G;
@@ -466,6 +467,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         } else if (vnec_slice_padding(a) <= 1.15) {
             f.variant = VNEC_VAR_S;
             f.ipt = 3;
+        } else if (vnec_csort_rule(a)) {
+            f.variant = VNEC_VAR_C;
         }
         tune = f;
     } else if (t && t->variant >= 0) {
@@ -529,11 +532,12 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         return vnec_fail(fp32 ? "an FP32 matrix takes encoding VNEC_ENC_FP32"
                               : "encoding VNEC_ENC_FP32 needs a matrix made with SpB_FP32");
     if (fp32) {
-        if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_S && tune.variant != VNEC_VAR_D &&
-            tune.variant != VNEC_VAR_G)
-            return vnec_fail("encoding VNEC_ENC_FP32: variants L, S, D and G");
+        if (tune.variant != VNEC_VAR_L && tune.variant != VNEC_VAR_C && tune.variant != VNEC_VAR_S &&
+            tune.variant != VNEC_VAR_D && tune.variant != VNEC_VAR_G)
+            return vnec_fail("encoding VNEC_ENC_FP32: variants L, C, S, D and G");
         if (tune.nrhs != 1) return vnec_fail("encoding VNEC_ENC_FP32 takes one right-hand side");
-        if (tune.variant == VNEC_VAR_L && tune.grid >= 0) tune.grid = -1; /* per-tile carries */
+        if ((tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) && tune.grid >= 0)
+            tune.grid = -1; /* per-tile carries */
     }
     const int merge_form = tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C;
     if (merge_form && tune.lanes != 64 && (tune.lanes & ~(VNEC_LANES_RELABEL | VNEC_LANES_WIDE)))
@@ -553,7 +557,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         tune.grid = -1;
     } else if (tune.variant == VNEC_VAR_C) {
         if (!valid_ipt(tune.ipt)) return vnec_fail("ipt %d: use an odd value in [3, 15]", tune.ipt);
-        if (tune.encoding != VNEC_ENC_FP64) return vnec_fail("variant C stores FP64 values only");
+        if (tune.encoding != VNEC_ENC_FP64 && tune.encoding != VNEC_ENC_FP32)
+            return vnec_fail("variant C stores FP64 values (or FP32 for FP32 matrices)");
         if (tune.nrhs != 1) return vnec_fail("variant C takes one right-hand side");
     } else if (tune.variant == VNEC_VAR_H) {
         if (!vnec_hip_hot_supported(tune.ipt, tune.lanes))
@@ -666,8 +671,16 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             if (tune.encoding == VNEC_ENC_HILO)
                 p->lo = (float *)vnec_malloc_dev(sizeof(float) * (size_t)p->nnz, &p->dev_bytes);
             if (!p->hi || (tune.encoding == VNEC_ENC_HILO && !p->lo)) rc = SpB_FAILURE;
-            if (rc == SpB_SUCCESS)
-                rc = vnec_hip_ok(vnec_hip_split_hilo(a->d_val, p->nnz, p->hi, p->lo, s), "split");
+            /* column-sorted tiles (FP32 matrices): the sorted values, whose
+             * FP64 copy is then dropped (FP32 values widen and narrow exactly) */
+            const double *src = tune.variant == VNEC_VAR_C ? p->cs_val : a->d_val;
+            if (rc == SpB_SUCCESS) rc = vnec_hip_ok(vnec_hip_split_hilo(src, p->nnz, p->hi, p->lo, s), "split");
+            if (rc == SpB_SUCCESS && tune.variant == VNEC_VAR_C) {
+                rc = vnec_hip_ok(hipStreamSynchronize(s), "sync");
+                vnec_free_dev(p->cs_val);
+                p->cs_val = NULL;
+                p->dev_bytes -= (int64_t)sizeof(double) * p->nnz;
+            }
         }
         if (rc == SpB_SUCCESS) rc = vnec_hip_ok(hipStreamSynchronize(s), "plan sync");
     }
@@ -706,7 +719,7 @@ static int64_t main_kernel_bytes(const struct vnec_plan_s *p)
     case VNEC_VAR_E:
         return nnz * (vb + 2) + 4 * (nr + 1) + coords + windows + vecs;
     case VNEC_VAR_C:
-        return nnz * (8 + 4 + (p->cs_pos ? 2 : 0)) + 4 * (nr + 1) + coords +
+        return nnz * (vb + 4 + (p->cs_pos ? 2 : 0)) + 4 * (nr + 1) + coords +
                (p->cs_base ? 4 * p->ntiles : 0) + vecs;
     default: /* L, H: CSR with int32 columns */
         return nnz * (vb + 4) + 4 * (nr + 1) + coords + vecs;
