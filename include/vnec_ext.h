@@ -105,7 +105,8 @@ typedef enum {
 typedef enum {
     VNEC_SELECT_LEARNED = -1,   /* the trained MLP over matrix statistics (the
                                    rule below when no model is installed)    */
-    VNEC_SELECT_AUTOTUNE = -2,  /* exhaustive: build and time every candidate */
+    VNEC_SELECT_AUTOTUNE = -2,  /* exhaustive: build and time every candidate
+                                   (FP32 matrices: every FP32 plan shape)    */
     VNEC_SELECT_RULE = -3,      /* S when 64-row slice padding <= 1.15, else
                                    persistent L at 7 items per thread        */
     VNEC_SELECT_REFERENCE = -4, /* the reference dispatch (nrows >= 3e6: L;

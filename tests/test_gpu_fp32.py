@@ -256,3 +256,40 @@ def test_fp32_reference_l_entry_takes_column_sorted_tiles_on_column_reuse():
     assert spb.spmv_vnec("L", y, A, xv, plan) == ""
     assert np.array_equal(y.to_array().view(np.uint32), y1.view(np.uint32))
     spb.free_vnec("L", plan, fp32=True)
+
+
+FP32_SHAPES = {("S", 2), ("S", 3), ("S", 5), ("G", 8), ("L", 5), ("L", 7), ("L", 11), ("C", 5),
+               ("C", 7), ("C", 9), ("C", 11), ("D", 7)}
+
+
+def test_fp32_autotune_times_the_fp32_shapes_and_caches_its_pick():
+    """The autotune mode on FP32 matrices times the FP32 plan shapes on the
+    matrix itself (slices, pattern slices, L and C with per-tile carries, D
+    at width 16); its pick is one of them, cached (a second plan takes the
+    same shape), faster than or as fast as the rule's pick within timing
+    noise, and within the FP32 gate."""
+    for m in (g.stencil27(40, seed=31), g.rmat(16, 16, seed=33), g.banded(200000, 6, seed=34)):
+        ptr, col, val, x = _f32(m)
+        A = spb.Matrix.from_csr(m.nrows, m.ncols, ptr, col, val, dtype=np.float32)
+        p = spb.Plan(A, "autotune")
+        info = p.info()
+        print(f"FP32 autotune {m.name}: {info.variant}{info.ipt} grid {info.grid}", flush=True)
+        assert info.encoding == "fp32" and (info.variant, info.ipt) in FP32_SHAPES
+        q = spb.Plan(A, "autotune")
+        assert (q.info().variant, q.info().ipt, q.info().lanes) == (info.variant, info.ipt, info.lanes)
+        q.free()
+        r = spb.Plan(A, "learned")  # the FP32 rule
+        dx = spb.DeviceArray.from_array(x)
+        dy = spb.DeviceArray(4 * m.nrows)
+        t_auto = min(p.time(dx.ptr, dy.ptr, 20) for _ in range(3))
+        t_rule = min(r.time(dx.ptr, dy.ptr, 20) for _ in range(3))
+        assert t_auto <= 1.10 * t_rule, (info, r.info(), t_auto, t_rule)
+        yv = spb.Vector(m.nrows, np.float32)
+        p.spmv(yv, spb.Vector.from_array(x, np.float32))
+        ok, worst = _gate(yv.to_array(), ptr, col, val, x)
+        assert ok, (info, worst)
+        for h in (p, r):
+            h.free()
+        dx.free()
+        dy.free()
+        A.free()

@@ -158,6 +158,7 @@ SpB_Info SpB_Matrix_new(SpB_Matrix *m, SpB_Type type, SpB_Index nrows, SpB_Index
     a->ptr = (SpB_Index *)calloc((size_t)nrows + 1, sizeof(SpB_Index));
     a->magic = SPB_MAGIC_MATRIX;
     a->tuned_cand = -1;
+    a->tuned_fp32 = -1;
     if (!a->ptr) {
         free(a);
         return vnec_fail("SpB_Matrix_new: out of memory");
@@ -181,6 +182,7 @@ static void matrix_drop_device(spb_csr *a)
     a->d_val = NULL;
     a->dev_ready = 0;
     a->tuned_cand = -1;
+    a->tuned_fp32 = -1;
     a->refined_ipt = 0;
     a->feat_ready = 0;
     a->pattern_ready = 0;

@@ -53,6 +53,7 @@ typedef struct spb_csr {
     double *d_val;      /* nnz, FP64 (FP32 values widened)   */
     struct vnec_plan_s *auto_plan; /* plan cached by SpB_mxv(..., NULL) */
     int tuned_cand;     /* autotuner's best candidate, -1 until measured */
+    int tuned_fp32;     /* FP32 autotuner's best shape, -1 until measured */
     int refined_ipt;    /* merge-path neighbourhood measured at this ipt
                            (0: not yet): the fastest of L / C under
                            both schedules                            */
@@ -216,8 +217,16 @@ SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t);
  * CSR-ordered (L) or column-sorted (C) tiles, persistent (grid 0) or
  * per-tile carries (grid -1) — at t's granularity, and, when x takes at
  * least 2 MiB, the same four with columns relabelled by descending use;
- * sets t's variant, grid and lanes to the fastest.  Cached on the matrix. */
+ * and column-sorted 512-thread tiles at t's granularity (9 above it),
+ * persistent, with per-tile carries, and (x >= 2 MiB) with per-tile
+ * carries over relabelled columns; sets t's variant, ipt, grid and lanes to
+ * the fastest.  Cached on the matrix. */
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
+/* The autotune mode on an FP32 matrix: every FP32 plan shape that applies
+ * (64-row slices, pattern slices, L and C with per-tile carries at several
+ * granularities, D at block width 16) built and timed on the matrix; t is
+ * set to the fastest.  Cached on the matrix. */
+SpB_Info vnec_autotune_fp32(spb_csr *a, vnec_tuning *t);
 /* The structural rules of the learned and rule modes: most rows fit one
  * column-offset pattern (variant G applies), and tiles reuse columns
  * (variant C beats L).  FP64 matrices. */

@@ -454,7 +454,9 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (!vnec_runtime_ready()) return SpB_FAILURE;
     vnec_tuning tune;
     const int fp32 = a->base.element_type == SpB_FP32;
-    if (fp32 && (!t || t->variant < 0)) {
+    if (fp32 && t && t->variant == VNEC_SELECT_AUTOTUNE) {
+        if (vnec_autotune_fp32(a, &tune) != SpB_SUCCESS) return SpB_FAILURE;
+    } else if (fp32 && (!t || t->variant < 0)) {
         /* FP32 matrices (the selector's measurements are of the FP64
          * kernels): pattern slices where the pattern rule holds, else the
          * 64-row slice layout when its padding is small, else merge-path

@@ -660,7 +660,7 @@ static SpB_Info autotune_multi(spb_csr *a, int nrhs, int *ipt)
 
 /* Best of three hipGraph-timed means of `iters` SpMVs of plan tuning u on
  * a (INFINITY when u does not build). */
-static SpB_Info time_tuning(spb_csr *a, const vnec_tuning *u, const double *dx, double *dy, int iters,
+static SpB_Info time_tuning(spb_csr *a, const vnec_tuning *u, const void *dx, void *dy, int iters,
                             double *ms_out)
 {
     *ms_out = INFINITY;
@@ -677,6 +677,60 @@ static SpB_Info time_tuning(spb_csr *a, const vnec_tuning *u, const double *dx, 
     }
     vnec_plan_free_ext(&p);
     return rc;
+}
+
+/* FP32 plan shapes the autotune mode times on an FP32 matrix (the FP64
+ * candidates' times do not carry over: values and x are half as wide). */
+static const vnec_tuning FP32_SHAPES[] = {
+    {VNEC_VAR_S, 2, 64, VNEC_ENC_FP32, 1, -1},  {VNEC_VAR_S, 3, 64, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_S, 5, 64, VNEC_ENC_FP32, 1, -1},  {VNEC_VAR_G, 8, 64, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_L, 5, 64, VNEC_ENC_FP32, 1, -1},  {VNEC_VAR_L, 7, 64, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_L, 11, 64, VNEC_ENC_FP32, 1, -1}, {VNEC_VAR_C, 5, 64, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_C, 7, 64, VNEC_ENC_FP32, 1, -1},  {VNEC_VAR_C, 9, 64, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_C, 11, 64, VNEC_ENC_FP32, 1, -1}, {VNEC_VAR_D, 7, 16, VNEC_ENC_FP32, 1, -1},
+};
+#define N_FP32_SHAPES ((int)(sizeof FP32_SHAPES / sizeof FP32_SHAPES[0]))
+
+SpB_Info vnec_autotune_fp32(spb_csr *a, vnec_tuning *t)
+{
+    if (a->tuned_fp32 >= 0) {
+        *t = FP32_SHAPES[a->tuned_fp32];
+        return SpB_SUCCESS;
+    }
+    uint32_t mask = 0;
+    if (vnec_candidate_mask_ext((SpB_Matrix)a, &mask) != SpB_SUCCESS) return SpB_FAILURE;
+    const int64_t nr = (int64_t)a->base.row, nc = (int64_t)a->base.col, nnz = (int64_t)a->nnz;
+    /* the structural conditions of the FP64 candidates: slices (padding),
+     * pattern slices (the dominant pattern); D blocks of 4-byte values */
+    const int slices = (mask & 7u) != 0, pattern = (mask >> CAND_G) & 1u;
+    const int dblocks = nnz > 0 && (uint64_t)nnz * 4ull * 16ull <= TUNE_MAX_D_BYTES;
+    float *hx = (float *)malloc(sizeof(float) * (size_t)(nc ? nc : 1));
+    float *dx = (float *)vnec_malloc_dev(sizeof(float) * (size_t)(nc ? nc : 1), NULL);
+    float *dy = (float *)vnec_malloc_dev(sizeof(float) * (size_t)(nr ? nr : 1), NULL);
+    SpB_Info rc = (hx && dx && dy) ? SpB_SUCCESS : vnec_fail("out of memory");
+    for (int64_t i = 0; rc == SpB_SUCCESS && i < nc; i++) hx[i] = 1.0f / (float)(1 + (i % 7));
+    if (rc == SpB_SUCCESS && nc)
+        rc = vnec_hip_ok(hipMemcpy(dx, hx, sizeof(float) * (size_t)nc, hipMemcpyHostToDevice), "h2d");
+    double best = INFINITY;
+    int bi = 5; /* L7, which applies to every matrix */
+    for (int c = 0; rc == SpB_SUCCESS && c < N_FP32_SHAPES; c++) {
+        const int v = FP32_SHAPES[c].variant;
+        if ((v == VNEC_VAR_S && !slices) || (v == VNEC_VAR_G && !pattern) || (v == VNEC_VAR_D && !dblocks))
+            continue;
+        double ms = INFINITY;
+        rc = time_tuning(a, &FP32_SHAPES[c], dx, dy, 20, &ms);
+        if (ms < best) {
+            best = ms;
+            bi = c;
+        }
+    }
+    free(hx);
+    vnec_free_dev(dx);
+    vnec_free_dev(dy);
+    if (rc != SpB_SUCCESS) return rc;
+    a->tuned_fp32 = bi;
+    *t = FP32_SHAPES[bi];
+    return SpB_SUCCESS;
 }
 
 SpB_Info vnec_refine_slice_plan(spb_csr *a, vnec_tuning *t)
