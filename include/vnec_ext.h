@@ -139,8 +139,9 @@ typedef struct {
  * in the same order as without it). */
 #define VNEC_LANES_RELABEL 1
 /* vnec_tuning.lanes of a C plan whose tiles hold 512 * ipt items (ipt 3,
- * 5, 7, 9 or 11; FP64, one right-hand side; 512-thread workgroups, from a
- * persistent launch or with per-tile carries) instead of 256 * ipt: each
+ * 5, 7, 9 or 11; one right-hand side; 512-thread workgroups, FP64 from a
+ * persistent launch or with per-tile carries, FP32 with per-tile carries)
+ * instead of 256 * ipt: each
  * tile's nonzeros are sorted by column over the larger window, so the x
  * gathers of a wavefront share more cache lines. */
 #define VNEC_LANES_WIDE 1024

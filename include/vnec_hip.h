@@ -143,11 +143,14 @@ int vnec_hip_spmv_stream(const vnec_merge_args *a, int nwg, int32_t *wg_row, dou
                          void *stream);
 int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, double *y,
                       int64_t nrows, void *stream);
-/* Strided persistent form of the one-workgroup-per-tile kernel (variants L
- * and C, encodings 0-2, one right-hand side): the same per-tile carries in
- * a.carry (for vnec_hip_carry_fixup) and the same results bit for bit;
- * nwg workgroups (a multiple of 8), those on XCD x sweeping x's share of
- * the tiles together.  vnec_hip_strided_grid gives the resident count. */
+/* Strided persistent form of the one-workgroup-per-tile kernel (variant L
+ * in every encoding, variant C in FP64 and the FP32 API's encoding, also in
+ * wide tiles; one right-hand side): the same per-tile carries in a.carry
+ * (for vnec_hip_carry_fixup) and the same results bit for bit; nwg
+ * workgroups (a multiple of 8), those on XCD x sweeping x's share of the
+ * tiles together.  vnec_hip_strided_grid gives the resident count;
+ * vnec_hip_strided_supported whether the kernel exists. */
+int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block);
 int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg);
 int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream);
 /* Variant H: the persistent kernel with the hot x entries in LDS (hot =

@@ -62,6 +62,7 @@ CASES = _cases()
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("variant,ipt,lanes", [("L", 3, 64), ("L", 7, 64), ("L", 15, 64),
                                                 ("C", 3, 64), ("C", 7, 64), ("C", 11, 64),
+                                                ("C", 5, 1024), ("C", 9, 1024),
                                                 ("S", 3, 64), ("D", 5, 16), ("D", 7, 8),
                                                 ("D", 9, 4)])
 def test_fp32_plans_match_the_fp32_serial_oracle(name, m, variant, ipt, lanes):
@@ -185,7 +186,7 @@ def test_fp32_entry_points_refuse_mixed_types():
     with pytest.raises(RuntimeError, match="relabelled columns"):
         spb.Plan(A32, "C", 7, lanes=1)
     with pytest.raises(RuntimeError, match="wide tiles"):
-        spb.Plan(A32, "C", 9, lanes=1024)
+        spb.Plan(A32, "C", 13, lanes=1024)
 
 
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
@@ -266,7 +267,7 @@ def test_fp32_autotune_times_the_fp32_shapes_and_caches_its_pick():
     """The autotune mode on FP32 matrices times the FP32 plan shapes on the
     matrix itself (slices, pattern slices, L and C with per-tile carries, D
     at width 16); its pick is one of them, cached (a second plan takes the
-    same shape), faster than or as fast as the rule's pick within timing
+    same shape; C in 512-thread tiles too), faster than or as fast as the rule's pick within timing
     noise, and within the FP32 gate."""
     for m in (g.stencil27(40, seed=31), g.rmat(16, 16, seed=33), g.banded(200000, 6, seed=34)):
         ptr, col, val, x = _f32(m)
@@ -275,6 +276,7 @@ def test_fp32_autotune_times_the_fp32_shapes_and_caches_its_pick():
         info = p.info()
         print(f"FP32 autotune {m.name}: {info.variant}{info.ipt} grid {info.grid}", flush=True)
         assert info.encoding == "fp32" and (info.variant, info.ipt) in FP32_SHAPES
+        assert info.lanes in (64, 16, spb.LANES_WIDE)
         q = spb.Plan(A, "autotune")
         assert (q.info().variant, q.info().ipt, q.info().lanes) == (info.variant, info.ipt, info.lanes)
         q.free()

@@ -393,10 +393,10 @@ typedef void (*strided_kernel_t)(vnec_merge_args, int);
 
 static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
 {
-    if (block == VNEC_WIDE_BLOCK) {  // wide column-sorted tiles: FP64, C
-        if (variant != 6 || enc != 0) return nullptr;
+    if (block == VNEC_WIDE_BLOCK) {  // wide column-sorted tiles: C, FP64 or the FP32 API
+        if (variant != 6 || (enc != 0 && enc != 3)) return nullptr;
 #define WTCASE(I) \
-    if (ipt == I) return k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK>;
+    if (ipt == I) return enc == 3 ? k_spmv_strided<I, 3, true, VNEC_WIDE_BLOCK> : k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK>;
         WTCASE(3) WTCASE(5) WTCASE(7) WTCASE(9) WTCASE(11)
 #undef WTCASE
         return nullptr;
@@ -412,6 +412,11 @@ static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
     return nullptr;
 #undef TIPT
 #undef TCASE
+}
+
+extern "C" int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block)
+{
+    return pick_strided(variant, ipt, encoding, block) != nullptr;
 }
 
 extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg)

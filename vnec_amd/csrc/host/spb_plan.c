@@ -549,11 +549,14 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
         !(merge_form && tune.encoding == VNEC_ENC_FP64 && tune.nrhs == 1))
         return vnec_fail("relabelled columns (lanes %d): variants L and C, FP64, one right-hand side",
                          VNEC_LANES_RELABEL);
+    /* FP64 wide tiles run persistent or with per-tile carries, FP32 ones
+     * (grid -1 above) with per-tile carries */
     if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_WIDE) &&
         !(tune.variant == VNEC_VAR_C && tune.nrhs == 1 &&
-          vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)))
-        return vnec_fail("wide tiles (lanes %d): variant C, FP64, one right-hand side, ipt 3, 5, 7, 9 or 11",
-                         VNEC_LANES_WIDE);
+          (tune.grid < 0 ? vnec_hip_strided_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)
+                         : vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK))))
+        return vnec_fail("wide tiles (lanes %d): variant C, FP64 or FP32, one right-hand side, ipt 3, 5, 7, 9 "
+                         "or 11", VNEC_LANES_WIDE);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
