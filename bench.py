@@ -258,6 +258,7 @@ def selection_label(spb, A, mode, info, k):
         return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
     flags = info.lanes if info.variant in ("L", "C") and info.lanes != 64 else 0
     form = (f"{info.variant}{info.ipt} grid {info.grid}" + (" 512-thread tiles" if flags & 1024 else "")
+            + (" gathers issued together" if flags & 2048 else "")
             + (" relabelled columns" if flags & 1 else ""))
     return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {form})"
 

@@ -120,7 +120,8 @@ typedef struct {
                      64, or the flags VNEC_LANES_RELABEL (columns
                      relabelled by descending use, x permuted before each
                      SpMV) and, for C, VNEC_LANES_WIDE (512-thread
-                     tiles), alone or or-ed together                     */
+                     tiles) and VNEC_LANES_GATHER (gathers issued
+                     together), alone or or-ed together                  */
     int encoding; /* vnec_encoding                                         */
     int nrhs;     /* right-hand sides per SpMV: 1, 2, 4, 8 or 16           */
     int grid;     /* 0: persistent, one workgroup per resident slot;
@@ -145,6 +146,13 @@ typedef struct {
  * tile's nonzeros are sorted by column over the larger window, so the x
  * gathers of a wavefront share more cache lines. */
 #define VNEC_LANES_WIDE 1024
+/* vnec_tuning.lanes flag of a C plan with per-tile carries (FP64, ipt 3, 5
+ * or 7, one right-hand side; alone or with VNEC_LANES_WIDE): each thread
+ * issues all of a tile's x gathers before using any, instead of waiting for
+ * each in turn.  More loads in flight per wavefront, for more registers:
+ * faster where x sits in the caches (batches of small matrices), slower
+ * where the gathers reach HBM. */
+#define VNEC_LANES_GATHER 2048
 
 typedef struct vnec_plan_s *vnec_plan;
 

@@ -110,6 +110,9 @@ typedef struct {
     const int32_t *cbase;     /* ntiles, or NULL                   */
     int block;                /* threads per tile of the persistent
                                  kernel: 0 (256) or VNEC_WIDE_BLOCK */
+    int gather;               /* strided launch of variant C: 1 when each
+                                 thread issues all of a tile's x gathers
+                                 before using any                  */
 } vnec_merge_args;
 
 /* Threads per tile of the wide column-sorted tiles (variant C, FP64, one
@@ -150,8 +153,8 @@ int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, do
  * workgroups (a multiple of 8), those on XCD x sweeping x's share of the
  * tiles together.  vnec_hip_strided_grid gives the resident count;
  * vnec_hip_strided_supported whether the kernel exists. */
-int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block);
-int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg);
+int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block, int gather);
+int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int gather, int *nwg);
 int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream);
 /* Variant H: the persistent kernel with the hot x entries in LDS (hot =
  * 4096 or 8192 slots at 5 or 7 items per thread; FP64, one right-hand

@@ -792,6 +792,19 @@ def test_wide_column_sorted_tiles(name):
     p = spb.Plan(A, "C", 7, lanes=1024, grid=-1)
     assert p.info().grid == -2  # wide tiles with per-tile carries run the strided launch
     p.free()
+    # gathers issued together (256- and 512-thread tiles): the same tiles and
+    # sums as the plan without the flag, so the same bits
+    for ipt in (3, 5, 7):
+        for lanes in (64, 1024):
+            y0 = _run_plan(A, m.x, "C", ipt, lanes=lanes, grid=-2)
+            flags = (lanes & spb.LANES_WIDE) | spb.LANES_GATHER
+            y1 = _run_plan(A, m.x, "C", ipt, lanes=flags, grid=-1)
+            assert np.array_equal(y0, y1), (ipt, lanes)
+    for bad in [dict(variant="C", ipt=9, lanes=spb.LANES_GATHER, grid=-1),
+                dict(variant="C", ipt=7, lanes=spb.LANES_GATHER, grid=0),
+                dict(variant="L", ipt=7, lanes=spb.LANES_GATHER, grid=-1)]:
+        with pytest.raises(RuntimeError, match="gathers issued together"):
+            spb.Plan(A, bad.pop("variant"), bad.pop("ipt"), **bad)
     for bad in [dict(variant="L", ipt=7, lanes=1024), dict(variant="C", ipt=13, lanes=1024),
                 dict(variant="C", ipt=9, lanes=1024, nrhs=4), dict(variant="C", ipt=9, lanes=3)]:
         with pytest.raises(RuntimeError, match="wide tiles|lanes"):

@@ -204,8 +204,8 @@ SIGNATURES = {
     "vnec_hip_stream_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "vnec_hip_spmv_stream": (C.c_int, [vp, C.c_int, vp, vp, vp]),
     "vnec_hip_wg_fixup": (C.c_int, [vp, vp, C.c_int, vp, i64, vp]),
-    "vnec_hip_strided_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int]),
-    "vnec_hip_strided_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, vp]),
+    "vnec_hip_strided_supported": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "vnec_hip_strided_grid": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp]),
     "vnec_hip_spmv_strided": (C.c_int, [vp, C.c_int, vp]),
     "vnec_hip_stream_copy": (C.c_int, [vp, vp, C.c_size_t, vp]),
     "vnec_hip_stream_read": (C.c_int, [vp, C.c_size_t, vp, vp]),

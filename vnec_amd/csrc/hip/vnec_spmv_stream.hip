@@ -288,7 +288,10 @@ __global__ __launch_bounds__(BS) void k_spmv_stream(vnec_merge_args a, int nwg,
 // flight on one XCD are neighbours and share its L2, while each workgroup
 // issues its next tile's loads before reducing the current one.
 // BS: threads per tile (256, or VNEC_WIDE_BLOCK for wide column-sorted tiles).
-template <int IPT, int ENC, bool SORT, int BS = VNEC_BLOCK>
+// GB: each thread issues all of its x gathers of a tile before using any
+// (more loads in flight per wavefront for more registers; the default waits
+// for each gather in turn).
+template <int IPT, int ENC, bool SORT, int BS = VNEC_BLOCK, bool GB = false>
 __global__ __launch_bounds__(BS) void k_spmv_strided(vnec_merge_args a, int nwg)
 {
     typedef typename enc_real<ENC>::type F;  // FP32 for the FP32 API's arithmetic
@@ -327,20 +330,49 @@ __global__ __launch_bounds__(BS) void k_spmv_strided(vnec_merge_args a, int nwg)
         const int nrow = A1.x - A0.x, nz = A1.y - A0.y;
 
         // --- row ends and products into LDS
+        if (GB) {
+            // the thread's gathers first (an item past the tile reads x[0],
+            // present whenever the tile holds a nonzero, and is not used)
+            F xv[IPT];
+            int pv[IPT];
+            if (nz > 0) {
 #pragma unroll
-        for (int u = 0; u < IPT; u++) {
-            const int k = u * BS + tid;
-            if (k < nrow) s_rowend[k] = cur.re[u];
-            if (k < nz) {
-                int c = cur.c[u], pos = k;
-                if (SORT) {
-                    pos = cur.p[u];
-                    if (a.cpos_shift > 0) {  // position packed above the column
-                        pos = (int)((uint32_t)c >> a.cpos_shift);
-                        c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
+                for (int u = 0; u < IPT; u++) {
+                    const int k = u * BS + tid;
+                    int c = cur.c[u], pos = k;
+                    if (SORT) {
+                        pos = cur.p[u];
+                        if (a.cpos_shift > 0) {  // position packed above the column
+                            pos = (int)((uint32_t)c >> a.cpos_shift);
+                            c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
+                        }
                     }
+                    pv[u] = pos;
+                    xv[u] = x[k < nz ? c : 0];
                 }
-                s_prod[pos] = widen(cur.v[u]) * x[c];
+            }
+#pragma unroll
+            for (int u = 0; u < IPT; u++) {
+                const int k = u * BS + tid;
+                if (k < nrow) s_rowend[k] = cur.re[u];
+                if (k < nz) s_prod[pv[u]] = widen(cur.v[u]) * xv[u];
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < IPT; u++) {
+                const int k = u * BS + tid;
+                if (k < nrow) s_rowend[k] = cur.re[u];
+                if (k < nz) {
+                    int c = cur.c[u], pos = k;
+                    if (SORT) {
+                        pos = cur.p[u];
+                        if (a.cpos_shift > 0) {  // position packed above the column
+                            pos = (int)((uint32_t)c >> a.cpos_shift);
+                            c = (c & ((1 << a.cpos_shift) - 1)) + cur.cb;
+                        }
+                    }
+                    s_prod[pos] = widen(cur.v[u]) * x[c];
+                }
             }
         }
         __syncthreads();
@@ -391,8 +423,18 @@ static int block_threads(int block) { return block == VNEC_WIDE_BLOCK ? VNEC_WID
 
 typedef void (*strided_kernel_t)(vnec_merge_args, int);
 
-static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
+static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block, int gather)
 {
+    if (gather) {  // gathers issued together: FP64 column-sorted tiles, 3 to 7 items per thread
+        if (variant != 6 || enc != 0) return nullptr;
+#define GCASE(I)                                                                         \
+    if (ipt == I) return block == VNEC_WIDE_BLOCK ? k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK, true> \
+                                                  : k_spmv_strided<I, 0, true, VNEC_BLOCK, true>;
+        if (block != 0 && block != VNEC_BLOCK && block != VNEC_WIDE_BLOCK) return nullptr;
+        GCASE(3) GCASE(5) GCASE(7)
+#undef GCASE
+        return nullptr;
+    }
     if (block == VNEC_WIDE_BLOCK) {  // wide column-sorted tiles: C, FP64 or the FP32 API
         if (variant != 6 || (enc != 0 && enc != 3)) return nullptr;
 #define WTCASE(I) \
@@ -414,14 +456,14 @@ static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block)
 #undef TCASE
 }
 
-extern "C" int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block)
+extern "C" int vnec_hip_strided_supported(int variant, int ipt, int encoding, int block, int gather)
 {
-    return pick_strided(variant, ipt, encoding, block) != nullptr;
+    return pick_strided(variant, ipt, encoding, block, gather) != nullptr;
 }
 
-extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int *nwg)
+extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int block, int gather, int *nwg)
 {
-    strided_kernel_t k = pick_strided(variant, ipt, encoding, block);
+    strided_kernel_t k = pick_strided(variant, ipt, encoding, block, gather);
     if (!k) return (int)hipErrorInvalidValue;
     int per_cu = 0, dev = 0, cus = 0;
     hipError_t e =
@@ -439,7 +481,7 @@ extern "C" int vnec_hip_strided_grid(int variant, int ipt, int encoding, int blo
 
 extern "C" int vnec_hip_spmv_strided(const vnec_merge_args *a, int nwg, void *stream)
 {
-    strided_kernel_t k = pick_strided(a->variant, a->ipt, a->encoding, a->block);
+    strided_kernel_t k = pick_strided(a->variant, a->ipt, a->encoding, a->block, a->gather);
     if (!k || a->nrhs != 1 || nwg < VNEC_XCDS || nwg % VNEC_XCDS) return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
     hipLaunchKernelGGL(k, dim3((unsigned)nwg), dim3((unsigned)block_threads(a->block)), 0, (hipStream_t)stream,

@@ -217,10 +217,11 @@ SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t);
  * CSR-ordered (L) or column-sorted (C) tiles, persistent (grid 0) or
  * per-tile carries (grid -1) — at t's granularity, and, when x takes at
  * least 2 MiB, the same four with columns relabelled by descending use;
- * and column-sorted 512-thread tiles at t's granularity (9 above it),
+ * column-sorted 512-thread tiles at t's granularity (9 above it),
  * persistent, with per-tile carries, and (x >= 2 MiB) with per-tile
- * carries over relabelled columns; sets t's variant, ipt, grid and lanes to
- * the fastest.  Cached on the matrix. */
+ * carries over relabelled columns; and 512-thread tiles with their gathers
+ * issued together at t's granularity (7 above it); sets t's variant, ipt,
+ * grid and lanes to the fastest.  Cached on the matrix. */
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
 /* The autotune mode on an FP32 matrix: every FP32 plan shape that applies
  * (64-row slices, pattern slices, L and C with per-tile carries at several

@@ -22,6 +22,10 @@ static int plan_block(const vnec_tuning *t)
     return t->variant == VNEC_VAR_C && t->lanes != 64 && (t->lanes & VNEC_LANES_WIDE) ? VNEC_WIDE_BLOCK : 0;
 }
 static int plan_threads(const vnec_tuning *t) { return plan_block(t) ? VNEC_WIDE_BLOCK : TILE_THREADS; }
+static int plan_gather(const vnec_tuning *t)
+{
+    return t->variant == VNEC_VAR_C && t->lanes != 64 && (t->lanes & VNEC_LANES_GATHER) ? 1 : 0;
+}
 
 static int valid_ipt(int ipt) { return ipt >= 3 && ipt <= 15 && (ipt & 1); }
 /* Tiles per resident workgroup from which per-tile-carry plans use the
@@ -542,9 +546,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             tune.grid = -1; /* per-tile carries */
     }
     const int merge_form = tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C;
-    if (merge_form && tune.lanes != 64 && (tune.lanes & ~(VNEC_LANES_RELABEL | VNEC_LANES_WIDE)))
-        return vnec_fail("lanes %d: variants L and C take 64 or the flags VNEC_LANES_RELABEL | VNEC_LANES_WIDE",
-                         tune.lanes);
+    if (merge_form && tune.lanes != 64 &&
+        (tune.lanes & ~(VNEC_LANES_RELABEL | VNEC_LANES_WIDE | VNEC_LANES_GATHER)))
+        return vnec_fail("lanes %d: variants L and C take 64 or the flags VNEC_LANES_RELABEL | VNEC_LANES_WIDE | "
+                         "VNEC_LANES_GATHER", tune.lanes);
     if ((tune.lanes == VNEC_LANES_RELABEL || (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_RELABEL))) &&
         !(merge_form && tune.encoding == VNEC_ENC_FP64 && tune.nrhs == 1))
         return vnec_fail("relabelled columns (lanes %d): variants L and C, FP64, one right-hand side",
@@ -553,10 +558,15 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
      * (grid -1 above) with per-tile carries */
     if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_WIDE) &&
         !(tune.variant == VNEC_VAR_C && tune.nrhs == 1 &&
-          (tune.grid < 0 ? vnec_hip_strided_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK)
+          (tune.grid < 0 ? vnec_hip_strided_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK, 0)
                          : vnec_hip_stream_supported(tune.variant, tune.ipt, tune.encoding, VNEC_WIDE_BLOCK))))
         return vnec_fail("wide tiles (lanes %d): variant C, FP64 or FP32, one right-hand side, ipt 3, 5, 7, 9 "
                          "or 11", VNEC_LANES_WIDE);
+    if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_GATHER) &&
+        !(tune.variant == VNEC_VAR_C && tune.nrhs == 1 && tune.grid < 0 &&
+          vnec_hip_strided_supported(tune.variant, tune.ipt, tune.encoding, plan_block(&tune), 1)))
+        return vnec_fail("gathers issued together (lanes %d): variant C, FP64, one right-hand side, ipt 3, 5 "
+                         "or 7, per-tile carries (grid -1 or -2)", VNEC_LANES_GATHER);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;
@@ -642,8 +652,10 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
             int snwg = 0;
             if (rc == SpB_SUCCESS && tune.nrhs == 1 &&
                 (tune.variant == VNEC_VAR_L || tune.variant == VNEC_VAR_C) &&
-                vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, plan_block(&tune), &snwg) == 0 &&
-                (tune.grid == -2 || plan_block(&tune) || p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
+                vnec_hip_strided_grid(tune.variant, tune.ipt, tune.encoding, plan_block(&tune), plan_gather(&tune),
+                                      &snwg) == 0 &&
+                (tune.grid == -2 || plan_block(&tune) || plan_gather(&tune) ||
+                 p->ntiles >= STRIDED_MIN_TILES_PER_WG * (int64_t)snwg)) {
                 p->snwg = snwg;
                 p->tune.grid = -2;
             }
@@ -845,6 +857,7 @@ static void fill_args(const struct vnec_plan_s *p, const void *x_dev, void *y_de
         m.cpos_shift = p->cs_shift;
         m.cbase = p->cs_base;
         m.block = plan_block(&p->tune);
+        m.gather = plan_gather(&p->tune);
     }
     if (p->tune.variant == VNEC_VAR_H) {
         m.col = p->hcol;

@@ -22,7 +22,8 @@
  * workgroup per tile) */
 static const vnec_tuning CANDIDATES[VNEC_NCAND] = {
     {VNEC_VAR_S, 2, 64, 0, 1, -1},  {VNEC_VAR_S, 3, 64, 0, 1, -1},  {VNEC_VAR_S, 5, 64, 0, 1, -1},
-    {VNEC_VAR_L, 5, 64, 0, 1, 0},   {VNEC_VAR_L, 7, 64, 0, 1, 0},   {VNEC_VAR_L, 9, 64, 0, 1, 0},
+    {VNEC_VAR_L, 5, 64, 0, 1, 0},   {VNEC_VAR_L, 7, 64, 0, 1, 0},
+    {VNEC_VAR_C, 7, VNEC_LANES_WIDE | VNEC_LANES_GATHER, 0, 1, -1},
     {VNEC_VAR_L, 11, 64, 0, 1, 0},  {VNEC_VAR_L, 5, 64, 0, 1, -1},  {VNEC_VAR_L, 7, 64, 0, 1, -1},
     {VNEC_VAR_L, 11, 64, 0, 1, -1}, {VNEC_VAR_E, 5, 64, 0, 1, -1},  {VNEC_VAR_E, 7, 64, 0, 1, -1},
     {VNEC_VAR_E, 5, 64, 0, 1, 0},   {VNEC_VAR_E, 7, 64, 0, 1, 0},
@@ -815,9 +816,12 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
     /* L or C tiles, persistent or with per-tile carries, at the pick's
      * granularity; the same over hot-first relabelled columns when x is
      * large; column-sorted 512-thread tiles, persistent or with per-tile
-     * carries, and over relabelled columns with per-tile carries */
+     * carries, and over relabelled columns with per-tile carries; and those
+     * tiles with their gathers issued together, at 7 items per thread or
+     * fewer (more registers per thread above) */
     const int relabel = 8 * nc >= RELABEL_MIN_X_BYTES, wipt = wide_ipt(t->ipt);
-    vnec_tuning forms[11];
+    const int gipt = t->ipt < 7 ? t->ipt : 7;
+    vnec_tuning forms[12];
     int nf = 0;
     for (int k = 0; k < (relabel ? 8 : 4); k++) {
         vnec_tuning u = *t;
@@ -832,6 +836,14 @@ SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t)
         u.ipt = wipt;
         u.grid = k == 0 ? 0 : -1;
         u.lanes = k == 2 ? (VNEC_LANES_WIDE | VNEC_LANES_RELABEL) : VNEC_LANES_WIDE;
+        forms[nf++] = u;
+    }
+    if (vnec_hip_strided_supported(VNEC_VAR_C, gipt, VNEC_ENC_FP64, VNEC_WIDE_BLOCK, 1)) {
+        vnec_tuning u = *t;
+        u.variant = VNEC_VAR_C;
+        u.ipt = gipt;
+        u.grid = -1;
+        u.lanes = VNEC_LANES_WIDE | VNEC_LANES_GATHER;
         forms[nf++] = u;
     }
     double best = INFINITY;

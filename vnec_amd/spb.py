@@ -234,13 +234,14 @@ class TuningInfo:
 # vnec_tuning.lanes flags of L and C plans (include/vnec_ext.h)
 LANES_RELABEL = 1  # columns relabelled hot-first, x permuted before each SpMV
 LANES_WIDE = 1024  # C only: 512-thread tiles of 512 * ipt items
+LANES_GATHER = 2048  # C only, per-tile carries: a thread's x gathers of a tile issued together
 
 
 class Plan:
     """GPU execution plan (vnec_plan_build_ext).  variant: a variant letter
     or a selection mode ("learned", "autotune", "rule", "reference");
     lanes: the D block width, H's hot entries, or for L and C 64 or the
-    flags LANES_RELABEL | LANES_WIDE."""
+    flags LANES_RELABEL | LANES_WIDE | LANES_GATHER."""
 
     def __init__(self, A: Matrix, variant: str | None = "L", ipt: int = 7, lanes: int = 64,
                  encoding: str | None = None, nrhs: int = 1, grid: int = 0):
