@@ -148,7 +148,9 @@ int vnec_hip_wg_fixup(const int32_t *wg_row, const double *wg_carry, int nwg, do
                       int64_t nrows, void *stream);
 /* Strided persistent form of the one-workgroup-per-tile kernel (variant L
  * in every encoding, variant C in FP64 and the FP32 API's encoding, also in
- * wide tiles; one right-hand side): the same per-tile carries in a.carry
+ * wide tiles, and for C in FP64 at ipt 3, 5 or 7 with each thread's gathers
+ * issued together when gather = 1 (a.gather); one right-hand side): the
+ * same per-tile carries in a.carry
  * (for vnec_hip_carry_fixup) and the same results bit for bit; nwg
  * workgroups (a multiple of 8), those on XCD x sweeping x's share of the
  * tiles together.  vnec_hip_strided_grid gives the resident count;
