@@ -146,8 +146,8 @@ typedef struct {
  * tile's nonzeros are sorted by column over the larger window, so the x
  * gathers of a wavefront share more cache lines. */
 #define VNEC_LANES_WIDE 1024
-/* vnec_tuning.lanes flag of a C plan with per-tile carries (FP64, ipt 3, 5
- * or 7, one right-hand side; alone or with VNEC_LANES_WIDE): each thread
+/* vnec_tuning.lanes flag of a C plan with per-tile carries (FP64 or FP32,
+ * ipt 3, 5 or 7, one right-hand side; alone or with VNEC_LANES_WIDE): each thread
  * issues all of a tile's x gathers before using any, instead of waiting for
  * each in turn.  More loads in flight per wavefront, for more registers:
  * faster where x sits in the caches (batches of small matrices), slower

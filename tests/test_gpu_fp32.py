@@ -62,7 +62,8 @@ CASES = _cases()
 @pytest.mark.parametrize("name,m", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("variant,ipt,lanes", [("L", 3, 64), ("L", 7, 64), ("L", 15, 64),
                                                 ("C", 3, 64), ("C", 7, 64), ("C", 11, 64),
-                                                ("C", 5, 1024), ("C", 9, 1024),
+                                                ("C", 5, 1024), ("C", 9, 1024), ("C", 5, 2048),
+                                                ("C", 7, 3072),
                                                 ("S", 3, 64), ("D", 5, 16), ("D", 7, 8),
                                                 ("D", 9, 4)])
 def test_fp32_plans_match_the_fp32_serial_oracle(name, m, variant, ipt, lanes):
@@ -276,7 +277,7 @@ def test_fp32_autotune_times_the_fp32_shapes_and_caches_its_pick():
         info = p.info()
         print(f"FP32 autotune {m.name}: {info.variant}{info.ipt} grid {info.grid}", flush=True)
         assert info.encoding == "fp32" and (info.variant, info.ipt) in FP32_SHAPES
-        assert info.lanes in (64, 16, spb.LANES_WIDE)
+        assert info.lanes in (64, 16, spb.LANES_WIDE, spb.LANES_WIDE | spb.LANES_GATHER)
         q = spb.Plan(A, "autotune")
         assert (q.info().variant, q.info().ipt, q.info().lanes) == (info.variant, info.ipt, info.lanes)
         q.free()

@@ -425,11 +425,15 @@ typedef void (*strided_kernel_t)(vnec_merge_args, int);
 
 static strided_kernel_t pick_strided(int variant, int ipt, int enc, int block, int gather)
 {
-    if (gather) {  // gathers issued together: FP64 column-sorted tiles, 3 to 7 items per thread
-        if (variant != 6 || enc != 0) return nullptr;
-#define GCASE(I)                                                                         \
-    if (ipt == I) return block == VNEC_WIDE_BLOCK ? k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK, true> \
-                                                  : k_spmv_strided<I, 0, true, VNEC_BLOCK, true>;
+    if (gather) {  // gathers issued together: column-sorted tiles, 3 to 7 items per thread
+        if (variant != 6 || (enc != 0 && enc != 3)) return nullptr;
+#define GCASE(I)                                                                                     \
+    if (ipt == I && enc == 0)                                                                        \
+        return block == VNEC_WIDE_BLOCK ? k_spmv_strided<I, 0, true, VNEC_WIDE_BLOCK, true>          \
+                                        : k_spmv_strided<I, 0, true, VNEC_BLOCK, true>;              \
+    if (ipt == I && enc == 3)                                                                        \
+        return block == VNEC_WIDE_BLOCK ? k_spmv_strided<I, 3, true, VNEC_WIDE_BLOCK, true>          \
+                                        : k_spmv_strided<I, 3, true, VNEC_BLOCK, true>;
         if (block != 0 && block != VNEC_BLOCK && block != VNEC_WIDE_BLOCK) return nullptr;
         GCASE(3) GCASE(5) GCASE(7)
 #undef GCASE

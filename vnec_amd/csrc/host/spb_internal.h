@@ -225,8 +225,9 @@ SpB_Info vnec_select_tuning_no_pattern(spb_csr *a, vnec_tuning *t);
 SpB_Info vnec_refine_merge_plan(spb_csr *a, vnec_tuning *t);
 /* The autotune mode on an FP32 matrix: every FP32 plan shape that applies
  * (64-row slices, pattern slices, L and C with per-tile carries at several
- * granularities, C also in 512-thread tiles, D at block width 16) built and
- * timed on the matrix; t is set to the fastest.  Cached on the matrix. */
+ * granularities, C also in 512-thread tiles, with and without gathers issued
+ * together, D at block width 16) built and timed on the matrix; t is set to
+ * the fastest.  Cached on the matrix. */
 SpB_Info vnec_autotune_fp32(spb_csr *a, vnec_tuning *t);
 /* The structural rules of the learned and rule modes: most rows fit one
  * column-offset pattern (variant G applies), and tiles reuse columns

@@ -565,8 +565,8 @@ SpB_Info vnec_plan_build_ext(vnec_plan *plan, const SpB_Matrix A, const vnec_tun
     if (merge_form && tune.lanes != 64 && (tune.lanes & VNEC_LANES_GATHER) &&
         !(tune.variant == VNEC_VAR_C && tune.nrhs == 1 && tune.grid < 0 &&
           vnec_hip_strided_supported(tune.variant, tune.ipt, tune.encoding, plan_block(&tune), 1)))
-        return vnec_fail("gathers issued together (lanes %d): variant C, FP64, one right-hand side, ipt 3, 5 "
-                         "or 7, per-tile carries (grid -1 or -2)", VNEC_LANES_GATHER);
+        return vnec_fail("gathers issued together (lanes %d): variant C, FP64 or FP32, one right-hand side, "
+                         "ipt 3, 5 or 7, per-tile carries (grid -1 or -2)", VNEC_LANES_GATHER);
     if (tune.variant == VNEC_VAR_G) {
         if (tune.nrhs != 1) return vnec_fail("variant G takes one right-hand side");
         tune.grid = -1;

@@ -691,6 +691,8 @@ static const vnec_tuning FP32_SHAPES[] = {
     {VNEC_VAR_C, 11, 64, VNEC_ENC_FP32, 1, -1}, {VNEC_VAR_D, 7, 16, VNEC_ENC_FP32, 1, -1},
     {VNEC_VAR_C, 9, VNEC_LANES_WIDE, VNEC_ENC_FP32, 1, -1},
     {VNEC_VAR_C, 11, VNEC_LANES_WIDE, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_C, 5, VNEC_LANES_WIDE | VNEC_LANES_GATHER, VNEC_ENC_FP32, 1, -1},
+    {VNEC_VAR_C, 7, VNEC_LANES_WIDE | VNEC_LANES_GATHER, VNEC_ENC_FP32, 1, -1},
 };
 #define N_FP32_SHAPES ((int)(sizeof FP32_SHAPES / sizeof FP32_SHAPES[0]))
 
