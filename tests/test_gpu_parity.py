@@ -810,3 +810,21 @@ def test_wide_column_sorted_tiles(name):
         with pytest.raises(RuntimeError, match="wide tiles|lanes"):
             spb.Plan(A, bad.pop("variant"), bad.pop("ipt"), **bad)
     A.free()
+
+
+@pytest.mark.parametrize("lanes", [spb.LANES_GATHER, spb.LANES_WIDE | spb.LANES_GATHER])
+def test_gathered_tiles_on_rows_without_nonzeros(lanes):
+    """Column-sorted tiles with gathers issued together on a matrix whose
+    tiles hold no nonzero (every row empty): the kernel skips the gathers of
+    such tiles and y is all zeros; a learned plan on the same matrix gives
+    the same."""
+    for ncols in (5,):
+        n = 7000
+        A = spb.Matrix.from_csr(n, ncols, np.zeros(n + 1, np.int64), np.zeros(0, np.int32), np.zeros(0))
+        for p in (spb.Plan(A, "C", 7, lanes=lanes, grid=-1), spb.Plan(A, "learned")):
+            xv = spb.Vector.from_array(np.ones(ncols))
+            yv = spb.Vector.from_array(np.full(n, 3.0))
+            p.spmv(yv, xv)
+            assert np.array_equal(yv.to_array(), np.zeros(n))
+            p.free()
+        A.free()
