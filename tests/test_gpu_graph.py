@@ -18,7 +18,8 @@ SHAPES = [("L", 7, 0, 1, "fp64", 64), ("L", 11, -1, 1, "fp64", 64), ("C", 9, 0, 
           ("C", 7, -2, 1, "fp64", 64), ("E", 5, 0, 1, "fp64", 64), ("S", 3, -1, 1, "fp64", 64),
           ("S", 3, -1, 1, "hilo", 64), ("D", 7, -1, 1, "fp64", 8), ("H", 5, 0, 1, "fp64", 4096),
           ("L", 5, -1, 8, "fp64", 64), ("L", 7, 0, 1, "hi", 64),
-          ("C", 9, -2, 1, "fp64", 1), ("C", 7, 0, 1, "fp64", 1024)]  # relabelled; wide tiles
+          ("C", 9, -2, 1, "fp64", 1), ("C", 7, 0, 1, "fp64", 1024),  # relabelled; wide tiles
+          ("C", 7, -1, 1, "fp64", 3072)]  # gathers issued together
 
 
 def _sync():
