@@ -393,9 +393,10 @@ SpB_VNEC_type IRD_VNEC(const SpB_Matrix A, SpB_Type type)
 
 /* ------------------------------------------------------ public plans */
 /* The device plan behind a Build_VNEC_* result runs the matching GPU
- * kernel: VNEC-L the merge-path CSR kernel (7 items per thread,
- * persistent for FP64, per-tile carries for FP32; tiles in column order
- * where the column-reuse rule holds), VNEC-D the wave64
+ * kernel: VNEC-L a merge-path plan (for FP64 the learned selector's pick
+ * when it is one, else 7 items per thread, persistent for FP64, per-tile
+ * carries for FP32; tiles in column order where the column-reuse rule
+ * holds), VNEC-D the wave64
  * D-block kernel (groups of the reference's nLanes window positions: 8 for
  * FP64, 16 for FP32), VNEC-S the 64-row slice kernel when the slice layout
  * applies (padding <= 3) and otherwise the learned selector's choice
@@ -407,8 +408,18 @@ static struct vnec_plan_s *gpu_plan_for(spb_csr *a, plan_kind kind)
     vnec_tuning t = {VNEC_SELECT_LEARNED, 7, 64, fp32 ? VNEC_ENC_FP32 : VNEC_ENC_FP64, 1, -1};
     if (fp32) t.variant = VNEC_VAR_L; /* grid -1: one workgroup per tile */
     if (kind == KIND_L) {
-        /* merge-path CSR; tiles stored in column order where columns are
+        /* merge-path CSR: for FP64 the learned selector's pick when it is a
+         * merge-path plan (CSR-ordered or column-sorted tiles, and beyond the
+         * model's training sizes the fastest of the forms measured around
+         * it); otherwise tiles stored in column order where columns are
          * reused (the same algorithm, fewer lines per gather) */
+        vnec_tuning sel;
+        if (!fp32 && vnec_select_tuning(a, VNEC_SELECT_LEARNED, 1, &sel) == SpB_SUCCESS &&
+            (sel.variant == VNEC_VAR_L || sel.variant == VNEC_VAR_C)) {
+            vnec_tuning lt = {VNEC_SELECT_LEARNED, 7, 64, VNEC_ENC_FP64, 1, -1};
+            if (vnec_plan_build_ext(&p, (SpB_Matrix)a, &lt) != SpB_SUCCESS) return NULL;
+            return p;
+        }
         t.variant = vnec_csort_rule(a) ? VNEC_VAR_C : VNEC_VAR_L;
         t.grid = fp32 ? -1 : 0;
     } else if (vnec_pattern_rule(a)) {
