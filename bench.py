@@ -254,12 +254,12 @@ def selection_label(spb, A, mode, info, k):
     # persistent picks report their workgroup count; per-tile-carry picks (-1)
     # run the strided launch (-2) by themselves once the tiles fill the GPU
     same_grid = c.grid == info.grid or (c.grid >= 0 and info.grid >= 0) or (c.grid < 0 and info.grid < 0)
-    if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
-        return f"learned (MLP pick: candidate {c.variant}{c.ipt})"
     flags = info.lanes if info.variant in ("L", "C") and info.lanes != 64 else 0
-    form = (f"{info.variant}{info.ipt} grid {info.grid}" + (" 512-thread tiles" if flags & 1024 else "")
-            + (" gathers issued together" if flags & 2048 else "")
-            + (" relabelled columns" if flags & 1 else ""))
+    extra = ((" 512-thread tiles" if flags & 1024 else "") + (" gathers issued together" if flags & 2048 else "")
+             + (" relabelled columns" if flags & 1 else ""))
+    if (c.variant, c.ipt, c.lanes) == (info.variant, info.ipt, info.lanes) and same_grid:
+        return f"learned (MLP pick: candidate {c.variant}{c.ipt}{extra})"
+    form = f"{info.variant}{info.ipt} grid {info.grid}{extra}"
     return f"learned (MLP pick {c.variant}{c.ipt}, measured neighbour kept: {form})"
 
 
