@@ -316,3 +316,36 @@ def test_noalloc_spmv_checks_sizes_before_any_device_work():
     assert "NULL array" in L.last_error()
     # an empty matrix is a no-op
     assert lib.vnec_csr_spmv_noalloc_ext(0, 0, 0, None, None, None, None, None, None) == 0
+
+
+def test_python_constants_match_the_headers():
+    """The Python mirror's constants (lanes flags, candidate and feature
+    counts, variants, encodings, the wide block) equal the C headers' own."""
+    txt = ""
+    for h in ("vnec_ext.h", "vnec_hip.h"):
+        txt += open(os.path.join(ROOT, "include", h)).read()
+
+    def define(name):
+        m = re.search(r"#define\s+" + name + r"\s+(\d+)", txt)
+        assert m, name
+        return int(m.group(1))
+
+    def enum(name):
+        m = re.search(r"\b" + name + r"\s*=\s*(-?\d+)", txt)
+        assert m, name
+        return int(m.group(1))
+
+    assert spb.LANES_RELABEL == define("VNEC_LANES_RELABEL")
+    assert spb.LANES_WIDE == define("VNEC_LANES_WIDE")
+    assert spb.LANES_GATHER == define("VNEC_LANES_GATHER")
+    assert L.NCAND == define("VNEC_NCAND")
+    assert L.NFEAT_INTS == define("VNEC_NFEAT_INTS")
+    for letter, name in (("L", "VNEC_VAR_L"), ("E", "VNEC_VAR_E"), ("S", "VNEC_VAR_S"), ("D", "VNEC_VAR_D"),
+                         ("H", "VNEC_VAR_H"), ("G", "VNEC_VAR_G"), ("C", "VNEC_VAR_C")):
+        assert spb.VNEC_VAR[letter] == enum(name), letter
+    for enc, name in (("fp64", "VNEC_ENC_FP64"), ("hilo", "VNEC_ENC_HILO"), ("hi", "VNEC_ENC_HI"),
+                      ("fp32", "VNEC_ENC_FP32")):
+        assert spb.VNEC_ENC[enc] == enum(name), enc
+    for mode, name in (("learned", "VNEC_SELECT_LEARNED"), ("autotune", "VNEC_SELECT_AUTOTUNE"),
+                       ("rule", "VNEC_SELECT_RULE"), ("reference", "VNEC_SELECT_REFERENCE")):
+        assert spb.SELECT[mode] == enum(name), mode
